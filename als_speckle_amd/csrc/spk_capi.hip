@@ -1,0 +1,733 @@
+// C ABI of the library (declared in include/speckle_hip.h).  Host code only:
+// argument checking, plan state, and the launch sequence of an iteration.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/speckle_hip.h"
+#include "spk_internal.hpp"
+
+using namespace spk;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define SPK_HIP(call)                                                        \
+  do {                                                                       \
+    hipError_t e_ = (call);                                                  \
+    if (e_ != hipSuccess)                                                    \
+      return fail(SPK_EHIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+#define SPK_REQUIRE(cond, code, msg) \
+  do {                               \
+    if (!(cond)) return fail(code, msg); \
+  } while (0)
+
+bool is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
+
+int default_tile(int n) { return n >= 4096 ? 4 : 8; }
+
+template <class T>
+int dalloc(T** p, size_t count) {
+  *p = nullptr;
+  if (count == 0) return SPK_OK;
+  hipError_t e = hipMalloc((void**)p, count * sizeof(T));
+  if (e != hipSuccess) {
+    *p = nullptr;
+    (void)hipGetLastError();
+    return fail(SPK_ENOMEM, "hipMalloc of " + std::to_string(count * sizeof(T)) +
+                                " bytes failed: " + hipGetErrorString(e));
+  }
+  return SPK_OK;
+}
+
+template <class T>
+void dfree(T*& p) {
+  if (p) (void)hipFree((void*)p);
+  p = nullptr;
+}
+
+}  // namespace
+
+struct spk_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  double* scratch = nullptr;   // small reduction workspace (align)
+  size_t scratch_cap = 0;
+};
+
+struct spk_plan {
+  spk_ctx* ctx = nullptr;
+  int n = 0, batch = 0, shared = 0, tile = 8, bands = 0;
+  c32* psi = nullptr;
+  c32* phi = nullptr;
+  c32* phib = nullptr;
+  float* mod = nullptr;
+  float* sup = nullptr;
+  c32* tw = nullptr;
+  double* summ2 = nullptr;
+  double* sumpart = nullptr;
+  double* errp = nullptr;
+  size_t errp_cap = 0;        // doubles
+  float* sw_tmp = nullptr;
+  float* sw_blur = nullptr;
+  float* sw_maxp = nullptr;
+  c32* fftbuf = nullptr;
+  size_t fftbuf_frames = 0;
+  Coef* dtab = nullptr;       // per-frame coefficient table [nsteps][batch]
+  int* dact = nullptr;        // per-frame activity [nsteps][batch]
+  size_t dtab_cap = 0;        // entries
+  Coef* htab = nullptr;       // pinned staging for dtab
+  int* hact = nullptr;
+  hipEvent_t upload_done = nullptr;
+  bool profiling = false;
+  std::vector<hipEvent_t> prof_ev[2];  // start/stop pairs per pass
+  bool mod_set = false, sup_set = false, psi_set = false;
+  bool phi_valid = false;     // phi == rowFFT(psi)
+  bool phib_fresh = false;    // phib == rowFFT(dm1(psi)) for the next DM step
+  float phib_g = 0.0f;
+  long long nn() const { return (long long)n * n; }
+  long long mstride() const { return shared ? 0 : nn(); }
+};
+
+namespace {
+
+int set_device(spk_ctx* ctx) {
+  SPK_REQUIRE(ctx, SPK_EINVAL, "null context");
+  SPK_HIP(hipSetDevice(ctx->device));
+  return SPK_OK;
+}
+
+// Real-space update coefficients, from phasing.py:1231-1328 (numpy branch)
+// and phasing_{hio,er,raar,sf,dm1,dm2}.cl (OpenCL branch).
+int make_coef(const spk_step& s, Coef* c) {
+  *c = Coef{0, 0, 0, 0, 0, 0, 0, 0};
+  const float b = s.beta;
+  switch (s.alg) {
+    case SPK_NOOP:
+      return SPK_OK;
+    case SPK_HIO:  // (1-s)*(psi_in - beta*psi_out) + s*psi_out
+      c->ao = 1.0f; c->bo = -b; c->bi = 1.0f;
+      break;
+    case SPK_ER:
+      if (s.semantics == SPK_SEM_OPENCL) c->ao = 1.0f;  // s*psi_out
+      else c->ai = 1.0f;                                 // s*psi_in
+      break;
+    case SPK_RAAR:  // (1-s)*(beta*psi_in + (1-2beta)*psi_out) + s*psi_out
+      c->ao = 1.0f; c->bo = 1.0f - 2.0f * b; c->bi = b;
+      break;
+    case SPK_SF:
+      c->ao = 1.0f;
+      if (s.semantics == SPK_SEM_OPENCL) c->bo = -1.0f;  // (1-s)(-psi_out)
+      else c->bi = -1.0f;                                 // (1-s)(-psi_in)
+      break;
+    case SPK_DM: {  // gamma_s = -1/beta: t1 = (1+beta)psi_out + psi_in - beta t
+      SPK_REQUIRE(b != 0.0f, SPK_EINVAL, "dm needs beta != 0");
+      c->ao = 1.0f + b; c->ai = 1.0f; c->at = -b;
+      c->bo = -b; c->bi = 1.0f;
+      c->flags |= kUseT;
+      break;
+    }
+    default:
+      return fail(SPK_EINVAL, "unknown algorithm " + std::to_string(s.alg));
+  }
+  if (s.semantics != SPK_SEM_NUMPY && s.semantics != SPK_SEM_OPENCL)
+    return fail(SPK_EINVAL, "unknown semantics " + std::to_string(s.semantics));
+  c->flags |= kActive;
+  return SPK_OK;
+}
+
+int make_blur(int n, float sigma, BlurW* w) {
+  SPK_REQUIRE(sigma > 0.0f && std::isfinite(sigma), SPK_EINVAL, "shrinkwrap sigma must be > 0");
+  int R = (int)std::ceil(6.0 * sigma);
+  if (R > n / 2) R = n / 2;
+  SPK_REQUIRE(R <= 47, SPK_EUNSUP, "shrinkwrap sigma too large (max 7.8 px)");
+  // normalisation over the whole circle of n samples centred at n/2
+  // (shape.gaussian centred at size/2, fftshift, divide by the sum:
+  // phasing.py:1787-1788)
+  const double s2 = 2.0 * (double)sigma * sigma;
+  double tot = 0.0;
+  for (int x = 0; x < n; ++x) {
+    const double d = x - n / 2;
+    tot += std::exp(-d * d / s2);
+  }
+  std::memset(w, 0, sizeof(*w));
+  for (int d = 0; d <= R; ++d) w->w[d] = (float)(std::exp(-(double)d * d / s2) / tot);
+  // d = +-n/2 is one sample on the circle: counted once
+  if (R == n / 2) w->w[R] *= 0.5f;
+  w->radius = R;
+  return SPK_OK;
+}
+
+PassArgs base_args(spk_plan* p) {
+  PassArgs a{};
+  a.psi = p->psi;
+  a.phi = p->phi;
+  a.phib = p->phib;
+  a.mod = p->mod;
+  a.sup = p->sup;
+  a.mstride = p->mstride();
+  a.sstride = p->mstride();
+  a.tw = p->tw;
+  a.errp = nullptr;
+  a.tab = nullptr;
+  a.inv_nn = (float)(1.0 / ((double)p->n * p->n));
+  return a;
+}
+
+int ensure_phib(spk_plan* p) {
+  if (p->phib) return SPK_OK;
+  return dalloc(&p->phib, (size_t)p->batch * p->nn());
+}
+
+int ensure_sw(spk_plan* p) {
+  if (p->sw_tmp) return SPK_OK;
+  int rc;
+  if ((rc = dalloc(&p->sw_tmp, (size_t)p->batch * p->nn()))) return rc;
+  if ((rc = dalloc(&p->sw_blur, (size_t)p->batch * p->nn()))) return rc;
+  if ((rc = dalloc(&p->sw_maxp, (size_t)p->batch * p->bands))) return rc;
+  return SPK_OK;
+}
+
+int refresh_phi(spk_plan* p) {
+  if (p->phi_valid) return SPK_OK;
+  SPK_HIP(launch_rowfft(p->n, p->tile, 0, p->psi, p->phi, nullptr, 0, 0.0f, p->tw, p->batch,
+                        p->ctx->stream));
+  p->phi_valid = true;
+  return SPK_OK;
+}
+
+// bracket one pass launch with events when profiling
+int prof_mark(spk_plan* p, int pass) {
+  if (!p->profiling) return SPK_OK;
+  hipEvent_t e;
+  SPK_HIP(hipEventCreate(&e));
+  SPK_HIP(hipEventRecord(e, p->ctx->stream));
+  p->prof_ev[pass].push_back(e);
+  return SPK_OK;
+}
+
+void prof_clear(spk_plan* p, int pass) {
+  for (hipEvent_t e : p->prof_ev[pass]) (void)hipEventDestroy(e);
+  p->prof_ev[pass].clear();
+}
+
+int do_shrinkwrap(spk_plan* p, int frame0, int frames, float sigma, float tau) {
+  SPK_REQUIRE(!(p->shared && p->batch > 1), SPK_EUNSUP,
+              "shrinkwrap needs a plan with one support per frame");
+  SPK_REQUIRE(tau >= 0.0f && tau <= 1.0f, SPK_EINVAL, "shrinkwrap tau must be in [0, 1]");
+  BlurW w;
+  int rc = make_blur(p->n, sigma, &w);
+  if (rc) return rc;
+  if ((rc = ensure_sw(p))) return rc;
+  SPK_HIP(launch_shrinkwrap(p->n, p->tile, p->psi, p->sup, p->mstride(), p->sw_tmp,
+                            p->sw_blur, p->sw_maxp, w, tau, frame0, frames, p->ctx->stream));
+  p->phib_fresh = false;
+  return SPK_OK;
+}
+
+}  // namespace
+
+// ===========================================================================
+extern "C" {
+
+const char* spk_last_error(void) { return g_err.c_str(); }
+
+const char* spk_version(void) { return "speckle_hip 0.1 gfx950"; }
+
+uint32_t spk_supported_sizes(void) {
+  uint32_t m = 0;
+  for (int l = 4; l <= 12; ++l) m |= 1u << l;
+  return m;
+}
+
+int spk_ctx_create(int device, void* stream, spk_ctx** out) {
+  SPK_REQUIRE(out, SPK_EINVAL, "null output pointer");
+  *out = nullptr;
+  int count = 0;
+  SPK_HIP(hipGetDeviceCount(&count));
+  SPK_REQUIRE(device >= 0 && device < count, SPK_EINVAL,
+              "device " + std::to_string(device) + " not present (" + std::to_string(count) +
+                  " devices)");
+  SPK_HIP(hipSetDevice(device));
+  spk_ctx* c = new spk_ctx;
+  c->device = device;
+  c->stream = (hipStream_t)stream;
+  *out = c;
+  return SPK_OK;
+}
+
+int spk_ctx_destroy(spk_ctx* ctx) {
+  if (!ctx) return SPK_OK;
+  (void)hipSetDevice(ctx->device);
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  else (void)hipDeviceSynchronize();
+  dfree(ctx->scratch);
+  delete ctx;
+  return SPK_OK;
+}
+
+int spk_ctx_set_stream(spk_ctx* ctx, void* stream) {
+  SPK_REQUIRE(ctx, SPK_EINVAL, "null context");
+  ctx->stream = (hipStream_t)stream;
+  return SPK_OK;
+}
+
+int spk_sync(spk_ctx* ctx) {
+  int rc = set_device(ctx);
+  if (rc) return rc;
+  if (ctx->stream) SPK_HIP(hipStreamSynchronize(ctx->stream));
+  else SPK_HIP(hipDeviceSynchronize());
+  return SPK_OK;
+}
+
+int spk_plan_bytes(int n, int batch, int shared, int with_dm, size_t* bytes) {
+  SPK_REQUIRE(bytes && is_pow2(n) && batch > 0, SPK_EINVAL, "bad plan size");
+  const size_t nn = (size_t)n * n;
+  const size_t frames_ms = shared ? 1 : (size_t)batch;
+  size_t b = 0;
+  b += 2 * nn * batch * sizeof(c32);             // psi, phi
+  if (with_dm) b += nn * batch * sizeof(c32);    // phib
+  b += 2 * nn * frames_ms * sizeof(float);       // mod, sup
+  b += n * sizeof(c32) + 2 * 1024 * batch * sizeof(double);
+  *bytes = b;
+  return SPK_OK;
+}
+
+int spk_plan_create(spk_ctx* ctx, int n, int batch, int shared, const spk_tune* tune,
+                    spk_plan** out) {
+  SPK_REQUIRE(out, SPK_EINVAL, "null output pointer");
+  *out = nullptr;
+  int rc = set_device(ctx);
+  if (rc) return rc;
+  SPK_REQUIRE(is_pow2(n), SPK_EINVAL, "n must be a power of two (got " + std::to_string(n) + ")");
+  SPK_REQUIRE(batch >= 1, SPK_EINVAL, "batch must be >= 1");
+  int tile = (tune && tune->tile) ? tune->tile : default_tile(n);
+  SPK_REQUIRE(size_supported(n, tile), SPK_EUNSUP,
+              "frame size " + std::to_string(n) + " with tile " + std::to_string(tile) +
+                  " not built (sizes 16..4096; 4096 needs tile 4)");
+  spk_plan* p = new spk_plan;
+  p->ctx = ctx;
+  p->n = n;
+  p->batch = batch;
+  p->shared = shared ? 1 : 0;
+  p->tile = tile;
+  p->bands = n / tile;
+  const size_t nn = (size_t)n * n;
+  const size_t fms = shared ? 1 : (size_t)batch;
+  if ((rc = dalloc(&p->psi, nn * batch)) || (rc = dalloc(&p->phi, nn * batch)) ||
+      (rc = dalloc(&p->mod, nn * fms)) || (rc = dalloc(&p->sup, nn * fms)) ||
+      (rc = dalloc(&p->tw, (size_t)n)) || (rc = dalloc(&p->summ2, (size_t)batch)) ||
+      (rc = dalloc(&p->sumpart, (size_t)64 * batch))) {
+    spk_plan_destroy(p);
+    return rc;
+  }
+  // forward twiddles W_n^k = exp(-2 pi i k / n), evaluated in FP64
+  std::vector<c32> tw(n);
+  for (int k = 0; k < n; ++k) {
+    const double ang = -2.0 * M_PI * (double)k / (double)n;
+    tw[k] = c32{(float)std::cos(ang), (float)std::sin(ang)};
+  }
+  hipError_t e = hipMemcpy(p->tw, tw.data(), n * sizeof(c32), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&p->upload_done, hipEventDisableTiming);
+  if (e != hipSuccess) {
+    spk_plan_destroy(p);
+    return fail(SPK_EHIP, std::string("plan setup: ") + hipGetErrorString(e));
+  }
+  *out = p;
+  return SPK_OK;
+}
+
+int spk_plan_destroy(spk_plan* p) {
+  if (!p) return SPK_OK;
+  if (p->ctx) {
+    (void)hipSetDevice(p->ctx->device);
+    if (p->ctx->stream) (void)hipStreamSynchronize(p->ctx->stream);
+    else (void)hipDeviceSynchronize();
+  }
+  dfree(p->psi); dfree(p->phi); dfree(p->phib); dfree(p->mod); dfree(p->sup);
+  dfree(p->tw); dfree(p->summ2); dfree(p->sumpart); dfree(p->errp);
+  dfree(p->sw_tmp); dfree(p->sw_blur); dfree(p->sw_maxp); dfree(p->fftbuf);
+  dfree(p->dtab); dfree(p->dact);
+  prof_clear(p, 0);
+  prof_clear(p, 1);
+  if (p->htab) (void)hipHostFree(p->htab);
+  if (p->hact) (void)hipHostFree(p->hact);
+  if (p->upload_done) (void)hipEventDestroy(p->upload_done);
+  delete p;
+  return SPK_OK;
+}
+
+static int frame_range(spk_plan* p, int frame, bool per_frame_array, int* f0, int* nf) {
+  // per_frame_array: the array holds one frame per plan frame (mod/sup of a
+  // non-shared plan, psi of any plan)
+  if (frame == -1) {
+    *f0 = 0;
+    *nf = per_frame_array ? p->batch : 1;
+    return SPK_OK;
+  }
+  SPK_REQUIRE(frame >= 0 && frame < p->batch, SPK_EINVAL,
+              "frame " + std::to_string(frame) + " out of range [0, " + std::to_string(p->batch) + ")");
+  if (!per_frame_array) {
+    SPK_REQUIRE(frame == 0, SPK_EINVAL, "a shared plan has one modulus/support (frame 0 or -1)");
+  }
+  *f0 = frame;
+  *nf = 1;
+  return SPK_OK;
+}
+
+int spk_set_modulus(spk_plan* p, int frame, const float* m) {
+  SPK_REQUIRE(p && m, SPK_EINVAL, "null plan or modulus");
+  int rc = set_device(p->ctx);
+  if (rc) return rc;
+  int f0, nf;
+  if ((rc = frame_range(p, frame, !p->shared, &f0, &nf))) return rc;
+  hipStream_t st = p->ctx->stream;
+  SPK_HIP(launch_pack_f32(p->n, p->tile, p->mod + (size_t)f0 * p->nn(), m, nf, st));
+  const int nblk = 64;
+  SPK_HIP(launch_sumsq(p->n, p->mod + (size_t)f0 * p->nn(), p->nn(), p->sumpart, nblk,
+                       p->summ2 + f0, nf, st));
+  if (p->shared) {
+    for (int f = 1; f < p->batch; ++f)
+      SPK_HIP(hipMemcpyAsync(p->summ2 + f, p->summ2, sizeof(double), hipMemcpyDeviceToDevice, st));
+  }
+  p->mod_set = true;
+  return SPK_OK;
+}
+
+int spk_set_support(spk_plan* p, int frame, const float* s) {
+  SPK_REQUIRE(p && s, SPK_EINVAL, "null plan or support");
+  int rc = set_device(p->ctx);
+  if (rc) return rc;
+  int f0, nf;
+  if ((rc = frame_range(p, frame, !p->shared, &f0, &nf))) return rc;
+  SPK_HIP(launch_pack_f32(p->n, p->tile, p->sup + (size_t)f0 * p->nn(), s, nf, p->ctx->stream));
+  p->sup_set = true;
+  p->phib_fresh = false;
+  return SPK_OK;
+}
+
+int spk_get_support(spk_plan* p, int frame, float* s) {
+  SPK_REQUIRE(p && s, SPK_EINVAL, "null plan or support");
+  int rc = set_device(p->ctx);
+  if (rc) return rc;
+  int f0, nf;
+  if ((rc = frame_range(p, frame, !p->shared, &f0, &nf))) return rc;
+  SPK_HIP(launch_unpack_f32(p->n, p->tile, s, p->sup + (size_t)f0 * p->nn(), nf, p->ctx->stream));
+  return SPK_OK;
+}
+
+int spk_set_psi(spk_plan* p, int frame, const float* guess, int mult_support) {
+  SPK_REQUIRE(p && guess, SPK_EINVAL, "null plan or guess");
+  int rc = set_device(p->ctx);
+  if (rc) return rc;
+  SPK_REQUIRE(!mult_support || p->sup_set, SPK_ESTATE, "support not set");
+  int f0, nf;
+  if ((rc = frame_range(p, frame, true, &f0, &nf))) return rc;
+  const float* sup = mult_support ? p->sup + (size_t)(p->shared ? 0 : f0) * p->nn() : nullptr;
+  SPK_HIP(launch_pack_c64(p->n, p->tile, p->psi + (size_t)f0 * p->nn(), (const c32*)guess, sup,
+                          p->mstride(), nf, p->ctx->stream));
+  p->psi_set = true;
+  p->phi_valid = false;
+  p->phib_fresh = false;
+  return SPK_OK;
+}
+
+int spk_get_psi(spk_plan* p, int frame, float* out) {
+  SPK_REQUIRE(p && out, SPK_EINVAL, "null plan or output");
+  int rc = set_device(p->ctx);
+  if (rc) return rc;
+  int f0, nf;
+  if ((rc = frame_range(p, frame, true, &f0, &nf))) return rc;
+  SPK_HIP(launch_unpack_c64(p->n, p->tile, (c32*)out, p->psi + (size_t)f0 * p->nn(), 1.0f, nf,
+                            p->ctx->stream));
+  return SPK_OK;
+}
+
+int spk_iterate(spk_plan* p, const spk_step* steps, int nsteps, int per_frame,
+                double* err_trace) {
+  SPK_REQUIRE(p, SPK_EINVAL, "null plan");
+  int rc = set_device(p->ctx);
+  if (rc) return rc;
+  SPK_REQUIRE(nsteps >= 0, SPK_EINVAL, "nsteps must be >= 0");
+  if (nsteps == 0) return SPK_OK;
+  SPK_REQUIRE(steps, SPK_EINVAL, "null steps");
+  SPK_REQUIRE(p->mod_set && p->sup_set && p->psi_set, SPK_ESTATE,
+              "cant iterate before loading modulus, support and a seed");
+  hipStream_t st = p->ctx->stream;
+  const int B = p->batch;
+  const int W = per_frame ? B : 1;  // step entries per iteration
+
+  // coefficients for every (step, frame entry)
+  std::vector<Coef> coefs((size_t)nsteps * W);
+  std::vector<int> act((size_t)nsteps * W);
+  std::vector<char> any_dm(nsteps, 0), any_sw(nsteps, 0);
+  for (int k = 0; k < nsteps; ++k) {
+    for (int f = 0; f < W; ++f) {
+      const spk_step& s = steps[(size_t)k * W + f];
+      if ((rc = make_coef(s, &coefs[(size_t)k * W + f]))) return rc;
+      act[(size_t)k * W + f] = s.alg != SPK_NOOP;
+      if (s.alg == SPK_DM) any_dm[k] = 1;
+      if (s.alg != SPK_NOOP && s.sw_sigma > 0.0f) any_sw[k] = 1;
+    }
+  }
+  // next-step DM: emit phib from the row pass of step k (no shrinkwrap between)
+  for (int k = 0; k + 1 < nsteps; ++k) {
+    if (!any_dm[k + 1] || any_sw[k]) continue;
+    for (int f = 0; f < W; ++f) {
+      const spk_step& nx = steps[(size_t)(k + 1) * W + f];
+      Coef& c = coefs[(size_t)k * W + f];
+      if (nx.alg == SPK_DM && (c.flags & kActive)) {
+        c.flags |= kEmitB;
+        c.g_next = 1.0f / nx.beta;
+      }
+    }
+  }
+  bool dm_any = false;
+  for (int k = 0; k < nsteps; ++k) dm_any |= any_dm[k] != 0;
+  if (dm_any && (rc = ensure_phib(p))) return rc;
+  if (per_frame && dm_any) {
+    // a step whose phib is produced by a rowfft (not by the previous row
+    // pass) needs one gamma for all frames of that step
+    for (int k = 0; k < nsteps; ++k) {
+      if (!any_dm[k]) continue;
+      float g = 0.0f;
+      for (int f = 0; f < W; ++f) {
+        const spk_step& s = steps[(size_t)k * W + f];
+        if (s.alg != SPK_DM) continue;
+        if (g == 0.0f) g = s.beta;
+        SPK_REQUIRE(s.beta == g, SPK_EUNSUP, "per-frame DM steps need one beta per step");
+      }
+    }
+  }
+
+  // per-frame table upload (pinned staging, reused once the previous upload ran)
+  if (per_frame) {
+    const size_t need = (size_t)nsteps * W;
+    if (need > p->dtab_cap) {
+      dfree(p->dtab);
+      dfree(p->dact);
+      if (p->htab) (void)hipHostFree(p->htab);
+      if (p->hact) (void)hipHostFree(p->hact);
+      p->htab = nullptr;
+      p->hact = nullptr;
+      p->dtab_cap = 0;
+      if ((rc = dalloc(&p->dtab, need)) || (rc = dalloc(&p->dact, need))) return rc;
+      SPK_HIP(hipHostMalloc((void**)&p->htab, need * sizeof(Coef), hipHostMallocDefault));
+      SPK_HIP(hipHostMalloc((void**)&p->hact, need * sizeof(int), hipHostMallocDefault));
+      p->dtab_cap = need;
+    } else {
+      SPK_HIP(hipEventSynchronize(p->upload_done));
+    }
+    std::memcpy(p->htab, coefs.data(), need * sizeof(Coef));
+    std::memcpy(p->hact, act.data(), need * sizeof(int));
+    SPK_HIP(hipMemcpyAsync(p->dtab, p->htab, need * sizeof(Coef), hipMemcpyHostToDevice, st));
+    SPK_HIP(hipMemcpyAsync(p->dact, p->hact, need * sizeof(int), hipMemcpyHostToDevice, st));
+    SPK_HIP(hipEventRecord(p->upload_done, st));
+  }
+
+  // error partials [step][frame][band]
+  if (err_trace) {
+    const size_t need = (size_t)nsteps * B * p->bands;
+    if (need > p->errp_cap) {
+      dfree(p->errp);
+      p->errp_cap = 0;
+      if ((rc = dalloc(&p->errp, need))) return rc;
+      p->errp_cap = need;
+    }
+  }
+
+  if ((rc = refresh_phi(p))) return rc;
+  PassArgs a = base_args(p);
+  for (int k = 0; k < nsteps; ++k) {
+    const size_t row = (size_t)k * W;
+    if (per_frame) {
+      a.tab = p->dtab + row;
+    } else {
+      a.tab = nullptr;
+      a.c = coefs[row];
+    }
+    if (any_dm[k] && !p->phib_fresh) {
+      // phib = rowFFT((s - gamma_m (1-s)) psi), dm1 (phasing_dm1.cl:16)
+      float beta = 0.0f;
+      for (int f = 0; f < W; ++f)
+        if (steps[row + f].alg == SPK_DM) { beta = steps[row + f].beta; break; }
+      SPK_HIP(launch_rowfft(p->n, p->tile, 2, p->psi, p->phib, p->sup, p->mstride(),
+                            1.0f / beta, p->tw, B, st));
+    }
+    a.errp = err_trace ? p->errp + (size_t)k * B * p->bands : nullptr;
+    if ((rc = prof_mark(p, 0))) return rc;
+    SPK_HIP(launch_colpass(p->n, p->tile, 0, a, B, any_dm[k] ? 2 : 1, st));
+    if ((rc = prof_mark(p, 0))) return rc;
+    bool emit = false;
+    for (int f = 0; f < W; ++f) emit |= (coefs[row + f].flags & kEmitB) != 0;
+    a.errp = nullptr;
+    if ((rc = prof_mark(p, 1))) return rc;
+    SPK_HIP(launch_rowpass(p->n, p->tile, any_dm[k] != 0, emit, a, B, st));
+    if ((rc = prof_mark(p, 1))) return rc;
+    p->phib_fresh = emit;
+    if (any_sw[k]) {
+      if (!per_frame) {
+        if ((rc = do_shrinkwrap(p, 0, B, steps[row].sw_sigma, steps[row].sw_tau))) return rc;
+      } else {
+        for (int f = 0; f < W; ++f) {
+          const spk_step& s = steps[row + f];
+          if (s.alg == SPK_NOOP || !(s.sw_sigma > 0.0f)) continue;
+          if ((rc = do_shrinkwrap(p, f, 1, s.sw_sigma, s.sw_tau))) return rc;
+        }
+      }
+    }
+  }
+  p->phi_valid = true;
+  if (err_trace) {
+    SPK_HIP(launch_err_finalize(p->errp, nsteps, B, p->bands, p->summ2, err_trace,
+                                per_frame ? p->dact : nullptr, st));
+  }
+  return SPK_OK;
+}
+
+int spk_shrinkwrap(spk_plan* p, int frame, float sigma, float tau) {
+  SPK_REQUIRE(p, SPK_EINVAL, "null plan");
+  int rc = set_device(p->ctx);
+  if (rc) return rc;
+  SPK_REQUIRE(p->psi_set && p->sup_set, SPK_ESTATE, "shrinkwrap needs psi and support");
+  int f0, nf;
+  if ((rc = frame_range(p, frame, true, &f0, &nf))) return rc;
+  return do_shrinkwrap(p, f0, nf, sigma, tau);
+}
+
+int spk_copy_roi(spk_plan* p, int frame, float* dst, int slots, int slot, int r0, int c0,
+                 int rows, int cols) {
+  SPK_REQUIRE(p && dst, SPK_EINVAL, "null plan or destination");
+  int rc = set_device(p->ctx);
+  if (rc) return rc;
+  SPK_REQUIRE(frame >= 0 && frame < p->batch, SPK_EINVAL, "frame out of range");
+  SPK_REQUIRE(slot >= 0 && slot < slots, SPK_EINVAL,
+              "savebuffer slot " + std::to_string(slot) + " out of range [0, " +
+                  std::to_string(slots) + ")");
+  SPK_REQUIRE(rows > 0 && cols > 0 && r0 >= 0 && c0 >= 0 && r0 + rows <= p->n &&
+                  c0 + cols <= p->n,
+              SPK_EINVAL, "ROI outside the frame");
+  c32* d = (c32*)dst + (size_t)slot * rows * cols;
+  SPK_HIP(launch_copy_roi(p->n, p->tile, p->psi + (size_t)frame * p->nn(), d, r0, c0, rows,
+                          cols, p->ctx->stream));
+  return SPK_OK;
+}
+
+int spk_fft2(spk_plan* p, const float* in, float* out, int count, int inverse) {
+  SPK_REQUIRE(p && in && out, SPK_EINVAL, "null plan or buffer");
+  SPK_REQUIRE(count >= 1, SPK_EINVAL, "count must be >= 1");
+  int rc = set_device(p->ctx);
+  if (rc) return rc;
+  if ((size_t)count > p->fftbuf_frames) {
+    dfree(p->fftbuf);
+    p->fftbuf_frames = 0;
+    if ((rc = dalloc(&p->fftbuf, (size_t)count * p->nn()))) return rc;
+    p->fftbuf_frames = count;
+  }
+  hipStream_t st = p->ctx->stream;
+  SPK_HIP(launch_pack_c64(p->n, p->tile, p->fftbuf, (const c32*)in, nullptr, 0, count, st));
+  SPK_HIP(launch_rowfft(p->n, p->tile, inverse ? 1 : 0, p->fftbuf, p->fftbuf, nullptr, 0, 0.0f,
+                        p->tw, count, st));
+  PassArgs a = base_args(p);
+  a.phi = p->fftbuf;
+  a.c.flags = kActive;
+  SPK_HIP(launch_colpass(p->n, p->tile, inverse ? 3 : 2, a, count, 1, st));
+  const float scl = inverse ? (float)(1.0 / ((double)p->n * p->n)) : 1.0f;
+  SPK_HIP(launch_unpack_c64(p->n, p->tile, (c32*)out, p->fftbuf, scl, count, st));
+  return SPK_OK;
+}
+
+int spk_align_global_phase(spk_ctx* ctx, float* data, int frames, int rows, int cols) {
+  SPK_REQUIRE(data && frames >= 1 && rows >= 1 && cols >= 1, SPK_EINVAL, "bad stack");
+  int rc = set_device(ctx);
+  if (rc) return rc;
+  const long long per = (long long)rows * cols;
+  int nblk = (int)std::min<long long>(64, (per + 255) / 256);
+  const size_t need = (size_t)frames * 5 * nblk;
+  if (need > ctx->scratch_cap) {
+    dfree(ctx->scratch);
+    ctx->scratch_cap = 0;
+    if ((rc = dalloc(&ctx->scratch, need))) return rc;
+    ctx->scratch_cap = need;
+  }
+  SPK_HIP(launch_align((c32*)data, frames, per, ctx->scratch, nblk, ctx->stream));
+  return SPK_OK;
+}
+
+int spk_copy_bytes(spk_ctx* ctx, void* dst, const void* src, size_t bytes) {
+  SPK_REQUIRE(dst && src && bytes % 16 == 0, SPK_EINVAL, "copy needs 16-byte multiples");
+  int rc = set_device(ctx);
+  if (rc) return rc;
+  SPK_HIP(launch_copy(dst, src, bytes, ctx->stream));
+  return SPK_OK;
+}
+
+int spk_plan_profile(spk_plan* p, int enable) {
+  SPK_REQUIRE(p, SPK_EINVAL, "null plan");
+  p->profiling = enable != 0;
+  return SPK_OK;
+}
+
+int spk_plan_profile_read(spk_plan* p, int pass, int* launches, double* total_ms) {
+  SPK_REQUIRE(p && launches && total_ms, SPK_EINVAL, "null argument");
+  SPK_REQUIRE(pass == 0 || pass == 1, SPK_EINVAL, "pass must be 0 (column) or 1 (row)");
+  int rc = set_device(p->ctx);
+  if (rc) return rc;
+  std::vector<hipEvent_t>& ev = p->prof_ev[pass];
+  double tot = 0.0;
+  int cnt = 0;
+  if (!ev.empty()) SPK_HIP(hipEventSynchronize(ev.back()));
+  for (size_t i = 0; i + 1 < ev.size(); i += 2) {
+    float ms = 0.0f;
+    SPK_HIP(hipEventElapsedTime(&ms, ev[i], ev[i + 1]));
+    tot += ms;
+    ++cnt;
+  }
+  prof_clear(p, pass);
+  *launches = cnt;
+  *total_ms = tot;
+  return SPK_OK;
+}
+
+int spk_time_passes(spk_plan* p, int which, int reps, const spk_step* step, float* ms_out) {
+  SPK_REQUIRE(p && step && ms_out && reps >= 1, SPK_EINVAL, "bad arguments");
+  SPK_REQUIRE(which >= 0 && which <= 2, SPK_EINVAL, "which: 0 column, 1 row, 2 both");
+  int rc = set_device(p->ctx);
+  if (rc) return rc;
+  SPK_REQUIRE(p->mod_set && p->sup_set && p->psi_set, SPK_ESTATE, "plan not loaded");
+  SPK_REQUIRE(step->alg != SPK_DM, SPK_EUNSUP, "timing helper covers single-projection steps");
+  Coef c;
+  if ((rc = make_coef(*step, &c))) return rc;
+  if ((rc = refresh_phi(p))) return rc;
+  hipStream_t st = p->ctx->stream;
+  PassArgs a = base_args(p);
+  a.c = c;
+  hipEvent_t e0, e1;
+  SPK_HIP(hipEventCreate(&e0));
+  SPK_HIP(hipEventCreate(&e1));
+  SPK_HIP(hipEventRecord(e0, st));
+  for (int r = 0; r < reps; ++r) {
+    if (which != 1) SPK_HIP(launch_colpass(p->n, p->tile, 0, a, p->batch, 1, st));
+    if (which != 0) SPK_HIP(launch_rowpass(p->n, p->tile, false, false, a, p->batch, st));
+  }
+  SPK_HIP(hipEventRecord(e1, st));
+  SPK_HIP(hipEventSynchronize(e1));
+  SPK_HIP(hipEventElapsedTime(ms_out, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return SPK_OK;
+}
+
+}  // extern "C"

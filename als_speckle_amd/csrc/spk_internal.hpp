@@ -1,0 +1,89 @@
+// Internal types shared by the kernels (spk_kernels.hip) and the C ABI
+// (spk_capi.hip).  Not part of the public interface.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "spk_fft_core.hpp"
+
+namespace spk {
+
+// Per-frame real-space update of one iteration, the reference's five update
+// kernels folded into one coefficient form:
+//   psi_new = s*(ao*psi_out + ai*psi_in + at*t) + (1-s)*(bo*psi_out + bi*psi_in + bt*t)
+// with t the DM auxiliary projection.  Values per algorithm are set on the
+// host (spk_capi.hip, make_coef) from phasing.py:1231-1328 and the .cl kernels.
+enum : int {
+  kActive = 1,    // frame takes part in this step
+  kUseT = 2,      // step is DM: needs t (second column pass on phib)
+  kEmitB = 4,     // next step is DM: row pass also writes phib = rowFFT(dm1(psi))
+};
+
+struct Coef {
+  float ao, ai, at;
+  float bo, bi, bt;
+  float g_next;   // gamma_m = 1/beta of the next (DM) step, used with kEmitB
+  int flags;
+};
+
+struct PassArgs {
+  c32* psi;               // tiled iterate, frame stride n*n
+  c32* phi;               // tiled row-transform of psi
+  c32* phib;              // tiled row-transform of the DM modified estimate
+  const float* mod;       // tiled modulus, frame stride mstride (0 = shared)
+  const float* sup;       // tiled support, frame stride sstride (0 = shared)
+  long long mstride;
+  long long sstride;
+  const c32* tw;          // forward twiddles W_n^k, k < n
+  double* errp;           // column pass: partial error sums [frame][band]
+  const Coef* tab;        // per-frame coefficients (nullptr: use c)
+  Coef c;
+  float inv_nn;           // 1/(n*n)
+};
+
+struct BlurW {
+  float w[48];  // w[d] for |d| <= radius, normalised over the circle
+  int radius;
+};
+
+// Kernel launchers (spk_kernels.hip).  All return hipError_t of the launch.
+// `tile` is the internal layout tile edge (4 or 8).
+bool size_supported(int n, int tile);
+hipError_t launch_colpass(int n, int tile, int mode, const PassArgs& a,
+                          int frames, int zdim, hipStream_t st);
+hipError_t launch_rowpass(int n, int tile, bool use_t, bool emit_b,
+                          const PassArgs& a, int frames, hipStream_t st);
+// mode 0: out = rowFFT(in); 1: out = rowIFFT(in) (unscaled);
+// 2: out = rowFFT((s - g*(1-s)) * in)
+hipError_t launch_rowfft(int n, int tile, int mode, const c32* in, c32* out,
+                         const float* sup, long long sstride, float g,
+                         const c32* tw, int frames, hipStream_t st);
+hipError_t launch_pack_c64(int n, int tile, c32* dst, const c32* src,
+                           const float* sup, long long sstride, int frames,
+                           hipStream_t st);
+hipError_t launch_unpack_c64(int n, int tile, c32* dst, const c32* src,
+                             float scale, int frames, hipStream_t st);
+hipError_t launch_pack_f32(int n, int tile, float* dst, const float* src,
+                           int frames, hipStream_t st);
+hipError_t launch_unpack_f32(int n, int tile, float* dst, const float* src,
+                             int frames, hipStream_t st);
+hipError_t launch_sumsq(int n, const float* mod, long long mstride,
+                        double* partial, int nblk, double* out, int frames,
+                        hipStream_t st);
+hipError_t launch_err_finalize(const double* errp, int nsteps, int frames,
+                               int bands, const double* summ2, double* trace,
+                               const int* active, hipStream_t st);
+hipError_t launch_shrinkwrap(int n, int tile, const c32* psi, float* sup,
+                             long long sstride, float* tmp, float* blur,
+                             float* maxp, const BlurW& w, float tau,
+                             int frame0, int frames, hipStream_t st);
+hipError_t launch_copy_roi(int n, int tile, const c32* psi, c32* dst, int r0,
+                           int c0, int rows, int cols, hipStream_t st);
+hipError_t launch_align(c32* data, int frames, long long per_frame,
+                        double* partial, int nblk, hipStream_t st);
+hipError_t launch_copy(void* dst, const void* src, size_t bytes,
+                       hipStream_t st);
+
+}  // namespace spk

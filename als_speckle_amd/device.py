@@ -1,0 +1,203 @@
+"""Device plumbing: a context (device + stream) and plans over torch tensors.
+
+PyTorch provides device memory and the stream; every computation is a call
+into ``libspeckle_hip.so``.  Nothing here computes on the host.
+"""
+from __future__ import annotations
+
+import ctypes
+from ctypes import byref, c_float, c_size_t, c_void_p
+
+import numpy as np
+import torch
+
+from . import _capi
+from ._capi import check
+
+
+def require_gpu() -> None:
+    if not torch.cuda.is_available():
+        raise RuntimeError(
+            "als_speckle_amd runs on a ROCm GPU only and there is none visible "
+            "(torch.cuda.is_available() is False); it has no CPU path")
+
+
+def _ptr(t: torch.Tensor) -> c_void_p:
+    return c_void_p(t.data_ptr())
+
+
+def _expect(t: torch.Tensor, dtype, shape, device, name):
+    if not isinstance(t, torch.Tensor):
+        raise TypeError(f"{name} must be a torch tensor")
+    if t.dtype != dtype:
+        raise TypeError(f"{name} must be {dtype}, got {t.dtype}")
+    if tuple(t.shape) != tuple(shape):
+        raise ValueError(f"{name} must have shape {tuple(shape)}, got {tuple(t.shape)}")
+    if t.device != device:
+        raise ValueError(f"{name} must be on {device}, got {t.device}")
+    if not t.is_contiguous():
+        raise ValueError(f"{name} must be contiguous")
+
+
+class Context:
+    """One device and one HIP stream (torch's current stream by default)."""
+
+    def __init__(self, device: int = 0, stream: torch.cuda.Stream | None = None):
+        require_gpu()
+        self.lib = _capi.load()
+        self.index = int(device)
+        self.device = torch.device("cuda", self.index)
+        self.stream = stream if stream is not None else torch.cuda.current_stream(self.device)
+        h = c_void_p()
+        check(self.lib.spk_ctx_create(self.index, c_void_p(self.stream.cuda_stream), byref(h)))
+        self.handle = h
+
+    def sync(self) -> None:
+        check(self.lib.spk_sync(self.handle))
+
+    def align_global_phase(self, data: torch.Tensor) -> torch.Tensor:
+        """In place on a (frames, rows, cols) complex64 device tensor."""
+        if data.dim() == 2:
+            data = data.unsqueeze(0)
+        f, r, c = data.shape
+        _expect(data, torch.complex64, (f, r, c), self.device, "data")
+        check(self.lib.spk_align_global_phase(self.handle, _ptr(data), f, r, c))
+        return data
+
+    def copy_bytes(self, dst: torch.Tensor, src: torch.Tensor) -> None:
+        nb = src.numel() * src.element_size()
+        check(self.lib.spk_copy_bytes(self.handle, _ptr(dst), _ptr(src), nb))
+
+    def close(self) -> None:
+        if getattr(self, "handle", None):
+            self.lib.spk_ctx_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def plan_bytes(n: int, batch: int, shared: bool, with_dm: bool = False) -> int:
+    out = c_size_t()
+    check(_capi.load().spk_plan_bytes(n, batch, int(shared), int(with_dm), byref(out)))
+    return out.value
+
+
+class Plan:
+    """``batch`` resident frames of n x n (see spk_plan_create)."""
+
+    def __init__(self, ctx: Context, n: int, batch: int = 1, shared: bool = True,
+                 tile: int = 0):
+        self.ctx, self.lib = ctx, ctx.lib
+        self.n, self.batch, self.shared = int(n), int(batch), bool(shared)
+        tune = _capi.SpkTune()
+        tune.tile = int(tile)
+        h = c_void_p()
+        check(self.lib.spk_plan_create(ctx.handle, self.n, self.batch, int(self.shared),
+                                       byref(tune), byref(h)))
+        self.handle = h
+
+    # -- helpers ---------------------------------------------------------
+    def _frames_shape(self, frame, per_frame):
+        n = self.n
+        if frame == -1 and per_frame and self.batch > 1:
+            return (self.batch, n, n)
+        return (n, n)
+
+    # -- data in / out ---------------------------------------------------
+    def set_modulus(self, m: torch.Tensor, frame: int = -1) -> None:
+        _expect(m, torch.float32, self._frames_shape(frame, not self.shared), self.ctx.device, "modulus")
+        check(self.lib.spk_set_modulus(self.handle, frame, _ptr(m)))
+
+    def set_support(self, s: torch.Tensor, frame: int = -1) -> None:
+        _expect(s, torch.float32, self._frames_shape(frame, not self.shared), self.ctx.device, "support")
+        check(self.lib.spk_set_support(self.handle, frame, _ptr(s)))
+
+    def get_support(self, frame: int = -1) -> torch.Tensor:
+        out = torch.empty(self._frames_shape(frame, not self.shared), dtype=torch.float32,
+                          device=self.ctx.device)
+        check(self.lib.spk_get_support(self.handle, frame, _ptr(out)))
+        return out
+
+    def set_psi(self, guess: torch.Tensor, frame: int = -1, mult_support: bool = True) -> None:
+        _expect(guess, torch.complex64, self._frames_shape(frame, True), self.ctx.device, "guess")
+        check(self.lib.spk_set_psi(self.handle, frame, _ptr(guess), int(mult_support)))
+
+    def get_psi(self, frame: int = -1) -> torch.Tensor:
+        out = torch.empty(self._frames_shape(frame, True), dtype=torch.complex64,
+                          device=self.ctx.device)
+        check(self.lib.spk_get_psi(self.handle, frame, _ptr(out)))
+        return out
+
+    # -- the loop --------------------------------------------------------
+    def iterate(self, steps, per_frame: bool = False, error_trace: bool = True,
+                err_out: torch.Tensor | None = None):
+        """steps: list of (alg, sem, beta, sw_sigma, sw_tau) codes.
+
+        Returns a (nsteps, batch) float64 device tensor of modulus errors, or
+        None.  With per_frame, ``steps`` has nsteps * batch entries.
+        """
+        steps = list(steps)
+        nsteps = len(steps) // (self.batch if per_frame else 1)
+        arr = _capi.make_steps(steps)
+        err = None
+        if error_trace:
+            err = err_out if err_out is not None else torch.empty(
+                (nsteps, self.batch), dtype=torch.float64, device=self.ctx.device)
+        check(self.lib.spk_iterate(self.handle, arr, nsteps, int(per_frame),
+                                   _ptr(err) if err is not None else None))
+        return err
+
+    def shrinkwrap(self, sigma: float, tau: float, frame: int = -1) -> None:
+        check(self.lib.spk_shrinkwrap(self.handle, frame, float(sigma), float(tau)))
+
+    def copy_roi(self, frame: int, dst: torch.Tensor, slot: int, r0: int, c0: int,
+                 rows: int, cols: int) -> None:
+        if dst.dim() != 3 or tuple(dst.shape[1:]) != (rows, cols):
+            raise ValueError("dst must be (slots, rows, cols)")
+        _expect(dst, torch.complex64, tuple(dst.shape), self.ctx.device, "savebuffer")
+        check(self.lib.spk_copy_roi(self.handle, frame, _ptr(dst), dst.shape[0], slot,
+                                    r0, c0, rows, cols))
+
+    def fft2(self, x: torch.Tensor, inverse: bool = False, out: torch.Tensor | None = None):
+        squeeze = x.dim() == 2
+        xx = x.unsqueeze(0) if squeeze else x
+        cnt = xx.shape[0]
+        _expect(xx, torch.complex64, (cnt, self.n, self.n), self.ctx.device, "input")
+        o = out if out is not None else torch.empty_like(xx)
+        check(self.lib.spk_fft2(self.handle, _ptr(xx), _ptr(o), cnt, int(inverse)))
+        return o[0] if squeeze else o
+
+    def profile(self, enable: bool) -> None:
+        check(self.lib.spk_plan_profile(self.handle, int(enable)))
+
+    def profile_read(self, which: int):
+        """(launches, total_ms) of pass 0 (column) or 1 (row) since the last read."""
+        n = ctypes.c_int()
+        ms = ctypes.c_double()
+        check(self.lib.spk_plan_profile_read(self.handle, which, byref(n), byref(ms)))
+        return n.value, ms.value
+
+    def time_passes(self, which: int, reps: int, step) -> float:
+        arr = _capi.make_steps([step])
+        ms = c_float()
+        check(self.lib.spk_time_passes(self.handle, which, reps, arr, byref(ms)))
+        return ms.value
+
+    def close(self) -> None:
+        if getattr(self, "handle", None):
+            self.lib.spk_plan_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def to_device(a: np.ndarray, device, dtype) -> torch.Tensor:
+    return torch.from_numpy(np.ascontiguousarray(a)).to(device=device, dtype=dtype)

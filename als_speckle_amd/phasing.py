@@ -1,0 +1,427 @@
+"""Drop-in for ``speckle.phasing`` (reference ``dhparks/als_speckle``).
+
+The ``phasing`` class keeps the reference object API
+(``speckle/phasing.py:33-1353``): ``load`` modulus / support / numtrials,
+``seed``, ``iterate`` with ER/HIO/RAAR/SF/DM, ``get``, ``status`` and the
+attributes ``savebuffer, support, r0, c0, rows, cols, numtrial, numtrials, N,
+shape, compute_device, can_iterate``.  All iteration work runs in
+``libspeckle_hip.so`` on the GPU; there is no CPU path (``force_cpu=True``
+raises).
+
+Additive keywords of ``iterate``: ``shrinkwrap`` (in-loop support refinement),
+``semantics`` ('numpy' = the reference CPU formulas, the default; 'opencl' =
+its GPU kernels, which differ for ER and SF), ``error_trace``.
+"""
+from __future__ import annotations
+
+import sys
+import warnings
+
+import numpy as np
+import torch
+
+from . import _capi
+from .device import Context, Plan, require_gpu, to_device
+from .masking import bounding_box
+
+__all__ = ["phasing", "expand_order", "shrinkwrap_schedule", "DeviceArray"]
+
+
+def expand_order(order, iterations):
+    """Algorithm name per iteration, ``phasing.py:244-260``.
+
+    ``order`` is an iterable of (name, count) entries, default
+    (('hio', 99), ('er', 1)); the expanded list repeats cyclically.
+    """
+    assert isinstance(order, (type(None), list, tuple)), "algorithm order must be iterable"
+    if order is None:
+        order = (("hio", 99), ("er", 1))
+    flat = []
+    for entry in order:
+        try:
+            name, count = str(entry[0]), int(entry[1])
+        except (IndexError, TypeError, ValueError):
+            raise AssertionError("entry %s in iteration order is improperly formatted" % (entry,))
+        flat.extend([name] * count)
+    assert flat, "algorithm order is empty"
+    for name in set(flat):
+        assert name in _capi.ALG_CODES, "real space enforcement algorithm %s is unknown" % name
+    k = len(flat)
+    return [flat[i % k] for i in range(int(iterations))]
+
+
+def shrinkwrap_schedule(iterations, every=20, sigma0=3.0, sigma1=1.5, tau=0.2):
+    """{iteration index (0-based): (sigma, tau)} for an in-loop shrinkwrap.
+
+    After iteration k (1-based) with k % every == 0; sigma goes linearly from
+    sigma0 to sigma1 over the iterations // every events.
+    """
+    events = int(iterations) // int(every)
+    out = {}
+    for e in range(events):
+        sig = sigma0 if events <= 1 else sigma0 + (sigma1 - sigma0) * e / (events - 1)
+        out[(e + 1) * every - 1] = (float(sig), float(tau))
+    return out
+
+
+class DeviceArray:
+    """Handle to a device-resident array of a reconstruction.
+
+    Plays the role of the reference's ``arrayWrapper`` (gpu.py:3-15):
+    ``phasing.get(handle)`` returns a host numpy copy.
+    """
+
+    def __init__(self, name, shape, dtype, fetch):
+        self.name, self.shape, self.dtype = name, tuple(shape), np.dtype(dtype)
+        self._fetch = fetch
+        self.isgpu = True
+
+    @property
+    def size(self):
+        return int(np.prod(self.shape))
+
+    @property
+    def ndim(self):
+        return len(self.shape)
+
+    def tensor(self) -> torch.Tensor:
+        return self._fetch()
+
+    def get(self) -> np.ndarray:
+        return self._fetch().cpu().numpy()
+
+    def __repr__(self):
+        return f"DeviceArray({self.name}, shape={self.shape}, dtype={self.dtype})"
+
+
+class phasing:
+    """A reconstruction resident on one MI355X (``speckle/phasing.py:33-179``).
+
+    Typical use is the reference's::
+
+        r = phasing()
+        r.load(modulus=m, support=s, numtrials=10)
+        for t in range(10):
+            r.seed()
+            r.iterate(500)
+        frames = r.get(r.savebuffer)
+    """
+
+    def __init__(self, force_cpu=False, force_np_fft=False, gpu_info=None):
+        if force_cpu:
+            raise RuntimeError(
+                "als_speckle_amd has no CPU path: force_cpu=True is not supported")
+        require_gpu()
+        # gpu_info: None, a device index, or a Context (the reference passes
+        # its OpenCL (context, device, queue, platform) tuple, gpu.py:399-421)
+        if isinstance(gpu_info, Context):
+            self.ctx = gpu_info
+        else:
+            self.ctx = Context(int(gpu_info) if gpu_info is not None else 0)
+        self.use_gpu = True
+        self.compute_device = "gpu"
+        self.gpu_info = self.ctx
+
+        # state flags, phasing.py:121-127
+        self.modulus_state = 0
+        self.support_state = 0
+        self.ipsf_state = 0
+        self.buffer_state = 0
+        self.rl_state = 0
+        self.spectrum_state = 0
+        self.can_iterate = False
+        self.numtrial = 0
+        self.array_dtypes = ("float32", "complex64")
+
+        self.N = None
+        self.shape = None
+        self.size = None
+        self.modulus = None
+        self.support = None
+        self.psi_in = None
+        self.numtrials = None
+        self.r0 = self.c0 = self.rows = self.cols = None
+        self.savebuffer = None
+        self.reconstruction = None
+        self.ipsf = None
+
+        self._plan = None
+        self._support_host = None   # host copy of the last loaded support (N x N)
+        self._seeded = False
+        self._trace = None           # device float64 trace of the last iterate()
+        self._traces = []
+
+    # ------------------------------------------------------------------
+    def _make_plan(self):
+        self._plan = Plan(self.ctx, self.N, batch=1, shared=True)
+        plan = self._plan
+        self.modulus = DeviceArray("modulus", self.shape, np.float32, lambda: self._modulus_dev)
+        self.psi_in = DeviceArray("psi_in", self.shape, np.complex64, lambda: plan.get_psi(-1))
+
+    def load(self, modulus=None, support=None, ipsf=None, spectrum=None, numtrials=None):
+        """Load data, ``phasing.py:291-505``; keyword arguments only, any order.
+
+        modulus: corner-centred N x N Fourier modulus (N a power of two).
+        support: square array <= N x N, normalised to max 1; a smaller
+            support is embedded top-left (phasing.py:466-485).
+        numtrials: number of independent trials (savebuffer slots + 1).
+        """
+        types = (type(None), np.ndarray)
+        assert isinstance(modulus, types), "modulus must be ndarray if supplied"
+        assert isinstance(support, types), "support must be ndarray if supplied"
+        assert isinstance(ipsf, types), "psf must be ndarray if supplied"
+        if ipsf is not None or spectrum is not None:
+            raise NotImplementedError(
+                "partial-coherence (ipsf / spectrum) loading is not built yet")
+
+        if modulus is not None:
+            assert modulus.ndim == 2
+            modulus = modulus.astype(np.float32)
+            if self.modulus_state == 2:
+                assert modulus.shape == self.shape
+            else:
+                assert modulus.shape[0] == modulus.shape[1]
+                n = modulus.shape[0]
+                assert n & (n - 1) == 0, "modulus size must be a power of 2"
+                self.N, self.shape, self.size = n, modulus.shape, modulus.size
+                self._make_plan()
+            self._modulus_dev = to_device(modulus, self.ctx.device, torch.float32)
+            self._plan.set_modulus(self._modulus_dev, -1)
+            self.modulus_state = 2
+
+        if support is not None:
+            assert support.ndim == 2, "support is %s-dim" % support.ndim
+            assert support.shape[0] == support.shape[1], \
+                "support shape is %s %s, must be square" % support.shape
+            support = support.astype(np.float32)
+            support *= 1.0 / support.max()
+            if self.support_state == 2:
+                self.numtrial = 0
+            self._support_host = support
+            bounds = bounding_box(support, pad=10)
+            self.r0, self.rows = int(bounds[0]), int(bounds[1] - bounds[0])
+            self.c0, self.cols = int(bounds[2]), int(bounds[3] - bounds[2])
+            self.support_state = 1
+
+        if numtrials is not None:
+            self.numtrials = int(numtrials)
+            if self.buffer_state == 0:
+                self.buffer_state = 1
+
+        if self.modulus_state == 2 and self.support_state > 0:
+            sup = self._support_host
+            if sup.shape != self.shape:
+                assert self.rows <= self.shape[0]
+                assert self.cols <= self.shape[1]
+                crop = sup[self.r0:self.r0 + self.rows, self.c0:self.c0 + self.cols]
+                new = np.zeros(self.shape, np.float32)
+                new[:self.rows, :self.cols] = crop
+                bounds = bounding_box(new, force_to_square=True, pad=4)
+                self.r0, self.rows = int(bounds[0]), int(bounds[1] - bounds[0])
+                self.c0, self.cols = int(bounds[2]), int(bounds[3] - bounds[2])
+                sup = new
+                self._support_host = sup
+            if self.support_state == 1:
+                self._support_dev = to_device(sup, self.ctx.device, torch.float32)
+                self._plan.set_support(self._support_dev, -1)
+                plan = self._plan
+                self.support = DeviceArray("support", self.shape, np.float32,
+                                           lambda: plan.get_support(-1))
+            self.support_state = 2
+            if self.buffer_state > 0:
+                shape = (self.numtrials + 1, self.rows, self.cols)
+                if self.buffer_state == 1 or tuple(self.savebuffer.shape) != shape:
+                    buf = torch.zeros(shape, dtype=torch.complex64, device=self.ctx.device)
+                    self._savebuffer = buf
+                    self.savebuffer = DeviceArray("savebuffer", shape, np.complex64,
+                                                  lambda: self._savebuffer)
+                    self.buffer_state = 2
+
+        if self.modulus_state == 2 and self.support_state == 2 and self.buffer_state == 2:
+            self.can_iterate = True
+
+    # ------------------------------------------------------------------
+    def seed(self, val=None, supplied=None):
+        """Random start, ``phasing.py:990-1031``.
+
+        ``np.random.seed(val)`` if given, then U[0,1) + iU[0,1) (real part
+        drawn first) from numpy's global legacy generator, times the support.
+        ``supplied`` embeds a guess top-left and rolls it onto the support by
+        the argmax of the magnitude cross-correlation (conditioning.align_frames
+        with use_mag_only, phasing.py:1008-1024).
+        """
+        assert self.N is not None, "cannot seed without N being set. load data first."
+        assert self.support_state == 2, "load a support before seeding"
+        if val is not None:
+            np.random.seed(val)
+        if supplied is None:
+            guess = np.random.rand(self.N, self.N) + 1j * np.random.rand(self.N, self.N)
+            guess = guess.astype(np.complex64)
+            g = to_device(guess, self.ctx.device, torch.complex64)
+        else:
+            s = np.asarray(supplied)
+            assert s.ndim == 2
+            assert s.shape[0] <= self.N and s.shape[1] <= self.N
+            if s.shape != (self.N, self.N):
+                s2 = np.zeros(self.shape, np.complex64)
+                s2[:s.shape[0], :s.shape[1]] = s.astype(np.complex64)
+                s = s2
+            g = to_device(s.astype(np.complex64), self.ctx.device, torch.complex64)
+            g = self._align_to_support(g)
+        self._plan.set_psi(g, -1, mult_support=True)
+        self._seeded = True
+
+    def _align_to_support(self, g: torch.Tensor) -> torch.Tensor:
+        """Roll g so |g| best overlaps the support (device FFT cross-correlation).
+
+        crosscorr.alignment_coordinates (crosscorr.py:526-568): the roll is
+        minus the argmax of |IFFT(F|g| * conj(F support))| taken as a signed
+        cyclic shift.
+        """
+        n = self.N
+        mag = torch.abs(g).to(torch.complex64).contiguous()
+        ref = self._support_dev.to(torch.complex64).contiguous()
+        fa = self._plan.fft2(mag)
+        fb = self._plan.fft2(ref)
+        corr = self._plan.fft2((fa * torch.conj(fb)).contiguous(), inverse=True)
+        k = int(torch.argmax(torch.abs(corr)).item())
+        dr, dc = k // n, k % n
+        if dr > n // 2:
+            dr -= n
+        if dc > n // 2:
+            dc -= n
+        return torch.roll(g, shifts=(-dr, -dc), dims=(0, 1)).contiguous()
+
+    # ------------------------------------------------------------------
+    def iterate(self, iterations, order=None, silent=True, beta=0.8, debug_on=None,
+                shrinkwrap=None, semantics="numpy", error_trace=True):
+        """Run iterations, ``phasing.py:181-289``.
+
+        order: iterable of (name, count), names in hio/er/raar/sf/dm, cycled;
+            default (('hio', 99), ('er', 1)).
+        beta: feedback parameter for hio/raar/dm (default 0.8).
+        silent: True, False or an int (progress every int iterations).
+        shrinkwrap: None, True, or a dict(every=20, sigma0=3, sigma1=1.5,
+            tau=0.2): support *= [G_sigma * |psi| > tau max] after every
+            `every` iterations (build extension; the reference refines the
+            support between rounds with refine_support).
+        semantics: 'numpy' (reference CPU formulas) or 'opencl' (reference GPU
+            kernels) for ER and SF.
+        error_trace: record the modulus error of every iteration
+            (``get_error_trace``).
+        At the end the ROI of the iterate is copied to savebuffer[numtrial].
+        """
+        if debug_on:
+            warnings.warn("debug_on snapshot dumps are not supported; ignored")
+        if not (self.modulus_state == 2 and self.support_state == 2):
+            print("cant iterate before loading support and modulus.")
+            print("modulus state: " + str(self.modulus_state))
+            print("support state: " + str(self.support_state))
+            print("buffer  state: " + str(self.buffer_state))
+            raise AssertionError("cant iterate before loading support and modulus")
+        assert self._seeded, "seed() before iterate()"
+        assert semantics in _capi.SEM_CODES, "semantics must be 'numpy' or 'opencl'"
+        iterations = int(iterations)
+        algs = expand_order(order, iterations)
+        sw = {}
+        if shrinkwrap:
+            kw = dict(every=20, sigma0=3.0, sigma1=1.5, tau=0.2)
+            if isinstance(shrinkwrap, dict):
+                kw.update(shrinkwrap)
+            sw = shrinkwrap_schedule(iterations, **kw)
+        sem = _capi.SEM_CODES[semantics]
+        steps = []
+        for k, a in enumerate(algs):
+            sig, tau = sw.get(k, (0.0, 0.0))
+            steps.append((_capi.ALG_CODES[a], sem, beta, sig, tau))
+
+        # progress printing needs host sync points; otherwise one call
+        every = None
+        if silent is not True:
+            every = 1 if (silent is False or not isinstance(silent, int)) else max(1, int(silent))
+        traces = []
+        if every is None or iterations == 0:
+            if iterations:
+                traces.append(self._plan.iterate(steps, error_trace=error_trace))
+        else:
+            for start in range(0, iterations, every):
+                chunk = steps[start:start + every]
+                traces.append(self._plan.iterate(chunk, error_trace=error_trace))
+                done = start + len(chunk)
+                if done % every == 0:
+                    self.ctx.sync()
+                    print("  iteration %s" % done)
+                    sys.stdout.flush()
+        if error_trace and traces:
+            self._trace = torch.cat([t[:, 0] for t in traces if t is not None])
+        else:
+            self._trace = None
+
+        if self.buffer_state == 2:
+            assert self.numtrial < self.numtrials + 1, \
+                "numtrial %d exceeds the savebuffer (%d slots)" % (self.numtrial, self.numtrials + 1)
+            self._plan.copy_roi(0, self._savebuffer, self.numtrial, self.r0, self.c0,
+                                self.rows, self.cols)
+        self.numtrial += 1
+
+    def get_error_trace(self):
+        """FP64 modulus error of each iteration of the last ``iterate`` call.
+
+        E_k = sqrt(sum(|F psi_k| - m)^2 / sum m^2) for the iterate entering
+        iteration k (build-defined; the reference computes none).
+        """
+        return None if self._trace is None else self._trace.cpu().numpy()
+
+    # ------------------------------------------------------------------
+    def get(self, something):
+        """Host copy of a device array (``gpu.py:389-397``); other values pass through."""
+        if isinstance(something, DeviceArray):
+            return something.get()
+        if isinstance(something, torch.Tensor):
+            return something.detach().cpu().numpy()
+        return something
+
+    def status(self):
+        """Print the data status, ``phasing.py:1033-1044``."""
+        ro = {2: "complete", 1: "incomplete", 0: "nothing"}
+        print("states:")
+        print("  modulus %s %s" % (ro[self.modulus_state], self.shape))
+        print("  support %s" % ro[self.support_state])
+        print("  ipsf    %s" % ro[self.ipsf_state])
+        print("  buffer  %s" % ro[self.buffer_state])
+        print("  spectrm %s" % ro[self.spectrum_state])
+        print("  device  %s" % self.compute_device)
+
+    def optimize_coherence(self, *args, **kwargs):
+        raise NotImplementedError("partial-coherence optimisation is not built yet")
+
+
+_default_ctx = None
+
+
+def _context():
+    global _default_ctx
+    if _default_ctx is None:
+        _default_ctx = Context(0)
+    return _default_ctx
+
+
+def align_global_phase(data_in):
+    """Remove the global phase of each frame, ``phasing.py:1355-1404``.
+
+    Per frame the rotation exp(i p) minimising sum (Re sin p + Im cos p)^2,
+    then a sign flip so that sum Re >= 0.  The reference finds p with
+    ``fminbound``; here p is the closed-form minimiser (the smaller
+    eigenvector of the Gram matrix of (Re, Im)), computed on the GPU with
+    FP64 sums.  Returns a new array of the input's shape and dtype; the
+    rotation is applied in complex64.
+    """
+    data = np.asarray(data_in)
+    assert isinstance(data, np.ndarray), "data must be array"
+    assert data.ndim in (2, 3), "data must be 2d or 3d"
+    assert np.iscomplexobj(data), "data must be complex"
+    ctx = _context()
+    t = to_device(data.astype(np.complex64), ctx.device, torch.complex64)
+    t3 = t.unsqueeze(0) if t.dim() == 2 else t
+    ctx.align_global_phase(t3)
+    return t.cpu().numpy().astype(data.dtype)

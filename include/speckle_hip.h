@@ -1,0 +1,187 @@
+/*
+ * speckle_hip.h -- C ABI of the MI355X (gfx950) iterative phase-retrieval
+ * library.  Plain pointers and sizes only; every device pointer is a HIP
+ * device allocation owned by the caller (e.g. a torch ROCm tensor's
+ * data_ptr()), every host pointer is ordinary host memory.
+ *
+ * The reference has no C ABI: its operator boundary is the name-based kernel
+ * dispatch gpu.common._kexec (speckle/gpu.py:295-372) plus the third-party
+ * FFT plan pyfft.cl.Plan (built at speckle/phasing.py:372-374, executed at
+ * speckle/phasing.py:1074-1075).  Each entry point below names the reference
+ * interface it replaces.
+ *
+ * Return codes: 0 on success, a negative SPK_E* value on failure; the message
+ * of the last failure on the calling thread is spk_last_error().  No entry
+ * point exits or aborts the process (the reference calls exit() on fatal
+ * errors, speckle/gpu.py:366-370, speckle/phasing.py:252-254).
+ *
+ * Threading: a context owns one HIP stream; all work of its plans is queued
+ * on that stream in call order and is asynchronous unless stated.  Different
+ * contexts may be driven from different host threads.
+ */
+#ifndef SPECKLE_HIP_H
+#define SPECKLE_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  SPK_OK = 0,
+  SPK_EINVAL = -1,   /* bad argument (shape, size, pointer, enum)          */
+  SPK_EHIP = -2,     /* a HIP runtime call failed                            */
+  SPK_ENOMEM = -3,   /* device allocation failed                             */
+  SPK_ESTATE = -4,   /* call out of order (e.g. iterate before set_modulus)  */
+  SPK_EUNSUP = -5    /* size or option not built into this library           */
+};
+
+/* algorithms of phasing._iteration (speckle/phasing.py:1329-1331) */
+enum { SPK_HIO = 0, SPK_ER = 1, SPK_RAAR = 2, SPK_SF = 3, SPK_DM = 4,
+       SPK_NOOP = 15 /* frame idle for this step (serving) */ };
+
+/* ER / SF formulas: the reference's numpy branch (phasing.py:1256-1258,
+ * 1281-1284) or its OpenCL kernels (phasing_er.cl:13-15, phasing_sf.cl:16-17) */
+enum { SPK_SEM_NUMPY = 0, SPK_SEM_OPENCL = 1 };
+
+typedef struct spk_ctx spk_ctx;
+typedef struct spk_plan spk_plan;
+
+/* One iteration of one frame: algorithm, feedback beta, semantics, and an
+ * optional in-loop shrinkwrap after the update (sigma > 0 enables it). */
+typedef struct spk_step {
+  int32_t alg;        /* SPK_HIO ... SPK_DM, SPK_NOOP                      */
+  int32_t semantics;  /* SPK_SEM_NUMPY or SPK_SEM_OPENCL                    */
+  float beta;         /* iterate(beta=) of phasing.py:181                    */
+  float sw_sigma;     /* > 0: shrinkwrap after this iteration, Gaussian sigma */
+  float sw_tau;       /* shrinkwrap threshold as a fraction of the max blur  */
+  int32_t pad_;
+} spk_step;
+
+/* Kernel shape choices for a plan (autotuner output).  Zero = library default. */
+typedef struct spk_tune {
+  int32_t tile;       /* tile edge of the internal layout: 4 or 8            */
+  int32_t reserved[7];
+} spk_tune;
+
+/* -------------------------------------------------------------------------
+ * Context and device.  Replaces gpu.init / gpu.common.start
+ * (speckle/gpu.py:399-421, 491-558): binds one device and one stream.
+ * stream: a hipStream_t (e.g. torch.cuda.current_stream().cuda_stream), or
+ * NULL for the device's null stream.  The context does not own the stream.
+ * ---------------------------------------------------------------------- */
+int spk_ctx_create(int device, void* stream, spk_ctx** out);
+int spk_ctx_destroy(spk_ctx* ctx);
+int spk_ctx_set_stream(spk_ctx* ctx, void* stream);
+/* blocks until all work queued on the context's stream has finished */
+int spk_sync(spk_ctx* ctx);
+/* message of the last failed call on this host thread ("" if none) */
+const char* spk_last_error(void);
+/* library version string and the sizes built in (bitmask of log2 N) */
+const char* spk_version(void);
+uint32_t spk_supported_sizes(void);
+
+/* -------------------------------------------------------------------------
+ * Plans.  A plan holds `batch` frames of n x n (n a power of two, as pyfft
+ * requires, speckle/phasing.py:1469) resident on the device: iterate psi,
+ * its row transform, modulus, support, twiddles and workspace.  Replaces the
+ * buffers allocated by phasing.load (speckle/phasing.py:360-374) and the
+ * pyfft Plan made there.  shared != 0: one modulus and one support serve
+ * every frame (independent trials of one pattern); otherwise each frame has
+ * its own (serving slots).
+ * ---------------------------------------------------------------------- */
+int spk_plan_create(spk_ctx* ctx, int n, int batch, int shared,
+                    const spk_tune* tune, spk_plan** out);
+int spk_plan_destroy(spk_plan* plan);
+/* bytes of device memory the plan holds (for HBM-budget admission) */
+int spk_plan_bytes(int n, int batch, int shared, int with_dm, size_t* bytes);
+
+/* -------------------------------------------------------------------------
+ * Data in and out of a plan.  All array arguments are device pointers to
+ * row-major n x n arrays.  frame = -1 addresses every frame (for a shared
+ * plan the single modulus / support).
+ * ---------------------------------------------------------------------- */
+/* phasing.load(modulus=) data path (phasing.py:347-376); also computes the
+ * FP64 sum of m^2 used by the modulus error */
+int spk_set_modulus(spk_plan* plan, int frame, const float* modulus);
+/* phasing.load(support=) data path (phasing.py:389-398); the caller has
+ * normalised the support to max 1 as the reference does */
+int spk_set_support(spk_plan* plan, int frame, const float* support);
+int spk_get_support(spk_plan* plan, int frame, float* support);
+/* phasing.seed data path (phasing.py:1026-1031): psi = guess, times the
+ * support when mult_support != 0 (common_multiply_f_f2.cl:10-13) */
+int spk_set_psi(spk_plan* plan, int frame, const float* guess_c64,
+                int mult_support);
+/* phasing.get(psi_in) (gpu.py:389-397) */
+int spk_get_psi(spk_plan* plan, int frame, float* psi_c64);
+
+/* -------------------------------------------------------------------------
+ * The loop.  Replaces phasing.iterate's per-iteration work
+ * (speckle/phasing.py:260-268 -> _iteration, 1088-1353: pyfft forward,
+ * abs_f2_f2, fourier, pyfft inverse, and hio/er/raar/sf/dm1/dm2 kernels).
+ *
+ * steps: host array.  per_frame == 0: nsteps entries applied to every frame;
+ * per_frame != 0: nsteps * batch entries, steps[k * batch + f] for frame f.
+ * err_trace: device pointer to nsteps * batch doubles (may be NULL):
+ * err_trace[k * batch + f] = sqrt(sum(|F psi_k| - m)^2 / sum m^2) for the
+ * iterate psi_k entering step k, accumulated in FP64.
+ * ---------------------------------------------------------------------- */
+int spk_iterate(spk_plan* plan, const spk_step* steps, int nsteps,
+                int per_frame, double* err_trace);
+
+/* Shrinkwrap now, on every frame: support *= [G_sigma (*) |psi| > tau*max].
+ * Build-defined in-loop form of refine_support (phasing.py:1747-1815). */
+int spk_shrinkwrap(spk_plan* plan, int frame, float sigma, float tau);
+
+/* ROI copy into a savebuffer slot, phasing.iterate (phasing.py:279-287) ->
+ * phasing_copy_to_buffer_f2.cl:6-19.  dst: device c64 (slots, rows, cols),
+ * row-major; writes slot `slot`.  Bounds are checked (the kernel of the
+ * reference has none). */
+int spk_copy_roi(spk_plan* plan, int frame, float* dst_c64, int slots,
+                 int slot, int r0, int c0, int rows, int cols);
+
+/* -------------------------------------------------------------------------
+ * Standalone transform.  Replaces pyfft.cl.Plan.execute(data_in, data_out,
+ * inverse) (phasing.py:1074-1075): count frames of n x n c64, row-major,
+ * out-of-place or in-place.  Forward unnormalised; inverse scaled by 1/n^2
+ * (numpy's ifft2 convention, phasing.py:1078-1083).
+ * ---------------------------------------------------------------------- */
+int spk_fft2(spk_plan* plan, const float* in_c64, float* out_c64, int count,
+             int inverse);
+
+/* -------------------------------------------------------------------------
+ * Post-processing of trial stacks (module functions of speckle.phasing).
+ * data: device c64 (frames, rows, cols), row-major, modified in place.
+ * ---------------------------------------------------------------------- */
+/* align_global_phase (phasing.py:1355-1404), closed form, FP64 sums */
+int spk_align_global_phase(spk_ctx* ctx, float* data_c64, int frames,
+                           int rows, int cols);
+
+/* -------------------------------------------------------------------------
+ * Measurement helpers.
+ * ---------------------------------------------------------------------- */
+/* plain device copy of `bytes` (multiple of 16), for HBM counter calibration */
+int spk_copy_bytes(spk_ctx* ctx, void* dst, const void* src, size_t bytes);
+/* Per-pass timing inside spk_iterate: with enable != 0 every column-pass and
+ * row-pass launch of this plan is bracketed by HIP events on the context's
+ * stream.  spk_plan_profile_read synchronises, returns the number of launches
+ * and their summed milliseconds for pass 0 (column) or 1 (row), and clears
+ * that pass's record. */
+int spk_plan_profile(spk_plan* plan, int enable);
+int spk_plan_profile_read(spk_plan* plan, int pass, int* launches,
+                          double* total_ms);
+
+/* Launch `reps` column passes (which = 0), row passes (1) or both (2) of a
+ * single-projection step on the plan's current state, timed with HIP events
+ * on the context's stream; *ms_out = elapsed milliseconds.  Advances the
+ * plan's state like real iterations.  Used to time kernels. */
+int spk_time_passes(spk_plan* plan, int which, int reps, const spk_step* step,
+                    float* ms_out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SPECKLE_HIP_H */

@@ -1,0 +1,86 @@
+"""The drop-in ``phasing`` object on the GPU: the reference's call sequences
+(examples/basic_phasing_example.py:17-43, advanced_phasing_example.py:20-60)."""
+import numpy as np
+import pytest
+
+from als_speckle_amd import synth
+from als_speckle_amd.phasing import align_global_phase, phasing
+from oracle import phasing_c128 as O
+
+pytestmark = pytest.mark.gpu
+
+
+def test_seed_reproduces_numpy_legacy_draws():
+    f = synth.make_frame(128, 0, shape="disk", radius=20, photons=1e6)
+    r = phasing()
+    r.load(modulus=f.modulus, support=f.support, numtrials=2)
+    r.seed(17)
+    got = r.get(r.psi_in)
+    np.random.seed(17)
+    want = (np.random.rand(128, 128) + 1j * np.random.rand(128, 128)).astype(np.complex64) * f.support
+    assert np.array_equal(got, want.astype(np.complex64))
+
+
+def test_basic_example_flow():
+    f = synth.make_frame(128, 1, shape="disk", radius=20, photons=1e6)
+    r = phasing()
+    r.load(modulus=f.modulus)
+    r.load(support=f.support, numtrials=3)
+    assert r.can_iterate
+    assert (r.r0, r.rows) == tuple(int(v) for v in (O.bounding_box(f.support, pad=10)[0],
+                                                    np.diff(O.bounding_box(f.support, pad=10)[:2])[0]))
+    finals = []
+    for t in range(3):
+        r.seed()
+        r.iterate(40, order=(("hio", 30), ("er", 10)), beta=0.9)
+        finals.append(r.get(r.psi_in))
+        tr = r.get_error_trace()
+        assert tr.shape == (40,) and np.all(np.isfinite(tr))
+    buf = r.get(r.savebuffer)
+    assert buf.shape == (4, r.rows, r.cols)
+    for t in range(3):
+        assert np.array_equal(buf[t], finals[t][r.r0:r.r0 + r.rows, r.c0:r.c0 + r.cols])
+    al = align_global_phase(buf[:3])
+    want = O.align_global_phase(buf[:3].astype(np.complex128))
+    assert np.linalg.norm(al - want) / np.linalg.norm(want) < 1e-5
+    assert r.numtrial == 3
+    with pytest.raises(AssertionError):
+        r.seed()
+        r.iterate(1)  # fourth trial: savebuffer has numtrials + 1 = 4 slots -> ok
+        r.seed()
+        r.iterate(1)  # fifth: out of slots
+
+
+def test_reload_support_resets_trial_counter():
+    f = synth.make_frame(64, 2, shape="disk", radius=10, photons=1e6)
+    r = phasing()
+    r.load(modulus=f.modulus, support=f.support, numtrials=2)
+    r.seed(1)
+    r.iterate(5)
+    assert r.numtrial == 1
+    small = f.support[16:48, 16:48]
+    r.load(support=small)         # embedded top-left, square bbox pad 4
+    assert r.numtrial == 0
+    assert r.get(r.support).shape == (64, 64)
+
+
+def test_shrinkwrap_keyword_shrinks_support():
+    f = synth.make_frame(256, 3, shape="polygon", fill=0.1, photons=1e7)
+    r = phasing()
+    r.load(modulus=f.modulus, support=f.support, numtrials=1)
+    r.seed(2)
+    before = r.get(r.support).sum()
+    r.iterate(60, shrinkwrap=True)
+    after = r.get(r.support)
+    assert after.sum() <= before and np.all(after <= f.support)
+
+
+def test_progress_printing(capsys):
+    f = synth.make_frame(64, 4, shape="disk", radius=10, photons=1e6)
+    r = phasing()
+    r.load(modulus=f.modulus, support=f.support, numtrials=1)
+    r.seed(3)
+    r.iterate(10, silent=5)
+    out = capsys.readouterr().out
+    assert "iteration 5" in out and "iteration 10" in out
+    assert r.get_error_trace().shape == (10,)

@@ -1,0 +1,116 @@
+"""Host-side logic of the product, on the CPU: helpers of the drop-in API,
+the synthetic generators, and the C-ABI library's exports (no compute calls).
+"""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from als_speckle_amd import masking, synth
+from als_speckle_amd import phasing as P
+from als_speckle_amd import _capi
+from oracle import phasing_c128 as O
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_bounding_box_reference_vectors():
+    box = np.zeros((256, 256))
+    box[18:90, 45:63] = 1
+    assert tuple(masking.bounding_box(box)) == (18, 90, 45, 63)
+    assert tuple(masking.bounding_box(box, pad=14)) == (4, 104, 31, 77)
+
+
+def test_bounding_box_matches_oracle():
+    rng = np.random.default_rng(3)
+    for _ in range(200):
+        n = int(rng.integers(16, 64))
+        m = np.zeros((n, n))
+        r0, c0 = rng.integers(0, n - 2, 2)
+        m[r0:r0 + rng.integers(1, n - r0), c0:c0 + rng.integers(1, n - c0)] = 1
+        kw = dict(pad=int(rng.integers(0, 12)), force_to_square=bool(rng.integers(0, 2)),
+                  power_of_2=bool(rng.integers(0, 2)))
+        assert tuple(masking.bounding_box(m, **kw)) == tuple(O.bounding_box(m, **kw)), kw
+
+
+def test_find_all_objects_matches_oracle():
+    rng = np.random.default_rng(4)
+    for _ in range(20):
+        m = (rng.random((20, 20)) > 0.75).astype(float)
+        a = masking.find_all_objects(m)
+        b = O.find_all_objects(m)
+        assert a.shape == b.shape
+        assert np.array_equal(a, b)
+
+
+def test_expand_order_matches_oracle():
+    for order, it in [(None, 250), ((("hio", 5), ("er", 5), ("sf", 1)), 13),
+                      ((("raar", 3), ("dm", 2)), 17)]:
+        assert P.expand_order(order, it) == O.expand_order(order, it)
+    with pytest.raises(AssertionError):
+        P.expand_order((("xyz", 2),), 4)
+    with pytest.raises(AssertionError):
+        P.expand_order((("hio",),), 4)
+
+
+def test_shrinkwrap_schedule_matches_oracle():
+    sw = P.shrinkwrap_schedule(1000)
+    assert sorted(sw) == [20 * k - 1 for k in range(1, 51)]
+    sig = O.sigma_schedule(50)
+    assert [sw[k][0] for k in sorted(sw)] == pytest.approx(sig)
+
+
+def test_synth_frame_properties():
+    f = synth.make_frame(256, 0, fill=0.15, photons=1e6)
+    fill = f.true_support.mean()
+    assert 0.12 < fill < 0.18
+    assert f.modulus.dtype == np.float32 and f.modulus.shape == (256, 256)
+    assert abs(np.sum(f.modulus.astype(np.float64) ** 2) - 1e6) < 5 * np.sqrt(1e6)
+    assert np.all(f.support >= f.true_support)
+    d = synth.make_frame(256, 1, shape="disk", radius=40, photons=1e6)
+    assert abs(d.true_support.sum() - np.pi * 40 ** 2) < 0.01 * np.pi * 40 ** 2
+    # deterministic
+    g = synth.make_frame(256, 0, fill=0.15, photons=1e6)
+    assert np.array_equal(g.modulus, f.modulus)
+
+
+def _declared_symbols():
+    src = open(os.path.join(ROOT, "include", "speckle_hip.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(spk_[a-z0-9_]+)\s*\(", src)))
+
+
+def test_library_exports_every_declared_symbol():
+    if not os.path.exists(_capi.LIB_PATH):
+        pytest.fail("libspeckle_hip.so not built (run __graft_entry__.build())")
+    lib = ctypes.CDLL(_capi.LIB_PATH)
+    decl = _declared_symbols()
+    assert len(decl) >= 20
+    for name in decl:
+        assert hasattr(lib, name), name
+    assert set(decl) == set(_capi.SIGNATURES), "ctypes table and header disagree"
+    # host-only calls that need no GPU
+    lib.spk_version.restype = ctypes.c_char_p
+    assert b"gfx950" in lib.spk_version()
+    lib.spk_supported_sizes.restype = ctypes.c_uint32
+    assert lib.spk_supported_sizes() == sum(1 << k for k in range(4, 13))
+
+
+def test_library_rejects_bad_arguments_without_gpu():
+    lib = _capi.load()
+    out = ctypes.c_size_t()
+    assert lib.spk_plan_bytes(1000, 1, 1, 0, ctypes.byref(out)) == -1   # not a power of 2
+    assert "bad plan size" in _capi.last_error()
+    assert lib.spk_plan_bytes(1024, 1, 1, 0, ctypes.byref(out)) == 0
+    assert out.value >= 1024 * 1024 * (16 + 8)
+
+
+def test_phasing_has_no_cpu_path():
+    with pytest.raises(RuntimeError):
+        P.phasing(force_cpu=True)
+    import torch
+    if not torch.cuda.is_available():
+        with pytest.raises(RuntimeError, match="GPU"):
+            P.phasing()

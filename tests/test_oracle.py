@@ -1,0 +1,299 @@
+"""Pins the complex128 oracle (oracle/phasing_c128.py) on the CPU.
+
+Each check is chosen so that a plausible mistake in the oracle (a dropped
+term, a wrong sign or index, a transposed operand) fails it: reference test
+vectors where the reference has them, otherwise closed forms, invariants,
+fixed points and brute force.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import phasing_c128 as O
+from oracle.ref_numpy import RefNumpyPhasing
+
+RNG = np.random.default_rng(1234)
+
+
+def crand(shape, rng=RNG):
+    return rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+
+
+# --------------------------------------------------------------------- FFT
+def dft_matrix(n):
+    k = np.arange(n)
+    return np.exp(-2j * np.pi * np.outer(k, k) / n)
+
+
+@pytest.mark.parametrize("n", [2, 4, 8, 16])
+def test_fft2_matches_brute_force_dft(n):
+    x = crand((n, n))
+    W = dft_matrix(n)
+    ref = W @ x @ W.T
+    assert np.allclose(O.fft2(x), ref, atol=1e-10 * n * n)
+    inv = np.conj(W) @ x @ np.conj(W).T / (n * n)
+    assert np.allclose(O.ifft2(x), inv, atol=1e-12 * n * n)
+
+
+def test_fft2_closed_forms():
+    n = 32
+    d = np.zeros((n, n), complex)
+    d[0, 0] = 1
+    assert np.allclose(O.fft2(d), 1)               # impulse -> constant
+    assert np.allclose(O.fft2(np.ones((n, n))), n * n * d)  # constant -> delta
+    k0, k1 = 3, 5
+    yy, xx = np.indices((n, n))
+    e = np.exp(2j * np.pi * (k0 * yy + k1 * xx) / n)
+    F = O.fft2(e)
+    expect = np.zeros((n, n), complex)
+    expect[k0, k1] = n * n                          # exponential -> delta
+    assert np.allclose(F, expect, atol=1e-9)
+    x = crand((n, n))
+    # shift theorem: roll by (a, b) multiplies by exp(-2 pi i (ka a + kb b)/n)
+    a, b = 4, 7
+    ph = np.exp(-2j * np.pi * (yy * a + xx * b) / n)
+    assert np.allclose(O.fft2(np.roll(x, (a, b), (0, 1))), O.fft2(x) * ph)
+    # Parseval and round trip
+    assert math.isclose(np.sum(np.abs(O.fft2(x)) ** 2), n * n * np.sum(np.abs(x) ** 2), rel_tol=1e-12)
+    assert np.allclose(O.ifft2(O.fft2(x)), x)
+
+
+# -------------------------------------------------------------- bounding box
+def test_bounding_box_reference_vectors():
+    # test/test_mask.py:19-31
+    box = np.zeros((256, 256))
+    box[18:90, 45:63] = 1
+    assert tuple(O.bounding_box(box)) == (18, 90, 45, 63)
+    assert tuple(O.bounding_box(box, pad=14)) == (4, 104, 31, 77)
+
+
+def test_bounding_box_brute_force():
+    rng = np.random.default_rng(5)
+    for _ in range(50):
+        m = np.zeros((40, 50))
+        r0, c0 = rng.integers(0, 30), rng.integers(0, 40)
+        m[r0:r0 + rng.integers(1, 10), c0:c0 + rng.integers(1, 10)] = rng.uniform(0.1, 1)
+        rr, cc = np.nonzero(m > 1e-10)
+        pad = int(rng.integers(0, 6))
+        exp = (max(rr.min() - pad, 0), min(rr.max() + 1 + pad, 40),
+               max(cc.min() - pad, 0), min(cc.max() + 1 + pad, 50))
+        assert tuple(O.bounding_box(m, pad=pad)) == exp
+
+
+def test_bounding_box_square_and_p2():
+    m = np.zeros((128, 128))
+    m[40:61, 50:58] = 1                      # 21 x 8
+    r0, r1, c0, c1 = O.bounding_box(m, force_to_square=True)
+    assert r1 - r0 == c1 - c0 == 22
+    r0, r1, c0, c1 = O.bounding_box(m, power_of_2=True)
+    assert (r1 - r0, c1 - c0) == (32, 8)
+
+
+# ----------------------------------------------------------- P_M, updates
+def known_object(n=32, seed=3):
+    rng = np.random.default_rng(seed)
+    yy, xx = np.indices((n, n))
+    s = ((yy - n / 2) ** 2 + (xx - n / 2) ** 2 < (n / 5) ** 2).astype(float)
+    obj = s * rng.uniform(0.5, 1, (n, n)) * np.exp(1j * rng.uniform(-np.pi, np.pi, (n, n)))
+    m = np.abs(np.fft.fft2(obj))
+    return obj, s, m
+
+
+def test_fourier_constraint_properties():
+    n = 32
+    x = crand((n, n))
+    m = np.abs(crand((n, n)))
+    out, Psi = O.fourier_constraint(x, m)
+    F = O.fft2(out)
+    assert np.allclose(np.abs(F), m)                              # |P_M| = m
+    assert np.allclose(np.angle(F), np.angle(Psi))                 # phase kept
+    out2, _ = O.fourier_constraint(out, m)
+    assert np.allclose(out2, out)                                  # idempotent
+    # zero guard: where Psi = 0 the replaced value is 0, not NaN
+    z = np.zeros((n, n), complex)
+    o, _ = O.fourier_constraint(z, m)
+    assert np.all(np.isfinite(o)) and np.allclose(o, 0)
+
+
+@pytest.mark.parametrize("sem", ["numpy", "opencl"])
+def test_update_special_cases(sem):
+    n = 8
+    pin, pout, t = crand((n, n)), crand((n, n)), crand((n, n))
+    one, zero = np.ones((n, n)), np.zeros((n, n))
+    b = 0.7
+    # inside the support every non-ER update returns psi_out (ER-numpy: psi_in)
+    for alg in ("hio", "raar", "sf"):
+        assert np.allclose(O.update(alg, pin, pout, one, b, sem), pout)
+    assert np.allclose(O.update("er", pin, pout, one, b, sem), pin if sem == "numpy" else pout)
+    # outside
+    assert np.allclose(O.update("hio", pin, pout, zero, b, sem), pin - b * pout)
+    assert np.allclose(O.update("raar", pin, pout, zero, b, sem), b * pin + (1 - 2 * b) * pout)
+    assert np.allclose(O.update("er", pin, pout, zero, b, sem), 0)
+    assert np.allclose(O.update("sf", pin, pout, zero, b, sem), -pin if sem == "numpy" else -pout)
+    # DM: inside (1+b) psi_out + psi_in - b t, outside psi_in - b psi_out
+    assert np.allclose(O.update("dm", pin, pout, one, b, sem, t), (1 + b) * pout + pin - b * t)
+    assert np.allclose(O.update("dm", pin, pout, zero, b, sem, t), pin - b * pout)
+    # HIO with beta = 0 outside keeps psi_in
+    assert np.allclose(O.update("hio", pin, pout, zero, 0.0, sem), pin)
+    # DM modified estimate
+    assert np.allclose(O.dm_modified(pin, one, b), pin)
+    assert np.allclose(O.dm_modified(pin, zero, b), -pin / b)
+
+
+@pytest.mark.parametrize("sem", ["numpy", "opencl"])
+@pytest.mark.parametrize("alg", ["hio", "er", "raar", "sf"])
+def test_known_object_is_fixed_point(alg, sem):
+    obj, s, m = known_object()
+    new, pout, err = O.iteration(obj, m, s, alg, 0.9, sem)
+    assert np.allclose(pout, obj, atol=1e-12)
+    assert np.allclose(new, obj, atol=1e-12)
+    assert err < 1e-13
+
+
+def test_dm_solution_doubles_inside_support():
+    obj, s, m = known_object()
+    new, _, _ = O.iteration(obj, m, s, "dm", 0.7)
+    assert np.allclose(new, 2 * obj, atol=1e-12)
+
+
+def test_modulus_error_closed_form():
+    Psi = np.array([[3 + 4j, 0], [1, -2j]])
+    m = np.array([[4.0, 1.0], [1.0, 1.0]])
+    # |Psi| = 5, 0, 1, 2 ; diffs 1, -1, 0, 1 ; sum m^2 = 19
+    assert math.isclose(O.modulus_error(Psi, m), math.sqrt(3 / 19), rel_tol=1e-15)
+
+
+def test_expand_order_reference_example():
+    # phasing.py:203-213
+    seq = O.expand_order((("hio", 5), ("er", 5), ("sf", 1)), 13)
+    assert seq == ["hio"] * 5 + ["er"] * 5 + ["sf", "hio", "hio"]
+    assert O.expand_order(None, 101) == ["hio"] * 99 + ["er"] + ["hio"]
+
+
+def test_seed_matches_numpy_legacy_global():
+    n = 16
+    s = (np.arange(n * n).reshape(n, n) % 3 == 0).astype(np.float32)
+    g = O.seed_guess(n, 42, s)
+    np.random.seed(42)
+    ref = (np.random.rand(n, n) + 1j * np.random.rand(n, n)).astype(np.complex64) * s
+    assert np.array_equal(g, ref.astype(np.complex128))
+
+
+# ---------------------------------------------------------------- shrinkwrap
+def test_gaussian_kernel_and_blur():
+    n = 64
+    k = O.gaussian_kernel((n, n), 2.0)
+    assert math.isclose(k.sum(), 1.0, rel_tol=1e-14)
+    assert np.unravel_index(k.argmax(), k.shape) == (0, 0)
+    assert np.allclose(O.blur_fft(np.full((n, n), 3.0), 2.0), 3.0)
+    d = np.zeros((n, n))
+    d[0, 0] = 1
+    assert np.allclose(O.blur_fft(d, 2.0), k, atol=1e-14)
+    # separable direct periodic convolution gives the same blur
+    x = np.abs(crand((n, n)))
+    g = np.exp(-((np.arange(n) - n // 2) ** 2) / (2 * 2.0 ** 2))
+    g = np.fft.ifftshift(g) / g.sum()
+    rows = np.array([[np.sum(g * np.roll(x[r], -c)) for c in range(n)] for r in range(n)])
+    both = np.array([[np.sum(g * np.roll(rows[:, c], -r)) for c in range(n)] for r in range(n)]).T
+    assert np.allclose(O.blur_fft(x, 2.0), both.T, atol=1e-12)
+
+
+def test_shrinkwrap_support_invariants():
+    obj, s, m = known_object(64)
+    psi = obj + 0.05 * crand(obj.shape)
+    new, b = O.shrinkwrap_support(psi, s, 1.5, 0.2)
+    assert np.all(new <= s)
+    keep0, _ = O.shrinkwrap_support(psi, s, 1.5, 0.0)
+    assert np.array_equal(keep0, s * (b > 0))
+    assert np.array_equal(new, s * (b > 0.2 * b.max()))
+
+
+def test_sigma_schedule():
+    sig = O.sigma_schedule(50)
+    assert sig[0] == 3.0 and math.isclose(sig[-1], 1.5) and len(sig) == 50
+    assert all(a > b for a, b in zip(sig, sig[1:]))
+
+
+def test_find_all_objects_matches_label():
+    from scipy import ndimage
+    rng = np.random.default_rng(9)
+    for _ in range(5):
+        m = (rng.random((24, 24)) > 0.8).astype(float)
+        m[0, :] = m[-1, :] = m[:, 0] = m[:, -1] = 0   # no wrap-around contacts
+        parts = O.find_all_objects(m)
+        lab, n = ndimage.label(m, structure=np.ones((3, 3)))
+        big = [lab == i for i in range(1, n + 1) if (lab == i).sum() > 1]
+        got = [p.astype(bool) for p in parts if p.sum() > 0]
+        assert len(got) == len(big)
+        for p in got:
+            assert any(np.array_equal(p, q) for q in big)
+        # isolated pixels give empty parts (flood fill never marks the start)
+        n_iso = sum(1 for i in range(1, n + 1) if (lab == i).sum() == 1)
+        assert sum(1 for p in parts if p.sum() == 0) == n_iso
+
+
+def test_find_all_objects_wraps():
+    m = np.zeros((10, 10))
+    m[0, 3:5] = 1
+    m[9, 3:5] = 1                 # touches row 0 across the edge
+    parts = O.find_all_objects(m)
+    assert len(parts) == 1 and parts[0].sum() == 4
+
+
+def test_refine_support_invariants():
+    obj, s, m = known_object(64)
+    avg = obj + 0.1 * crand(obj.shape)
+    ref, blurred = O.refine_support(s, avg, blur=2, local_threshold=0.2, global_threshold=0.1)
+    assert np.all(ref <= s)                                     # shrink only
+    keep, _ = O.refine_support(s, avg, blur=2, local_threshold=0.0, global_threshold=0.0)
+    assert np.array_equal(keep, s * (blurred > 0))
+    const, _ = O.refine_support(s, np.ones_like(s), blur=2, local_threshold=0.5)
+    assert np.array_equal(const, s)
+
+
+# ------------------------------------------------------------- align phase
+def test_align_closed_form_matches_fminbound():
+    rng = np.random.default_rng(11)
+    data = crand((6, 20, 24), rng) + (rng.standard_normal((6, 1, 1)) * 3)
+    data = data * np.exp(1j * rng.uniform(0, 2 * np.pi, (6, 1, 1)))
+    a = O.align_global_phase(data)
+    b = O.align_global_phase_closed(data)
+    assert np.allclose(a, b, atol=1e-7 * np.abs(data).max())
+    for f in b:
+        assert f.real.sum() >= 0
+        # no rotation lowers the imaginary energy further
+        for p in np.linspace(0, 2 * np.pi, 37):
+            assert np.sum((f * np.exp(1j * p)).imag ** 2) >= np.sum(f.imag ** 2) - 1e-9
+
+
+def test_align_is_rotation_invariant():
+    rng = np.random.default_rng(12)
+    f = crand((16, 16), rng) + 2
+    a = O.align_global_phase_closed(f)
+    b = O.align_global_phase_closed(f * np.exp(1j * 1.234))
+    assert np.allclose(a, b, atol=1e-12)
+
+
+# ------------------------------------------------------------- prtf / rftf
+def test_prtf_rftf_closed_forms():
+    obj, s, m = known_object(32)
+    frames = np.stack([obj, obj, obj])
+    assert np.allclose(O.prtf(frames, 32), 1.0)
+    rng = np.random.default_rng(4)
+    rnd = np.stack([crand((32, 32), rng) for _ in range(400)])
+    assert abs(O.prtf(rnd, 32).mean() - math.sqrt(math.pi / 4 / 400)) < 0.01
+    assert np.allclose(O.rftf(obj, m + 1e-30), 1.0)
+
+
+# ------------------------------------------------ restated reference path
+def test_ref_numpy_one_iteration_matches_oracle():
+    obj, s, m = known_object(32)
+    rng = np.random.default_rng(2)
+    psi = (s * (rng.random((32, 32)) + 1j * rng.random((32, 32)))).astype(np.complex64)
+    for alg in ("hio", "er", "raar", "sf", "dm"):
+        r = RefNumpyPhasing(m, s, beta=0.8)
+        r.psi_in = psi.copy()
+        r.iteration(alg)
+        ref, _, _ = O.iteration(psi.astype(np.complex128), m.astype(np.float32), s, alg, 0.8)
+        assert np.linalg.norm(r.psi_in - ref) / np.linalg.norm(ref) < 1e-5, alg
