@@ -60,12 +60,25 @@ void dfree(T*& p) {
 
 }  // namespace
 
+// A context is reference counted: spk_ctx_create holds one reference and
+// every plan holds one, so a context outlives its plans whatever order the
+// caller destroys them in.
 struct spk_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   double* scratch = nullptr;   // small reduction workspace (align)
   size_t scratch_cap = 0;
+  int refs = 1;
 };
+
+static void ctx_release(spk_ctx* ctx) {
+  if (--ctx->refs > 0) return;
+  (void)hipSetDevice(ctx->device);
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  else (void)hipDeviceSynchronize();
+  if (ctx->scratch) (void)hipFree(ctx->scratch);
+  delete ctx;
+}
 
 struct spk_plan {
   spk_ctx* ctx = nullptr;
@@ -270,11 +283,7 @@ int spk_ctx_create(int device, void* stream, spk_ctx** out) {
 
 int spk_ctx_destroy(spk_ctx* ctx) {
   if (!ctx) return SPK_OK;
-  (void)hipSetDevice(ctx->device);
-  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
-  else (void)hipDeviceSynchronize();
-  dfree(ctx->scratch);
-  delete ctx;
+  ctx_release(ctx);
   return SPK_OK;
 }
 
@@ -319,6 +328,7 @@ int spk_plan_create(spk_ctx* ctx, int n, int batch, int shared, const spk_tune* 
                   " not built (sizes 16..4096; 4096 needs tile 4)");
   spk_plan* p = new spk_plan;
   p->ctx = ctx;
+  ctx->refs++;
   p->n = n;
   p->batch = batch;
   p->shared = shared ? 1 : 0;
@@ -365,6 +375,7 @@ int spk_plan_destroy(spk_plan* p) {
   if (p->htab) (void)hipHostFree(p->htab);
   if (p->hact) (void)hipHostFree(p->hact);
   if (p->upload_done) (void)hipEventDestroy(p->upload_done);
+  if (p->ctx) ctx_release(p->ctx);
   delete p;
   return SPK_OK;
 }

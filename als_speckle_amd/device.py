@@ -5,7 +5,10 @@ into ``libspeckle_hip.so``.  Nothing here computes on the host.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes
+import sys
+import weakref
 from ctypes import byref, c_float, c_size_t, c_void_p
 
 import numpy as np
@@ -13,6 +16,19 @@ import torch
 
 from . import _capi
 from ._capi import check
+
+
+_live_plans: "weakref.WeakSet[Plan]" = weakref.WeakSet()
+_live_ctxs: "weakref.WeakSet[Context]" = weakref.WeakSet()
+
+
+@atexit.register
+def _close_all() -> None:
+    # release device objects while the HIP runtime is still up
+    for p in list(_live_plans):
+        p.close()
+    for c in list(_live_ctxs):
+        c.close()
 
 
 def require_gpu() -> None:
@@ -51,6 +67,7 @@ class Context:
         h = c_void_p()
         check(self.lib.spk_ctx_create(self.index, c_void_p(self.stream.cuda_stream), byref(h)))
         self.handle = h
+        _live_ctxs.add(self)
 
     def sync(self) -> None:
         check(self.lib.spk_sync(self.handle))
@@ -74,6 +91,8 @@ class Context:
             self.handle = None
 
     def __del__(self):
+        if sys.is_finalizing():
+            return
         try:
             self.close()
         except Exception:
@@ -99,6 +118,7 @@ class Plan:
         check(self.lib.spk_plan_create(ctx.handle, self.n, self.batch, int(self.shared),
                                        byref(tune), byref(h)))
         self.handle = h
+        _live_plans.add(self)
 
     # -- helpers ---------------------------------------------------------
     def _frames_shape(self, frame, per_frame):
@@ -193,6 +213,8 @@ class Plan:
             self.handle = None
 
     def __del__(self):
+        if sys.is_finalizing():
+            return
         try:
             self.close()
         except Exception:

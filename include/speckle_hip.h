@@ -73,6 +73,8 @@ typedef struct spk_tune {
  * NULL for the device's null stream.  The context does not own the stream.
  * ---------------------------------------------------------------------- */
 int spk_ctx_create(int device, void* stream, spk_ctx** out);
+/* releases the caller's reference; plans hold their own, so the context is
+ * freed when it and all its plans have been destroyed, in any order */
 int spk_ctx_destroy(spk_ctx* ctx);
 int spk_ctx_set_stream(spk_ctx* ctx, void* stream);
 /* blocks until all work queued on the context's stream has finished */

@@ -300,3 +300,16 @@ def test_align_global_phase_matches_fminbound(gpu_ctx):
     gpu_ctx.align_global_phase(t)
     want = O.align_global_phase(data.astype(np.complex128))
     assert rel(t.cpu().numpy(), want) < 1e-5
+
+
+def test_context_outlives_plans():
+    """Destroying a context before its plans is safe (plans hold a reference)."""
+    from als_speckle_amd.device import Context
+    c = Context(0)
+    p = Plan(c, 64)
+    q = Plan(c, 32)
+    c.close()
+    f = frame(64)
+    p.set_modulus(dev(f.modulus, torch.float32))
+    p.close()
+    q.close()
