@@ -26,7 +26,7 @@ class SpkStep(ctypes.Structure):
 
 
 class SpkTune(ctypes.Structure):
-    _fields_ = [("tile", c_int32), ("reserved", c_int32 * 7)]
+    _fields_ = [("tile", c_int32), ("ept", c_int32), ("reserved", c_int32 * 6)]
 
 
 class SpkError(RuntimeError):
@@ -60,6 +60,7 @@ SIGNATURES = {
     "spk_fft2": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_int]),
     "spk_align_global_phase": (c_int, [c_void_p, c_void_p, c_int, c_int, c_int]),
     "spk_copy_bytes": (c_int, [c_void_p, c_void_p, c_void_p, c_size_t]),
+    "spk_stack_mean": (c_int, [c_void_p, c_void_p, c_int, c_int, c_int, c_void_p]),
     "spk_time_passes": (c_int, [c_void_p, c_int, c_int, POINTER(SpkStep), POINTER(c_float)]),
     "spk_plan_profile": (c_int, [c_void_p, c_int]),
     "spk_plan_profile_read": (c_int, [c_void_p, c_int, POINTER(c_int), POINTER(c_double)]),

@@ -36,7 +36,9 @@ int fail(int code, const std::string& msg) {
 
 bool is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
 
-int default_tile(int n) { return n >= 4096 ? 4 : 8; }
+int default_tile(int n) { (void)n; return 4; }
+// 8 values per thread (more threads, shorter chains) wherever it fits
+int default_ept(int n, int tile) { return (n / 8) * tile <= 1024 ? 8 : 16; }
 
 template <class T>
 int dalloc(T** p, size_t count) {
@@ -82,7 +84,7 @@ static void ctx_release(spk_ctx* ctx) {
 
 struct spk_plan {
   spk_ctx* ctx = nullptr;
-  int n = 0, batch = 0, shared = 0, tile = 8, bands = 0;
+  int n = 0, batch = 0, shared = 0, tile = 8, ept = 16, bands = 0;
   c32* psi = nullptr;
   c32* phi = nullptr;
   c32* phib = nullptr;
@@ -98,16 +100,16 @@ struct spk_plan {
   float* sw_maxp = nullptr;
   c32* fftbuf = nullptr;
   size_t fftbuf_frames = 0;
-  Coef* dtab = nullptr;       // per-frame coefficient table [nsteps][batch]
-  int* dact = nullptr;        // per-frame activity [nsteps][batch]
-  size_t dtab_cap = 0;        // entries
-  Coef* htab = nullptr;       // pinned staging for dtab
-  int* hact = nullptr;
+  // per-call tables (coefficients, activity, frame maps, shrinkwrap
+  // parameters) staged in pinned memory and copied in one transfer
+  unsigned char* stage_d = nullptr;
+  unsigned char* stage_h = nullptr;
+  size_t stage_cap = 0;
   hipEvent_t upload_done = nullptr;
+  bool upload_pending = false;
   bool profiling = false;
   std::vector<hipEvent_t> prof_ev[2];  // start/stop pairs per pass
   bool mod_set = false, sup_set = false, psi_set = false;
-  bool phi_valid = false;     // phi == rowFFT(psi)
   bool phib_fresh = false;    // phib == rowFFT(dm1(psi)) for the next DM step
   float phib_g = 0.0f;
   long long nn() const { return (long long)n * n; }
@@ -213,11 +215,48 @@ int ensure_sw(spk_plan* p) {
   return SPK_OK;
 }
 
-int refresh_phi(spk_plan* p) {
-  if (p->phi_valid) return SPK_OK;
-  SPK_HIP(launch_rowfft(p->n, p->tile, 0, p->psi, p->phi, nullptr, 0, 0.0f, p->tw, p->batch,
-                        p->ctx->stream));
-  p->phi_valid = true;
+// phi = rowFFT(psi) for frames [f0, f0 + nf)
+int rowfft_frames(spk_plan* p, int f0, int nf) {
+  const size_t fo = (size_t)f0 * p->nn();
+  SPK_HIP(launch_rowfft(p->n, p->tile, p->ept, 0, p->psi + fo, p->phi + fo, nullptr, 0, 0.0f,
+                        p->tw, nullptr, nf, p->ctx->stream));
+  return SPK_OK;
+}
+
+// Host-side builder of one staged upload: append arrays, then copy once.
+struct Stage {
+  std::vector<unsigned char> buf;
+  template <class T>
+  size_t add(const T* v, size_t count) {
+    size_t off = (buf.size() + 255) & ~size_t(255);
+    buf.resize(off + count * sizeof(T));
+    if (count) std::memcpy(buf.data() + off, v, count * sizeof(T));
+    return off;
+  }
+};
+
+int upload_stage(spk_plan* p, const Stage& stg) {
+  const size_t need = stg.buf.size();
+  if (need == 0) return SPK_OK;
+  if (p->upload_pending) SPK_HIP(hipEventSynchronize(p->upload_done));
+  p->upload_pending = false;
+  if (need > p->stage_cap) {
+    // queued kernels of earlier calls may still read the old tables
+    SPK_HIP(hipStreamSynchronize(p->ctx->stream));
+    dfree(p->stage_d);
+    if (p->stage_h) (void)hipHostFree(p->stage_h);
+    p->stage_h = nullptr;
+    p->stage_cap = 0;
+    const size_t cap = std::max<size_t>(need, 64 * 1024);
+    int rc = dalloc(&p->stage_d, cap);
+    if (rc) return rc;
+    SPK_HIP(hipHostMalloc((void**)&p->stage_h, cap, hipHostMallocDefault));
+    p->stage_cap = cap;
+  }
+  std::memcpy(p->stage_h, stg.buf.data(), need);
+  SPK_HIP(hipMemcpyAsync(p->stage_d, p->stage_h, need, hipMemcpyHostToDevice, p->ctx->stream));
+  SPK_HIP(hipEventRecord(p->upload_done, p->ctx->stream));
+  p->upload_pending = true;
   return SPK_OK;
 }
 
@@ -236,16 +275,24 @@ void prof_clear(spk_plan* p, int pass) {
   p->prof_ev[pass].clear();
 }
 
+int make_swparam(int n, float sigma, float tau, SwParam* sp) {
+  SPK_REQUIRE(tau >= 0.0f && tau <= 1.0f, SPK_EINVAL, "shrinkwrap tau must be in [0, 1]");
+  int rc = make_blur(n, sigma, &sp->w);
+  if (rc) return rc;
+  sp->tau = tau;
+  return SPK_OK;
+}
+
 int do_shrinkwrap(spk_plan* p, int frame0, int frames, float sigma, float tau) {
   SPK_REQUIRE(!(p->shared && p->batch > 1), SPK_EUNSUP,
               "shrinkwrap needs a plan with one support per frame");
-  SPK_REQUIRE(tau >= 0.0f && tau <= 1.0f, SPK_EINVAL, "shrinkwrap tau must be in [0, 1]");
-  BlurW w;
-  int rc = make_blur(p->n, sigma, &w);
+  SwParam sp;
+  int rc = make_swparam(p->n, sigma, tau, &sp);
   if (rc) return rc;
   if ((rc = ensure_sw(p))) return rc;
   SPK_HIP(launch_shrinkwrap(p->n, p->tile, p->psi, p->sup, p->mstride(), p->sw_tmp,
-                            p->sw_blur, p->sw_maxp, w, tau, frame0, frames, p->ctx->stream));
+                            p->sw_blur, p->sw_maxp, nullptr, sp, nullptr, frame0, frames,
+                            p->ctx->stream));
   p->phib_fresh = false;
   return SPK_OK;
 }
@@ -323,9 +370,12 @@ int spk_plan_create(spk_ctx* ctx, int n, int batch, int shared, const spk_tune* 
   SPK_REQUIRE(is_pow2(n), SPK_EINVAL, "n must be a power of two (got " + std::to_string(n) + ")");
   SPK_REQUIRE(batch >= 1, SPK_EINVAL, "batch must be >= 1");
   int tile = (tune && tune->tile) ? tune->tile : default_tile(n);
-  SPK_REQUIRE(size_supported(n, tile), SPK_EUNSUP,
+  int ept = (tune && tune->ept) ? tune->ept : default_ept(n, tile);
+  SPK_REQUIRE(size_supported(n, tile, ept), SPK_EUNSUP,
               "frame size " + std::to_string(n) + " with tile " + std::to_string(tile) +
-                  " not built (sizes 16..4096; 4096 needs tile 4)");
+                  " and " + std::to_string(ept) +
+                  " values per thread not built (sizes 16..4096, tile 4 or 8, 8 or 16 "
+                  "values per thread, at most 1024 threads per workgroup, no tile 8 at 4096)");
   spk_plan* p = new spk_plan;
   p->ctx = ctx;
   ctx->refs++;
@@ -333,6 +383,7 @@ int spk_plan_create(spk_ctx* ctx, int n, int batch, int shared, const spk_tune* 
   p->batch = batch;
   p->shared = shared ? 1 : 0;
   p->tile = tile;
+  p->ept = ept;
   p->bands = n / tile;
   const size_t nn = (size_t)n * n;
   const size_t fms = shared ? 1 : (size_t)batch;
@@ -369,11 +420,10 @@ int spk_plan_destroy(spk_plan* p) {
   dfree(p->psi); dfree(p->phi); dfree(p->phib); dfree(p->mod); dfree(p->sup);
   dfree(p->tw); dfree(p->summ2); dfree(p->sumpart); dfree(p->errp);
   dfree(p->sw_tmp); dfree(p->sw_blur); dfree(p->sw_maxp); dfree(p->fftbuf);
-  dfree(p->dtab); dfree(p->dact);
+  dfree(p->stage_d);
   prof_clear(p, 0);
   prof_clear(p, 1);
-  if (p->htab) (void)hipHostFree(p->htab);
-  if (p->hact) (void)hipHostFree(p->hact);
+  if (p->stage_h) (void)hipHostFree(p->stage_h);
   if (p->upload_done) (void)hipEventDestroy(p->upload_done);
   if (p->ctx) ctx_release(p->ctx);
   delete p;
@@ -405,7 +455,7 @@ int spk_set_modulus(spk_plan* p, int frame, const float* m) {
   int f0, nf;
   if ((rc = frame_range(p, frame, !p->shared, &f0, &nf))) return rc;
   hipStream_t st = p->ctx->stream;
-  SPK_HIP(launch_pack_f32(p->n, p->tile, p->mod + (size_t)f0 * p->nn(), m, nf, st));
+  SPK_HIP(launch_pack_mod(p->n, p->tile, p->mod + (size_t)f0 * p->nn(), m, nf, st));
   const int nblk = 64;
   SPK_HIP(launch_sumsq(p->n, p->mod + (size_t)f0 * p->nn(), p->nn(), p->sumpart, nblk,
                        p->summ2 + f0, nf, st));
@@ -449,8 +499,10 @@ int spk_set_psi(spk_plan* p, int frame, const float* guess, int mult_support) {
   const float* sup = mult_support ? p->sup + (size_t)(p->shared ? 0 : f0) * p->nn() : nullptr;
   SPK_HIP(launch_pack_c64(p->n, p->tile, p->psi + (size_t)f0 * p->nn(), (const c32*)guess, sup,
                           p->mstride(), nf, p->ctx->stream));
+  // the row transform of the new iterate, for these frames only (the others
+  // keep theirs)
+  if ((rc = rowfft_frames(p, f0, nf))) return rc;
   p->psi_set = true;
-  p->phi_valid = false;
   p->phib_fresh = false;
   return SPK_OK;
 }
@@ -508,50 +560,81 @@ int spk_iterate(spk_plan* p, const spk_step* steps, int nsteps, int per_frame,
   bool dm_any = false;
   for (int k = 0; k < nsteps; ++k) dm_any |= any_dm[k] != 0;
   if (dm_any && (rc = ensure_phib(p))) return rc;
-  if (per_frame && dm_any) {
-    // a step whose phib is produced by a rowfft (not by the previous row
-    // pass) needs one gamma for all frames of that step
-    for (int k = 0; k < nsteps; ++k) {
-      if (!any_dm[k]) continue;
-      float g = 0.0f;
-      for (int f = 0; f < W; ++f) {
-        const spk_step& s = steps[(size_t)k * W + f];
-        if (s.alg != SPK_DM) continue;
-        if (g == 0.0f) g = s.beta;
-        SPK_REQUIRE(s.beta == g, SPK_EUNSUP, "per-frame DM steps need one beta per step");
-      }
-    }
+  bool sw_any = false;
+  for (int k = 0; k < nsteps; ++k) sw_any |= any_sw[k] != 0;
+  if (sw_any) {
+    SPK_REQUIRE(!(p->shared && B > 1), SPK_EUNSUP,
+                "shrinkwrap needs a plan with one support per frame");
+    if ((rc = ensure_sw(p))) return rc;
   }
 
-  // per-frame table upload (pinned staging, reused once the previous upload ran)
+  // Per-frame schedules: per step the list of active frames (launch grids
+  // cover only those), of DM frames needing a fresh phib, and of frames with
+  // a shrinkwrap and their parameters.
+  Stage stg;
+  std::vector<size_t> map_off(nsteps), dm_off(nsteps), sw_off(nsteps), swp_off(nsteps);
+  std::vector<int> map_n(nsteps, B), dm_n(nsteps, 0), sw_n(nsteps, 0);
+  std::vector<float> dm_g(nsteps, 0.0f);
+  std::vector<SwParam> sw_first(nsteps);
+  size_t coef_off = 0, act_off = 0;
   if (per_frame) {
-    const size_t need = (size_t)nsteps * W;
-    if (need > p->dtab_cap) {
-      dfree(p->dtab);
-      dfree(p->dact);
-      if (p->htab) (void)hipHostFree(p->htab);
-      if (p->hact) (void)hipHostFree(p->hact);
-      p->htab = nullptr;
-      p->hact = nullptr;
-      p->dtab_cap = 0;
-      if ((rc = dalloc(&p->dtab, need)) || (rc = dalloc(&p->dact, need))) return rc;
-      SPK_HIP(hipHostMalloc((void**)&p->htab, need * sizeof(Coef), hipHostMallocDefault));
-      SPK_HIP(hipHostMalloc((void**)&p->hact, need * sizeof(int), hipHostMallocDefault));
-      p->dtab_cap = need;
-    } else {
-      SPK_HIP(hipEventSynchronize(p->upload_done));
+    coef_off = stg.add(coefs.data(), coefs.size());
+    act_off = stg.add(act.data(), act.size());
+    std::vector<int> lst;
+    std::vector<SwParam> prm;
+    for (int k = 0; k < nsteps; ++k) {
+      const size_t row = (size_t)k * W;
+      lst.clear();
+      for (int f = 0; f < B; ++f)
+        if (act[row + f]) lst.push_back(f);
+      map_n[k] = (int)lst.size();
+      map_off[k] = stg.add(lst.data(), lst.size());
+      if (any_dm[k]) {
+        lst.clear();
+        for (int f = 0; f < B; ++f) {
+          const spk_step& sp = steps[row + f];
+          if (sp.alg != SPK_DM) continue;
+          if (dm_g[k] == 0.0f) dm_g[k] = 1.0f / sp.beta;
+          SPK_REQUIRE(1.0f / sp.beta == dm_g[k], SPK_EUNSUP,
+                      "per-frame DM steps need one beta per step");
+          lst.push_back(f);
+        }
+        dm_n[k] = (int)lst.size();
+        dm_off[k] = stg.add(lst.data(), lst.size());
+      }
+      if (any_sw[k]) {
+        lst.clear();
+        prm.clear();
+        for (int f = 0; f < B; ++f) {
+          const spk_step& sp = steps[row + f];
+          if (sp.alg == SPK_NOOP || !(sp.sw_sigma > 0.0f)) continue;
+          SwParam q;
+          if ((rc = make_swparam(p->n, sp.sw_sigma, sp.sw_tau, &q))) return rc;
+          lst.push_back(f);
+          prm.push_back(q);
+        }
+        sw_n[k] = (int)lst.size();
+        sw_off[k] = stg.add(lst.data(), lst.size());
+        swp_off[k] = stg.add(prm.data(), prm.size());
+      }
     }
-    std::memcpy(p->htab, coefs.data(), need * sizeof(Coef));
-    std::memcpy(p->hact, act.data(), need * sizeof(int));
-    SPK_HIP(hipMemcpyAsync(p->dtab, p->htab, need * sizeof(Coef), hipMemcpyHostToDevice, st));
-    SPK_HIP(hipMemcpyAsync(p->dact, p->hact, need * sizeof(int), hipMemcpyHostToDevice, st));
-    SPK_HIP(hipEventRecord(p->upload_done, st));
+    if ((rc = upload_stage(p, stg))) return rc;
+  } else {
+    for (int k = 0; k < nsteps; ++k) {
+      if (any_dm[k]) dm_g[k] = 1.0f / steps[k].beta;
+      if (any_sw[k] &&
+          (rc = make_swparam(p->n, steps[k].sw_sigma, steps[k].sw_tau, &sw_first[k])))
+        return rc;
+    }
   }
+  unsigned char* dbase = p->stage_d;
+  auto dptr = [&](size_t off) { return dbase + off; };
 
   // error partials [step][frame][band]
   if (err_trace) {
     const size_t need = (size_t)nsteps * B * p->bands;
     if (need > p->errp_cap) {
+      SPK_HIP(hipStreamSynchronize(st));
       dfree(p->errp);
       p->errp_cap = 0;
       if ((rc = dalloc(&p->errp, need))) return rc;
@@ -559,51 +642,52 @@ int spk_iterate(spk_plan* p, const spk_step* steps, int nsteps, int per_frame,
     }
   }
 
-  if ((rc = refresh_phi(p))) return rc;
   PassArgs a = base_args(p);
   for (int k = 0; k < nsteps; ++k) {
     const size_t row = (size_t)k * W;
     if (per_frame) {
-      a.tab = p->dtab + row;
+      if (map_n[k] == 0) continue;   // every frame idle this step
+      a.tab = (const Coef*)dptr(coef_off) + row;
+      a.fmap = (const int*)dptr(map_off[k]);
     } else {
       a.tab = nullptr;
+      a.fmap = nullptr;
       a.c = coefs[row];
     }
+    const int nact = map_n[k];
     if (any_dm[k] && !p->phib_fresh) {
       // phib = rowFFT((s - gamma_m (1-s)) psi), dm1 (phasing_dm1.cl:16)
-      float beta = 0.0f;
-      for (int f = 0; f < W; ++f)
-        if (steps[row + f].alg == SPK_DM) { beta = steps[row + f].beta; break; }
-      SPK_HIP(launch_rowfft(p->n, p->tile, 2, p->psi, p->phib, p->sup, p->mstride(),
-                            1.0f / beta, p->tw, B, st));
+      SPK_HIP(launch_rowfft(p->n, p->tile, p->ept, 2, p->psi, p->phib, p->sup, p->mstride(),
+                            dm_g[k], p->tw, per_frame ? (const int*)dptr(dm_off[k]) : nullptr,
+                            per_frame ? dm_n[k] : B, st));
     }
     a.errp = err_trace ? p->errp + (size_t)k * B * p->bands : nullptr;
     if ((rc = prof_mark(p, 0))) return rc;
-    SPK_HIP(launch_colpass(p->n, p->tile, 0, a, B, any_dm[k] ? 2 : 1, st));
+    SPK_HIP(launch_colpass(p->n, p->tile, p->ept, 0, a, nact, any_dm[k] ? 2 : 1, st));
     if ((rc = prof_mark(p, 0))) return rc;
     bool emit = false;
     for (int f = 0; f < W; ++f) emit |= (coefs[row + f].flags & kEmitB) != 0;
     a.errp = nullptr;
     if ((rc = prof_mark(p, 1))) return rc;
-    SPK_HIP(launch_rowpass(p->n, p->tile, any_dm[k] != 0, emit, a, B, st));
+    SPK_HIP(launch_rowpass(p->n, p->tile, p->ept, any_dm[k] != 0, emit, a, nact, st));
     if ((rc = prof_mark(p, 1))) return rc;
     p->phib_fresh = emit;
     if (any_sw[k]) {
-      if (!per_frame) {
-        if ((rc = do_shrinkwrap(p, 0, B, steps[row].sw_sigma, steps[row].sw_tau))) return rc;
+      if (per_frame) {
+        SPK_HIP(launch_shrinkwrap(p->n, p->tile, p->psi, p->sup, p->mstride(), p->sw_tmp,
+                                  p->sw_blur, p->sw_maxp, (const SwParam*)dptr(swp_off[k]),
+                                  SwParam{}, (const int*)dptr(sw_off[k]), 0, sw_n[k], st));
       } else {
-        for (int f = 0; f < W; ++f) {
-          const spk_step& s = steps[row + f];
-          if (s.alg == SPK_NOOP || !(s.sw_sigma > 0.0f)) continue;
-          if ((rc = do_shrinkwrap(p, f, 1, s.sw_sigma, s.sw_tau))) return rc;
-        }
+        SPK_HIP(launch_shrinkwrap(p->n, p->tile, p->psi, p->sup, p->mstride(), p->sw_tmp,
+                                  p->sw_blur, p->sw_maxp, nullptr, sw_first[k], nullptr, 0, B,
+                                  st));
       }
+      p->phib_fresh = false;
     }
   }
-  p->phi_valid = true;
   if (err_trace) {
     SPK_HIP(launch_err_finalize(p->errp, nsteps, B, p->bands, p->summ2, err_trace,
-                                per_frame ? p->dact : nullptr, st));
+                                per_frame ? (const int*)dptr(act_off) : nullptr, st));
   }
   return SPK_OK;
 }
@@ -642,6 +726,7 @@ int spk_fft2(spk_plan* p, const float* in, float* out, int count, int inverse) {
   int rc = set_device(p->ctx);
   if (rc) return rc;
   if ((size_t)count > p->fftbuf_frames) {
+    SPK_HIP(hipStreamSynchronize(p->ctx->stream));
     dfree(p->fftbuf);
     p->fftbuf_frames = 0;
     if ((rc = dalloc(&p->fftbuf, (size_t)count * p->nn()))) return rc;
@@ -649,12 +734,12 @@ int spk_fft2(spk_plan* p, const float* in, float* out, int count, int inverse) {
   }
   hipStream_t st = p->ctx->stream;
   SPK_HIP(launch_pack_c64(p->n, p->tile, p->fftbuf, (const c32*)in, nullptr, 0, count, st));
-  SPK_HIP(launch_rowfft(p->n, p->tile, inverse ? 1 : 0, p->fftbuf, p->fftbuf, nullptr, 0, 0.0f,
-                        p->tw, count, st));
+  SPK_HIP(launch_rowfft(p->n, p->tile, p->ept, inverse ? 1 : 0, p->fftbuf, p->fftbuf, nullptr,
+                        0, 0.0f, p->tw, nullptr, count, st));
   PassArgs a = base_args(p);
   a.phi = p->fftbuf;
   a.c.flags = kActive;
-  SPK_HIP(launch_colpass(p->n, p->tile, inverse ? 3 : 2, a, count, 1, st));
+  SPK_HIP(launch_colpass(p->n, p->tile, p->ept, inverse ? 3 : 2, a, count, 1, st));
   const float scl = inverse ? (float)(1.0 / ((double)p->n * p->n)) : 1.0f;
   SPK_HIP(launch_unpack_c64(p->n, p->tile, (c32*)out, p->fftbuf, scl, count, st));
   return SPK_OK;
@@ -668,12 +753,24 @@ int spk_align_global_phase(spk_ctx* ctx, float* data, int frames, int rows, int 
   int nblk = (int)std::min<long long>(64, (per + 255) / 256);
   const size_t need = (size_t)frames * 5 * nblk;
   if (need > ctx->scratch_cap) {
+    if (ctx->stream) SPK_HIP(hipStreamSynchronize(ctx->stream));
+    else SPK_HIP(hipDeviceSynchronize());
     dfree(ctx->scratch);
     ctx->scratch_cap = 0;
     if ((rc = dalloc(&ctx->scratch, need))) return rc;
     ctx->scratch_cap = need;
   }
   SPK_HIP(launch_align((c32*)data, frames, per, ctx->scratch, nblk, ctx->stream));
+  return SPK_OK;
+}
+
+int spk_stack_mean(spk_ctx* ctx, const float* data, int frames, int rows, int cols,
+                   float* out) {
+  SPK_REQUIRE(data && out && frames >= 1 && rows >= 1 && cols >= 1, SPK_EINVAL, "bad stack");
+  int rc = set_device(ctx);
+  if (rc) return rc;
+  SPK_HIP(launch_stack_mean((const c32*)data, (long long)rows * cols, frames, (c32*)out,
+                            ctx->stream));
   return SPK_OK;
 }
 
@@ -721,7 +818,6 @@ int spk_time_passes(spk_plan* p, int which, int reps, const spk_step* step, floa
   SPK_REQUIRE(step->alg != SPK_DM, SPK_EUNSUP, "timing helper covers single-projection steps");
   Coef c;
   if ((rc = make_coef(*step, &c))) return rc;
-  if ((rc = refresh_phi(p))) return rc;
   hipStream_t st = p->ctx->stream;
   PassArgs a = base_args(p);
   a.c = c;
@@ -730,8 +826,8 @@ int spk_time_passes(spk_plan* p, int which, int reps, const spk_step* step, floa
   SPK_HIP(hipEventCreate(&e1));
   SPK_HIP(hipEventRecord(e0, st));
   for (int r = 0; r < reps; ++r) {
-    if (which != 1) SPK_HIP(launch_colpass(p->n, p->tile, 0, a, p->batch, 1, st));
-    if (which != 0) SPK_HIP(launch_rowpass(p->n, p->tile, false, false, a, p->batch, st));
+    if (which != 1) SPK_HIP(launch_colpass(p->n, p->tile, p->ept, 0, a, p->batch, 1, st));
+    if (which != 0) SPK_HIP(launch_rowpass(p->n, p->tile, p->ept, false, false, a, p->batch, st));
   }
   SPK_HIP(hipEventRecord(e1, st));
   SPK_HIP(hipEventSynchronize(e1));

@@ -190,8 +190,9 @@ SPK_HD int stockham_out(int j, int r, int ns, int R) {
   return (j / ns) * ns * R + (j % ns) + r * ns;
 }
 
-// values per thread of one length-N transform
-template <int N>
+// values per thread of one length-N transform (a workgroup holds TT
+// transforms of N/ept threads each)
+template <int N, int TT>
 constexpr int ept() {
   return N < 16 ? N : 16;
 }
@@ -213,18 +214,31 @@ SPK_HD void tile_rc(int n, int p, int& r, int& c) {
   c = tc * TT + w % TT;
 }
 // row pass: element i of thread t on row b of row-band `band` (column t + i*T)
-template <int N, int TT>
+template <int N, int TT, int E = ept<N, TT>()>
 SPK_HD int row_pos(int band, int b, int t, int i) {
-  constexpr int T = N / ept<N>();
+  constexpr int T = N / E;
   const int col = t + i * T;
   return band * (N / TT) * (TT * TT) + (col / TT) * (TT * TT) + b * TT + (col % TT);
 }
 // column pass: element i of thread t on column b of column-band `band` (row t + i*T)
-template <int N, int TT>
+template <int N, int TT, int E = ept<N, TT>()>
 SPK_HD int col_pos(int band, int b, int t, int i) {
-  constexpr int T = N / ept<N>();
+  constexpr int T = N / E;
   const int row = t + i * T;
   return ((row / TT) * (N / TT) + band) * (TT * TT) + (row % TT) * TT + b;
+}
+
+// Modulus layout (read only by the column pass): column-band-major, a band
+// of TT columns is n*TT contiguous floats, row-major inside the band, so a
+// wavefront of the column pass reads it linearly.
+template <int TT>
+SPK_HD int mcb_pos(int n, int r, int c) {
+  return (c / TT) * n * TT + r * TT + (c % TT);
+}
+template <int N, int TT, int E = ept<N, TT>()>
+SPK_HD int mcol_pos(int band, int b, int t, int i) {
+  constexpr int T = N / E;
+  return band * N * TT + (t + i * T) * TT + b;
 }
 
 // LDS slot of element p of transform b (TT transforms per workgroup,
@@ -246,8 +260,19 @@ constexpr int lds_elems() {
 // thread's registers for the next stage.  Kernels put barriers between the
 // two; the CPU emulation runs every thread's half-step in turn.
 // ---------------------------------------------------------------------------
-template <int N, int E, int TT, bool INV, int S>
-SPK_HD void stage_butterflies(c32 (&x)[E], int t, int b, c32* lds, const c32* tw) {
+// Twiddle accessors: tw(m) = W_N^m = exp(-2 pi i m / N), 0 <= m < N.
+struct TwTable {  // full table (global memory or LDS)
+  const c32* p;
+  SPK_HD c32 operator()(int m) const { return p[m]; }
+};
+template <int N>
+struct TwTwoLevel {  // W^m = hi[m >> 6] * lo[m & 63]; hi: N/64 entries, lo: 64
+  const c32* p;      // hi followed by lo
+  SPK_HD c32 operator()(int m) const { return mul(p[m >> 6], p[(N >> 6) + (m & 63)]); }
+};
+
+template <int N, int E, int TT, bool INV, int S, class TW>
+SPK_HD void stage_butterflies(c32 (&x)[E], int t, int b, c32* lds, const TW& tw) {
   using P = RadixPlan<N, E>;
   constexpr int T = N / E;
   constexpr int R = P::radix(S);
@@ -264,7 +289,7 @@ SPK_HD void stage_butterflies(c32 (&x)[E], int t, int b, c32* lds, const c32* tw
     if constexpr (NS > 1) {
       const int k = j & (NS - 1);
       constexpr int STRIDE = N / (NS * R);
-      for (int r = 1; r < R; ++r) v[r] = twiddle<INV>(v[r], tw[(r * k) * STRIDE]);
+      for (int r = 1; r < R; ++r) v[r] = twiddle<INV>(v[r], tw((r * k) * STRIDE));
     }
     dft<R, INV>(v);
     if constexpr (LAST) {

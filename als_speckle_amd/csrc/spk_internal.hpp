@@ -32,13 +32,14 @@ struct PassArgs {
   c32* psi;               // tiled iterate, frame stride n*n
   c32* phi;               // tiled row-transform of psi
   c32* phib;              // tiled row-transform of the DM modified estimate
-  const float* mod;       // tiled modulus, frame stride mstride (0 = shared)
+  const float* mod;       // column-band-major modulus, frame stride mstride (0 = shared)
   const float* sup;       // tiled support, frame stride sstride (0 = shared)
   long long mstride;
   long long sstride;
   const c32* tw;          // forward twiddles W_n^k, k < n
   double* errp;           // column pass: partial error sums [frame][band]
   const Coef* tab;        // per-frame coefficients (nullptr: use c)
+  const int* fmap;        // frames to process: blockIdx.y -> frame (nullptr: identity)
   Coef c;
   float inv_nn;           // 1/(n*n)
 };
@@ -48,18 +49,24 @@ struct BlurW {
   int radius;
 };
 
+struct SwParam {  // one frame's shrinkwrap: blur weights and threshold fraction
+  BlurW w;
+  float tau;
+};
+
 // Kernel launchers (spk_kernels.hip).  All return hipError_t of the launch.
 // `tile` is the internal layout tile edge (4 or 8).
-bool size_supported(int n, int tile);
-hipError_t launch_colpass(int n, int tile, int mode, const PassArgs& a,
+// `ept` is the number of values per thread of one transform (8 or 16).
+bool size_supported(int n, int tile, int ept);
+hipError_t launch_colpass(int n, int tile, int ept, int mode, const PassArgs& a,
                           int frames, int zdim, hipStream_t st);
-hipError_t launch_rowpass(int n, int tile, bool use_t, bool emit_b,
+hipError_t launch_rowpass(int n, int tile, int ept, bool use_t, bool emit_b,
                           const PassArgs& a, int frames, hipStream_t st);
 // mode 0: out = rowFFT(in); 1: out = rowIFFT(in) (unscaled);
 // 2: out = rowFFT((s - g*(1-s)) * in)
-hipError_t launch_rowfft(int n, int tile, int mode, const c32* in, c32* out,
+hipError_t launch_rowfft(int n, int tile, int ept, int mode, const c32* in, c32* out,
                          const float* sup, long long sstride, float g,
-                         const c32* tw, int frames, hipStream_t st);
+                         const c32* tw, const int* fmap, int frames, hipStream_t st);
 hipError_t launch_pack_c64(int n, int tile, c32* dst, const c32* src,
                            const float* sup, long long sstride, int frames,
                            hipStream_t st);
@@ -69,20 +76,28 @@ hipError_t launch_pack_f32(int n, int tile, float* dst, const float* src,
                            int frames, hipStream_t st);
 hipError_t launch_unpack_f32(int n, int tile, float* dst, const float* src,
                              int frames, hipStream_t st);
+// modulus: row-major -> column-band-major (the column pass's read order)
+hipError_t launch_pack_mod(int n, int tile, float* dst, const float* src,
+                           int frames, hipStream_t st);
 hipError_t launch_sumsq(int n, const float* mod, long long mstride,
                         double* partial, int nblk, double* out, int frames,
                         hipStream_t st);
 hipError_t launch_err_finalize(const double* errp, int nsteps, int frames,
                                int bands, const double* summ2, double* trace,
                                const int* active, hipStream_t st);
+// frames: fmap[0..frames) with parameters tab[0..frames), or (fmap null)
+// the range frame0.. with p0 (tab null) / tab
 hipError_t launch_shrinkwrap(int n, int tile, const c32* psi, float* sup,
                              long long sstride, float* tmp, float* blur,
-                             float* maxp, const BlurW& w, float tau,
-                             int frame0, int frames, hipStream_t st);
+                             float* maxp, const SwParam* tab, const SwParam& p0,
+                             const int* fmap, int frame0, int frames,
+                             hipStream_t st);
 hipError_t launch_copy_roi(int n, int tile, const c32* psi, c32* dst, int r0,
                            int c0, int rows, int cols, hipStream_t st);
 hipError_t launch_align(c32* data, int frames, long long per_frame,
                         double* partial, int nblk, hipStream_t st);
+hipError_t launch_stack_mean(const c32* data, long long per_frame, int frames,
+                             c32* out, hipStream_t st);
 hipError_t launch_copy(void* dst, const void* src, size_t bytes,
                        hipStream_t st);
 

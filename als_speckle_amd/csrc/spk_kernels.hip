@@ -17,6 +17,8 @@
 // fourier + pyfft inverse + update kernel (phasing.py:1187-1353).
 #include <hip/hip_runtime.h>
 #include <math.h>
+
+#include <algorithm>
 #include <stdint.h>
 
 #include "spk_internal.hpp"
@@ -24,35 +26,97 @@
 namespace spk {
 
 // ---------------------------------------------------------------------------
+// LDS plan of an FFT kernel: exchange buffer(s) plus the twiddle table.
+// Double buffering (exchanges alternate between two buffers) needs one
+// barrier per exchange instead of two; it is used where both buffers and
+// the twiddle table fit in the 160 KB of a gfx950 CU.  The full twiddle
+// table (N entries) is used where it fits, else the two-level one.
+// ---------------------------------------------------------------------------
+constexpr int kLdsBytes = 160 * 1024;
+
+template <int N, int TT>
+struct LdsPlan {
+  static constexpr int BUF = lds_elems<N, TT>();
+  static constexpr bool DB = (2 * BUF + N) * 8 <= kLdsBytes;
+  static constexpr int NBUF = DB ? 2 : 1;
+  static constexpr bool FULL_TW = (NBUF * BUF + N) * 8 <= kLdsBytes;
+  static constexpr int TW_ELEMS = FULL_TW ? N : (N >> 6) + 64;
+  static constexpr int TOTAL = NBUF * BUF + TW_ELEMS;
+  static_assert(TOTAL * 8 <= kLdsBytes, "FFT kernel does not fit in LDS");
+  static_assert(FULL_TW || N >= 64, "two-level twiddles need N >= 64");
+};
+
+// load the kernel's twiddle table into LDS (all threads; no barrier here:
+// stage 0 uses no twiddles and ends with a barrier)
+template <int N, int TT>
+__device__ __forceinline__ void load_twiddles(c32* twl, const c32* __restrict__ tw) {
+  using L = LdsPlan<N, TT>;
+  const int nt = blockDim.x;
+  if constexpr (L::FULL_TW) {
+    for (int k = threadIdx.x; k < N; k += nt) twl[k] = tw[k];
+  } else {
+    for (int k = threadIdx.x; k < (N >> 6); k += nt) twl[k] = tw[k * 64];
+    for (int k = threadIdx.x; k < 64; k += nt) twl[(N >> 6) + k] = tw[k];
+  }
+}
+
+template <int N, int TT>
+__device__ __forceinline__ auto twiddle_view(const c32* twl) {
+  if constexpr (LdsPlan<N, TT>::FULL_TW) return TwTable{twl};
+  else return TwTwoLevel<N>{twl};
+}
+
+// ---------------------------------------------------------------------------
 // Register-resident Stockham FFT with LDS exchanges.  The per-thread stage
 // logic and every address computation live in spk_fft_core.hpp, shared with
 // the CPU emulation in the test suite; kernels add the barriers.
 // ---------------------------------------------------------------------------
-template <int N, int E, int TT, bool INV, int S>
+template <int N, int E, int TT, bool INV, int S, class TW>
 __device__ __forceinline__ void fft_stage(c32 (&x)[E], const int t, const int b,
-                                          c32* __restrict__ lds,
-                                          const c32* __restrict__ tw) {
+                                          c32* __restrict__ lds, const TW& tw) {
   using P = RadixPlan<N, E>;
-  stage_butterflies<N, E, TT, INV, S>(x, t, b, lds, tw);
+  using L = LdsPlan<N, TT>;
+  c32* buf = lds + (L::DB ? (S & 1) * L::BUF : 0);
+  stage_butterflies<N, E, TT, INV, S>(x, t, b, buf, tw);
   if constexpr (S != P::kStages - 1) {
     __syncthreads();
-    stage_gather<N, E, TT>(x, t, b, lds);
-    __syncthreads();
+    stage_gather<N, E, TT>(x, t, b, buf);
+    if constexpr (!L::DB) __syncthreads();
     fft_stage<N, E, TT, INV, S + 1>(x, t, b, lds, tw);
   }
 }
 
-template <int N, int E, int TT, bool INV>
-__device__ __forceinline__ void fft_regs(c32 (&x)[E], int t, int b, c32* lds,
-                                         const c32* __restrict__ tw) {
+// One transform.  FIRST: no earlier LDS use in this kernel.  With double
+// buffering, an earlier transform's last exchange may still be read by
+// slower threads; a barrier is needed when that exchange used buffer 0,
+// which this transform's first stage writes.
+template <int N, int E, int TT, bool INV, bool FIRST, class TW>
+__device__ __forceinline__ void fft_regs(c32 (&x)[E], int t, int b, c32* lds, const TW& tw) {
+  using P = RadixPlan<N, E>;
+  using L = LdsPlan<N, TT>;
+  if constexpr (!FIRST && L::DB && P::kStages >= 2 && ((P::kStages - 2) & 1) == 0)
+    __syncthreads();
   fft_stage<N, E, TT, INV, 0>(x, t, b, lds, tw);
 }
 
-// deterministic workgroup sum (fixed tree), result valid in thread 0.
-// `sh` must hold blockDim.x doubles and be free for use.
+// deterministic workgroup sum, result valid in thread 0.  Full wavefronts
+// reduce with xor shuffles (fixed order), then thread 0 adds the per-wave
+// sums in order.  `sh` must hold max(blockDim.x, 64) doubles; a barrier
+// first makes earlier LDS users safe.
 __device__ __forceinline__ double block_sum(double v, double* sh) {
   const int tid = threadIdx.x;
   const int nt = blockDim.x;
+  __syncthreads();
+  if ((nt & 63) == 0) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if ((tid & 63) == 0) sh[tid >> 6] = v;
+    __syncthreads();
+    double s = 0.0;
+    if (tid == 0)
+      for (int w = 0; w < (nt >> 6); ++w) s += sh[w];
+    return s;
+  }
   sh[tid] = v;
   __syncthreads();
   for (int s = 1; s < nt; s <<= 1) {
@@ -65,6 +129,16 @@ __device__ __forceinline__ double block_sum(double v, double* sh) {
 __device__ __forceinline__ float block_max(float v, float* sh) {
   const int tid = threadIdx.x;
   const int nt = blockDim.x;
+  __syncthreads();
+  if ((nt & 63) == 0) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+    if ((tid & 63) == 0) sh[tid >> 6] = v;
+    __syncthreads();
+    float m = sh[0];
+    for (int w = 1; w < (nt >> 6); ++w) m = fmaxf(m, sh[w]);
+    return m;
+  }
   sh[tid] = v;
   __syncthreads();
   for (int s = 1; s < nt; s <<= 1) {
@@ -79,13 +153,13 @@ __device__ __forceinline__ float block_max(float v, float* sh) {
 //               MODE 2: forward FFT only; MODE 3: inverse FFT only
 // grid (n/TT bands, frames, z) ; z = 1 processes phib (DM) without error
 // ---------------------------------------------------------------------------
-template <int N, int TT, int MODE>
-__global__ void __launch_bounds__((N / ept<N>()) * TT) k_colpass(PassArgs a) {
-  constexpr int E = ept<N>();
-  constexpr int T = N / E;
+template <int N, int TT, int E, int MODE>
+__global__ void __launch_bounds__((N / E) * TT) k_colpass(PassArgs a) {
   constexpr int NB = N / TT;
-  __shared__ c32 lds[lds_elems<N, TT>()];
-  const int band = blockIdx.x, f = blockIdx.y, z = blockIdx.z;
+  using L = LdsPlan<N, TT>;
+  __shared__ c32 lds[L::TOTAL];
+  const int band = blockIdx.x, z = blockIdx.z;
+  const int f = a.fmap ? a.fmap[blockIdx.y] : blockIdx.y;
   const Coef c = a.tab ? a.tab[f] : a.c;
   if (!(c.flags & kActive)) return;
   if (z && !(c.flags & kUseT)) return;
@@ -93,37 +167,51 @@ __global__ void __launch_bounds__((N / ept<N>()) * TT) k_colpass(PassArgs a) {
   c32* __restrict__ data = (z ? a.phib : a.phi) + fo;
   const int tid = threadIdx.x, b = tid % TT, t = tid / TT;
 
+  // issue every global load up front: data, modulus, twiddle table
   c32 x[E];
 #pragma unroll
-  for (int i = 0; i < E; ++i) x[i] = data[col_pos<N, TT>(band, b, t, i)];
+  for (int i = 0; i < E; ++i) x[i] = data[col_pos<N, TT, E>(band, b, t, i)];
+  // the modulus is loaded with the data where registers allow (16 values per
+  // thread), otherwise after the forward transform
+  constexpr bool PREFETCH = E <= 16 && (N / E) * TT <= 512;
+  const float* __restrict__ mod = a.mod + (size_t)f * a.mstride;
+  float m[MODE == 0 ? E : 1];
+  if constexpr (MODE == 0 && PREFETCH) {
+#pragma unroll
+    for (int i = 0; i < E; ++i) m[i] = mod[mcol_pos<N, TT, E>(band, b, t, i)];
+  }
+  c32* twl = lds + L::NBUF * L::BUF;
+  load_twiddles<N, TT>(twl, a.tw);
+  const auto tw = twiddle_view<N, TT>(twl);
+
   double err = 0.0;
   if constexpr (MODE == 3) {
-    fft_regs<N, E, TT, true>(x, t, b, lds, a.tw);
+    fft_regs<N, E, TT, true, true>(x, t, b, lds, tw);
   } else {
-    fft_regs<N, E, TT, false>(x, t, b, lds, a.tw);
+    fft_regs<N, E, TT, false, true>(x, t, b, lds, tw);
   }
   if constexpr (MODE == 0) {
-    const float* __restrict__ mod = a.mod + (size_t)f * a.mstride;
+    if constexpr (!PREFETCH) {
+#pragma unroll
+      for (int i = 0; i < E; ++i) m[i] = mod[mcol_pos<N, TT, E>(band, b, t, i)];
+    }
 #pragma unroll
     for (int i = 0; i < E; ++i) {
-      const float m = mod[col_pos<N, TT>(band, b, t, i)];
-      const float a2 = x[i].x * x[i].x + x[i].y * x[i].y;
+      const float am = sqrtf(x[i].x * x[i].x + x[i].y * x[i].y);
       // modulus replacement m * Psi / |Psi| (phasing_fourier.cl:12-15,
       // phasing.py:1217); 0 where |Psi| = 0 (the reference gives NaN there)
-      const float r = a2 > 0.0f ? m / sqrtf(a2) : 0.0f;
-      const double ad = sqrt((double)x[i].x * x[i].x + (double)x[i].y * x[i].y);
-      const double d = ad - (double)m;
+      const float r = am > 0.0f ? m[i] / am : 0.0f;
+      // error term: exact difference and square in FP64 of the FP32 |Psi|
+      const double d = (double)am - (double)m[i];
       err += d * d;
       x[i] = scale(x[i], r);
     }
-    fft_regs<N, E, TT, true>(x, t, b, lds, a.tw);
+    fft_regs<N, E, TT, true, false>(x, t, b, lds, tw);
   }
 #pragma unroll
-  for (int i = 0; i < E; ++i) data[col_pos<N, TT>(band, b, t, i)] = x[i];
+  for (int i = 0; i < E; ++i) data[col_pos<N, TT, E>(band, b, t, i)] = x[i];
   if constexpr (MODE == 0) {
     if (z == 0 && a.errp) {
-      // LDS is free: the last FFT stage is register-only and every thread
-      // has passed the barrier that follows its last LDS read
       const double s = block_sum(err, reinterpret_cast<double*>(lds));
       if (tid == 0) a.errp[(size_t)f * NB + band] = s;
     }
@@ -135,58 +223,78 @@ __global__ void __launch_bounds__((N / ept<N>()) * TT) k_colpass(PassArgs a) {
 // new psi -> phi (and of dm1(psi) -> phib when the next step is DM).
 // grid (n/TT bands, frames)
 // ---------------------------------------------------------------------------
-template <int N, int TT, bool USE_T, bool EMIT_B>
-__global__ void __launch_bounds__((N / ept<N>()) * TT) k_rowpass(PassArgs a) {
-  constexpr int E = ept<N>();
-  constexpr int T = N / E;
-  constexpr int NB = N / TT;
-  __shared__ c32 lds[lds_elems<N, TT>()];
-  const int band = blockIdx.x, f = blockIdx.y;
+template <int N, int TT, int E, bool USE_T, bool EMIT_B>
+__global__ void __launch_bounds__((N / E) * TT) k_rowpass(PassArgs a) {
+  using L = LdsPlan<N, TT>;
+  __shared__ c32 lds[L::TOTAL];
+  const int band = blockIdx.x;
+  const int f = a.fmap ? a.fmap[blockIdx.y] : blockIdx.y;
   const Coef c = a.tab ? a.tab[f] : a.c;
   if (!(c.flags & kActive)) return;
   const size_t fo = (size_t)f * N * N;
   const int tid = threadIdx.x, b = tid % TT, t = tid / TT;
-#define SPK_RPOS(i) row_pos<N, TT>(band, b, t, (i))
+#define SPK_RPOS(i) row_pos<N, TT, E>(band, b, t, (i))
 
+  // every global load up front: phi (and phib), psi_in, support, twiddles
   c32 x[E];
 #pragma unroll
   for (int i = 0; i < E; ++i) x[i] = a.phi[fo + SPK_RPOS(i)];
-  fft_regs<N, E, TT, true>(x, t, b, lds, a.tw);
-
   c32 tv[USE_T ? E : 1];
   if constexpr (USE_T) {
 #pragma unroll
     for (int i = 0; i < E; ++i) tv[i] = a.phib[fo + SPK_RPOS(i)];
-    fft_regs<N, E, TT, true>(tv, t, b, lds, a.tw);
+  }
+  // psi_in and the support are loaded with phi where registers allow (16
+  // values per thread, no DM), otherwise after the transforms
+  constexpr bool PREFETCH = E <= 16 && (N / E) * TT <= 512 && !USE_T;
+  const float* __restrict__ sup = a.sup + (size_t)f * a.sstride;
+  c32 pin[E];
+  float sv[E];
+  if constexpr (PREFETCH) {
+#pragma unroll
+    for (int i = 0; i < E; ++i) {
+      pin[i] = a.psi[fo + SPK_RPOS(i)];
+      sv[i] = sup[SPK_RPOS(i)];
+    }
+  }
+  c32* twl = lds + L::NBUF * L::BUF;
+  load_twiddles<N, TT>(twl, a.tw);
+  const auto tw = twiddle_view<N, TT>(twl);
+
+  fft_regs<N, E, TT, true, true>(x, t, b, lds, tw);
+  if constexpr (USE_T) fft_regs<N, E, TT, true, false>(tv, t, b, lds, tw);
+  if constexpr (!PREFETCH) {
+#pragma unroll
+    for (int i = 0; i < E; ++i) {
+      pin[i] = a.psi[fo + SPK_RPOS(i)];
+      sv[i] = sup[SPK_RPOS(i)];
+    }
   }
 
-  const float* __restrict__ sup = a.sup + (size_t)f * a.sstride;
   c32 y[EMIT_B ? E : 1];
 #pragma unroll
   for (int i = 0; i < E; ++i) {
-    const int p = SPK_RPOS(i);
-    const float s = sup[p];
+    const float s = sv[i];
     const float u = 1.0f - s;
-    const c32 pin = a.psi[fo + p];
     const c32 pout = scale(x[i], a.inv_nn);
-    c32 in_s = add(scale(pout, c.ao), scale(pin, c.ai));
-    c32 out_s = add(scale(pout, c.bo), scale(pin, c.bi));
+    c32 in_s = add(scale(pout, c.ao), scale(pin[i], c.ai));
+    c32 out_s = add(scale(pout, c.bo), scale(pin[i], c.bi));
     if constexpr (USE_T) {
       const c32 tt = scale(tv[i], a.inv_nn);
       in_s = add(in_s, scale(tt, c.at));
       out_s = add(out_s, scale(tt, c.bt));
     }
     const c32 nv = add(scale(in_s, s), scale(out_s, u));
-    a.psi[fo + p] = nv;
+    a.psi[fo + SPK_RPOS(i)] = nv;
     x[i] = nv;
     if constexpr (EMIT_B) y[i] = scale(nv, s - c.g_next * u);
   }
   if constexpr (EMIT_B) {
-    fft_regs<N, E, TT, false>(y, t, b, lds, a.tw);
+    fft_regs<N, E, TT, false, false>(y, t, b, lds, tw);
 #pragma unroll
     for (int i = 0; i < E; ++i) a.phib[fo + SPK_RPOS(i)] = y[i];
   }
-  fft_regs<N, E, TT, false>(x, t, b, lds, a.tw);
+  fft_regs<N, E, TT, false, false>(x, t, b, lds, tw);
 #pragma unroll
   for (int i = 0; i < E; ++i) a.phi[fo + SPK_RPOS(i)] = x[i];
 #undef SPK_RPOS
@@ -196,30 +304,30 @@ __global__ void __launch_bounds__((N / ept<N>()) * TT) k_rowpass(PassArgs a) {
 // Row transform of whole frames (tiled in -> tiled out, may alias).
 // MODE 0 forward, 1 inverse (unscaled), 2 forward of (s - g(1-s)) * in
 // ---------------------------------------------------------------------------
-template <int N, int TT, int MODE>
-__global__ void __launch_bounds__((N / ept<N>()) * TT)
+template <int N, int TT, int E, int MODE>
+__global__ void __launch_bounds__((N / E) * TT)
     k_rowfft(const c32* in, c32* out, const float* sup, long long sstride,
-             float g, const c32* __restrict__ tw) {
-  constexpr int E = ept<N>();
-  constexpr int T = N / E;
-  constexpr int NB = N / TT;
-  __shared__ c32 lds[lds_elems<N, TT>()];
-  const int band = blockIdx.x, f = blockIdx.y;
+             float g, const c32* __restrict__ tw, const int* fmap) {
+  using L = LdsPlan<N, TT>;
+  __shared__ c32 lds[L::TOTAL];
+  const int band = blockIdx.x, f = fmap ? fmap[blockIdx.y] : blockIdx.y;
   const size_t fo = (size_t)f * N * N;
   const int tid = threadIdx.x, b = tid % TT, t = tid / TT;
   c32 x[E];
 #pragma unroll
   for (int i = 0; i < E; ++i) {
-    const int p = row_pos<N, TT>(band, b, t, i);
+    const int p = row_pos<N, TT, E>(band, b, t, i);
     x[i] = in[fo + p];
     if constexpr (MODE == 2) {
       const float s = sup[(size_t)f * sstride + p];
       x[i] = scale(x[i], s - g * (1.0f - s));
     }
   }
-  fft_regs<N, E, TT, MODE == 1>(x, t, b, lds, tw);
+  c32* twl = lds + L::NBUF * L::BUF;
+  load_twiddles<N, TT>(twl, tw);
+  fft_regs<N, E, TT, MODE == 1, true>(x, t, b, lds, twiddle_view<N, TT>(twl));
 #pragma unroll
-  for (int i = 0; i < E; ++i) out[fo + row_pos<N, TT>(band, b, t, i)] = x[i];
+  for (int i = 0; i < E; ++i) out[fo + row_pos<N, TT, E>(band, b, t, i)] = x[i];
 }
 
 // ---------------------------------------------------------------------------
@@ -255,6 +363,16 @@ __global__ void __launch_bounds__(256) k_pack_f32(float* dst, const float* src, 
   const size_t fo = (size_t)blockIdx.y * n * n;
   int r, c;
   tile_rc<TT>(n, p, r, c);
+  dst[fo + p] = src[fo + (size_t)r * n + c];
+}
+
+// row-major -> column-band-major (modulus)
+template <int TT>
+__global__ void __launch_bounds__(256) k_pack_mod(float* dst, const float* src, int n) {
+  const int p = blockIdx.x * 256 + threadIdx.x;
+  const size_t fo = (size_t)blockIdx.y * n * n;
+  const int band = p / (n * TT), w = p % (n * TT);
+  const int r = w / TT, c = band * TT + w % TT;
   dst[fo + p] = src[fo + (size_t)r * n + c];
 }
 
@@ -319,12 +437,17 @@ static __global__ void k_err_finalize(const double* errp, int nsteps, int frames
 // ---------------------------------------------------------------------------
 // Shrinkwrap: periodic separable Gaussian blur of |psi| (rows then columns),
 // per-band max, then support *= [blur > tau * max] with the threshold in FP64.
+// grid.y walks the frames to process: frame = fmap[y] (or y), parameters
+// tab[y] (or p0).
 // ---------------------------------------------------------------------------
 template <int N, int TT>
-__global__ void __launch_bounds__(256) k_blur_rows(const c32* psi, float* tmp, BlurW w) {
+__global__ void __launch_bounds__(256) k_blur_rows(const c32* psi, float* tmp,
+                                                   const SwParam* tab, SwParam p0,
+                                                   const int* fmap) {
   constexpr int LS = N + TT;  // padded LDS row stride
   __shared__ float sh[TT * LS];
-  const int band = blockIdx.x, f = blockIdx.y;
+  const int band = blockIdx.x, f = fmap ? fmap[blockIdx.y] : blockIdx.y;
+  const BlurW& w = tab ? tab[blockIdx.y].w : p0.w;
   const size_t off = (size_t)f * N * N + (size_t)band * N * TT;
   // the band is contiguous: element k -> tile k/TT^2, row (k/TT)%TT, col
   for (int k = threadIdx.x; k < N * TT; k += 256) {
@@ -348,12 +471,14 @@ __global__ void __launch_bounds__(256) k_blur_rows(const c32* psi, float* tmp, B
 
 template <int N, int TT>
 __global__ void __launch_bounds__(256) k_blur_cols(const float* tmp, float* blur,
-                                                   float* maxp, BlurW w) {
+                                                   float* maxp, const SwParam* tab,
+                                                   SwParam p0, const int* fmap) {
   constexpr int NB = N / TT;
   constexpr int LS = N + 8;
   __shared__ float sh[TT * LS];
   __shared__ float red[256];
-  const int band = blockIdx.x, f = blockIdx.y;
+  const int band = blockIdx.x, f = fmap ? fmap[blockIdx.y] : blockIdx.y;
+  const BlurW& w = tab ? tab[blockIdx.y].w : p0.w;
   const size_t fo = (size_t)f * N * N;
   for (int k = threadIdx.x; k < N * TT; k += 256) {
     const int tr = k / (TT * TT), within = k % (TT * TT);
@@ -379,10 +504,13 @@ __global__ void __launch_bounds__(256) k_blur_cols(const float* tmp, float* blur
 
 // grid (n*n/1024, frames): 4 elements per thread
 static __global__ void __launch_bounds__(256) k_sw_threshold(float* sup, long long sstride,
-                                                      const float* blur, const float* maxp,
-                                                      int nb, float tau, long long count) {
+                                                             const float* blur, const float* maxp,
+                                                             int nb, long long count,
+                                                             const SwParam* tab, SwParam p0,
+                                                             const int* fmap) {
   __shared__ float red[256];
-  const int f = blockIdx.y;
+  const int f = fmap ? fmap[blockIdx.y] : blockIdx.y;
+  const float tau = tab ? tab[blockIdx.y].tau : p0.tau;
   float mx = -INFINITY;
   for (int i = threadIdx.x; i < nb; i += 256) mx = fmaxf(mx, maxp[(size_t)f * nb + i]);
   const double thr = (double)tau * (double)block_max(mx, red);
@@ -456,6 +584,24 @@ static __global__ void __launch_bounds__(256) k_align_apply(c32* data, long long
 }
 
 // ---------------------------------------------------------------------------
+// mean over a stack of frames (the trial average, phasing.py:960-961):
+// out[i] = (1/frames) sum_f data[f][i], summed in FP64 in frame order
+// ---------------------------------------------------------------------------
+static __global__ void __launch_bounds__(256) k_stack_mean(const c32* data, long long per_frame,
+                                                           int frames, c32* out) {
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < per_frame;
+       i += (long long)gridDim.x * 256) {
+    double re = 0.0, im = 0.0;
+    for (int f = 0; f < frames; ++f) {
+      const c32 v = data[(size_t)f * per_frame + i];
+      re += v.x;
+      im += v.y;
+    }
+    out[i] = c32{(float)(re / frames), (float)(im / frames)};
+  }
+}
+
+// ---------------------------------------------------------------------------
 // plain copy (HBM counter calibration)
 // ---------------------------------------------------------------------------
 static __global__ void __launch_bounds__(256) k_copy16(uint4* dst, const uint4* src, size_t n16) {
@@ -482,43 +628,43 @@ static __global__ void __launch_bounds__(256) k_copy16(uint4* dst, const uint4* 
 
 #define SPK_SIZES(X) X(16) X(32) X(64) X(128) X(256) X(512) X(1024) X(2048) X(4096)
 
-template <int N, int TT>
+template <int N, int TT, int E>
 static hipError_t colpass_nt(int mode, const PassArgs& a, int frames, int zdim,
                              hipStream_t st) {
-  dim3 grid(N / TT, frames, zdim), block((N / ept<N>()) * TT);
+  dim3 grid(N / TT, frames, zdim), block((N / E) * TT);
   switch (mode) {
-    case 0: hipLaunchKernelGGL((k_colpass<N, TT, 0>), grid, block, 0, st, a); break;
-    case 2: hipLaunchKernelGGL((k_colpass<N, TT, 2>), grid, block, 0, st, a); break;
-    case 3: hipLaunchKernelGGL((k_colpass<N, TT, 3>), grid, block, 0, st, a); break;
+    case 0: hipLaunchKernelGGL((k_colpass<N, TT, E, 0>), grid, block, 0, st, a); break;
+    case 2: hipLaunchKernelGGL((k_colpass<N, TT, E, 2>), grid, block, 0, st, a); break;
+    case 3: hipLaunchKernelGGL((k_colpass<N, TT, E, 3>), grid, block, 0, st, a); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
 }
 
-template <int N, int TT>
+template <int N, int TT, int E>
 static hipError_t rowpass_nt(bool use_t, bool emit_b, const PassArgs& a, int frames,
                              hipStream_t st) {
-  dim3 grid(N / TT, frames), block((N / ept<N>()) * TT);
+  dim3 grid(N / TT, frames), block((N / E) * TT);
   if (use_t && emit_b)
-    hipLaunchKernelGGL((k_rowpass<N, TT, true, true>), grid, block, 0, st, a);
+    hipLaunchKernelGGL((k_rowpass<N, TT, E, true, true>), grid, block, 0, st, a);
   else if (use_t)
-    hipLaunchKernelGGL((k_rowpass<N, TT, true, false>), grid, block, 0, st, a);
+    hipLaunchKernelGGL((k_rowpass<N, TT, E, true, false>), grid, block, 0, st, a);
   else if (emit_b)
-    hipLaunchKernelGGL((k_rowpass<N, TT, false, true>), grid, block, 0, st, a);
+    hipLaunchKernelGGL((k_rowpass<N, TT, E, false, true>), grid, block, 0, st, a);
   else
-    hipLaunchKernelGGL((k_rowpass<N, TT, false, false>), grid, block, 0, st, a);
+    hipLaunchKernelGGL((k_rowpass<N, TT, E, false, false>), grid, block, 0, st, a);
   return hipGetLastError();
 }
 
-template <int N, int TT>
+template <int N, int TT, int E>
 static hipError_t rowfft_nt(int mode, const c32* in, c32* out, const float* sup,
-                            long long sstride, float g, const c32* tw, int frames,
-                            hipStream_t st) {
-  dim3 grid(N / TT, frames), block((N / ept<N>()) * TT);
+                            long long sstride, float g, const c32* tw, const int* fmap,
+                            int frames, hipStream_t st) {
+  dim3 grid(N / TT, frames), block((N / E) * TT);
   switch (mode) {
-    case 0: hipLaunchKernelGGL((k_rowfft<N, TT, 0>), grid, block, 0, st, in, out, sup, sstride, g, tw); break;
-    case 1: hipLaunchKernelGGL((k_rowfft<N, TT, 1>), grid, block, 0, st, in, out, sup, sstride, g, tw); break;
-    case 2: hipLaunchKernelGGL((k_rowfft<N, TT, 2>), grid, block, 0, st, in, out, sup, sstride, g, tw); break;
+    case 0: hipLaunchKernelGGL((k_rowfft<N, TT, E, 0>), grid, block, 0, st, in, out, sup, sstride, g, tw, fmap); break;
+    case 1: hipLaunchKernelGGL((k_rowfft<N, TT, E, 1>), grid, block, 0, st, in, out, sup, sstride, g, tw, fmap); break;
+    case 2: hipLaunchKernelGGL((k_rowfft<N, TT, E, 2>), grid, block, 0, st, in, out, sup, sstride, g, tw, fmap); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
@@ -526,66 +672,89 @@ static hipError_t rowfft_nt(int mode, const c32* in, c32* out, const float* sup,
 
 template <int N, int TT>
 static hipError_t shrinkwrap_nt(const c32* psi, float* sup, long long sstride, float* tmp,
-                                float* blur, float* maxp, const BlurW& w, float tau,
-                                int frames, hipStream_t st) {
+                                float* blur, float* maxp, const SwParam* tab, const SwParam& p0,
+                                const int* fmap, int frames, hipStream_t st) {
   dim3 grid(N / TT, frames);
-  hipLaunchKernelGGL((k_blur_rows<N, TT>), grid, dim3(256), 0, st, psi, tmp, w);
-  hipLaunchKernelGGL((k_blur_cols<N, TT>), grid, dim3(256), 0, st, tmp, blur, maxp, w);
+  hipLaunchKernelGGL((k_blur_rows<N, TT>), grid, dim3(256), 0, st, psi, tmp, tab, p0, fmap);
+  hipLaunchKernelGGL((k_blur_cols<N, TT>), grid, dim3(256), 0, st, tmp, blur, maxp, tab, p0,
+                     fmap);
   const long long count = (long long)N * N;
   dim3 g2((unsigned)((count + 1023) / 1024), frames);
   hipLaunchKernelGGL(k_sw_threshold, g2, dim3(256), 0, st, sup, sstride, blur, maxp,
-                     N / TT, tau, count);
+                     N / TT, count, tab, p0, fmap);
   return hipGetLastError();
 }
 
-// tile 8 is not built for n = 4096 (its LDS exchange would not fit)
-template <int N>
-constexpr int tile8() {
-  return N == 4096 ? 4 : 8;
+// Shapes built per frame size: tile 4 and 8 (not 8 at 4096, whose LDS
+// exchange would not fit), 16 values per thread, and 8 values per thread
+// where the workgroup stays within 1024 threads.
+template <int N, int TT, int E>
+constexpr bool shape_ok() {
+  return !(N == 4096 && TT == 8) && (N / E) * TT <= 1024 && E <= N;
 }
 
-#define SPK_TILE_SWITCH(n, CALL4, CALL8)        \
-  if (tile == 4) return CALL4;                  \
-  if (tile == 8 && (n) != 4096) return CALL8;   \
+#define SPK_SHAPE_SWITCH(n, CALLT)                                       \
+  if (tile == 4 && ept == 16) return CALLT(4, 16);                       \
+  if (tile == 4 && ept == 8) return CALLT(4, 8);                         \
+  if (tile == 8 && ept == 16) return CALLT(8, 16);                       \
+  if (tile == 8 && ept == 8) return CALLT(8, 8);                         \
   return hipErrorInvalidValue;
 
 #define SPK_SIZED_DECL(n)                                                                   \
-  hipError_t colpass_##n(int tile, int mode, const PassArgs& a, int frames, int zdim,      \
-                         hipStream_t st);                                                   \
-  hipError_t rowpass_##n(int tile, bool use_t, bool emit_b, const PassArgs& a, int frames, \
-                         hipStream_t st);                                                   \
-  hipError_t rowfft_##n(int tile, int mode, const c32* in, c32* out, const float* sup,     \
-                        long long sstride, float g, const c32* tw, int frames,             \
-                        hipStream_t st);                                                    \
+  hipError_t colpass_##n(int tile, int ept, int mode, const PassArgs& a, int frames,       \
+                         int zdim, hipStream_t st);                                         \
+  hipError_t rowpass_##n(int tile, int ept, bool use_t, bool emit_b, const PassArgs& a,    \
+                         int frames, hipStream_t st);                                       \
+  hipError_t rowfft_##n(int tile, int ept, int mode, const c32* in, c32* out,              \
+                        const float* sup, long long sstride, float g, const c32* tw,       \
+                        const int* fmap, int frames, hipStream_t st);                       \
   hipError_t shrinkwrap_##n(int tile, const c32* psi, float* sup, long long sstride,       \
-                            float* tmp, float* blur, float* maxp, const BlurW& w,          \
-                            float tau, int frames, hipStream_t st);
+                            float* tmp, float* blur, float* maxp, const SwParam* tab,      \
+                            const SwParam& p0, const int* fmap, int frames, hipStream_t st);
+
+// a shape that is not built compiles to an error return, not a kernel
+#define SPK_CALL_COL(TT_, E_)                                                   \
+  (shape_ok<SPK_N, TT_, E_>() ? colpass_nt<SPK_N, (shape_ok<SPK_N, TT_, E_>() ? TT_ : 4), \
+                                           (shape_ok<SPK_N, TT_, E_>() ? E_ : 16)>(     \
+                                    mode, a, frames, zdim, st)                          \
+                              : hipErrorInvalidValue)
+#define SPK_CALL_ROW(TT_, E_)                                                   \
+  (shape_ok<SPK_N, TT_, E_>() ? rowpass_nt<SPK_N, (shape_ok<SPK_N, TT_, E_>() ? TT_ : 4), \
+                                           (shape_ok<SPK_N, TT_, E_>() ? E_ : 16)>(     \
+                                    use_t, emit_b, a, frames, st)                       \
+                              : hipErrorInvalidValue)
+#define SPK_CALL_FFT(TT_, E_)                                                   \
+  (shape_ok<SPK_N, TT_, E_>() ? rowfft_nt<SPK_N, (shape_ok<SPK_N, TT_, E_>() ? TT_ : 4),  \
+                                          (shape_ok<SPK_N, TT_, E_>() ? E_ : 16)>(      \
+                                    mode, in, out, sup, sstride, g, tw, fmap, frames, st) \
+                              : hipErrorInvalidValue)
 
 #define SPK_SIZED_DEF(n)                                                                    \
-  hipError_t colpass_##n(int tile, int mode, const PassArgs& a, int frames, int zdim,      \
-                         hipStream_t st) {                                                  \
-    SPK_TILE_SWITCH(n, (colpass_nt<n, 4>(mode, a, frames, zdim, st)),                      \
-                    (colpass_nt<n, tile8<n>()>(mode, a, frames, zdim, st)))                \
+  hipError_t colpass_##n(int tile, int ept, int mode, const PassArgs& a, int frames,       \
+                         int zdim, hipStream_t st) {                                        \
+    constexpr int SPK_N = n;                                                                \
+    SPK_SHAPE_SWITCH(n, SPK_CALL_COL)                                                       \
   }                                                                                         \
-  hipError_t rowpass_##n(int tile, bool use_t, bool emit_b, const PassArgs& a, int frames, \
-                         hipStream_t st) {                                                  \
-    SPK_TILE_SWITCH(n, (rowpass_nt<n, 4>(use_t, emit_b, a, frames, st)),                   \
-                    (rowpass_nt<n, tile8<n>()>(use_t, emit_b, a, frames, st)))             \
+  hipError_t rowpass_##n(int tile, int ept, bool use_t, bool emit_b, const PassArgs& a,    \
+                         int frames, hipStream_t st) {                                      \
+    constexpr int SPK_N = n;                                                                \
+    SPK_SHAPE_SWITCH(n, SPK_CALL_ROW)                                                       \
   }                                                                                         \
-  hipError_t rowfft_##n(int tile, int mode, const c32* in, c32* out, const float* sup,     \
-                        long long sstride, float g, const c32* tw, int frames,             \
-                        hipStream_t st) {                                                   \
-    SPK_TILE_SWITCH(n, (rowfft_nt<n, 4>(mode, in, out, sup, sstride, g, tw, frames, st)),  \
-                    (rowfft_nt<n, tile8<n>()>(mode, in, out, sup, sstride, g, tw, frames,  \
-                                              st)))                                         \
+  hipError_t rowfft_##n(int tile, int ept, int mode, const c32* in, c32* out,              \
+                        const float* sup, long long sstride, float g, const c32* tw,       \
+                        const int* fmap, int frames, hipStream_t st) {                      \
+    constexpr int SPK_N = n;                                                                \
+    SPK_SHAPE_SWITCH(n, SPK_CALL_FFT)                                                       \
   }                                                                                         \
   hipError_t shrinkwrap_##n(int tile, const c32* psi, float* sup, long long sstride,       \
-                            float* tmp, float* blur, float* maxp, const BlurW& w,          \
-                            float tau, int frames, hipStream_t st) {                        \
-    SPK_TILE_SWITCH(n, (shrinkwrap_nt<n, 4>(psi, sup, sstride, tmp, blur, maxp, w, tau,    \
-                                            frames, st)),                                   \
-                    (shrinkwrap_nt<n, tile8<n>()>(psi, sup, sstride, tmp, blur, maxp, w,   \
-                                                  tau, frames, st)))                        \
+                            float* tmp, float* blur, float* maxp, const SwParam* tab,      \
+                            const SwParam& p0, const int* fmap, int frames, hipStream_t st) { \
+    if (tile == 4) return shrinkwrap_nt<n, 4>(psi, sup, sstride, tmp, blur, maxp, tab, p0, \
+                                              fmap, frames, st);                            \
+    if (tile == 8 && n != 4096)                                                             \
+      return shrinkwrap_nt<n, (n == 4096 ? 4 : 8)>(psi, sup, sstride, tmp, blur, maxp, tab, \
+                                                   p0, fmap, frames, st);                   \
+    return hipErrorInvalidValue;                                                            \
   }
 
 SPK_SIZES(SPK_SIZED_DECL)
@@ -620,16 +789,13 @@ SPK_SIZED_DEF(4096)
 
 #if SPK_BUILD_COMMON
 
-bool size_supported(int n, int tile) {
+bool size_supported(int n, int tile, int ept) {
   if (tile != 4 && tile != 8) return false;
-  switch (n) {
-    case 16: case 32: case 64: case 128: case 256: case 512: case 1024: case 2048:
-      return true;
-    case 4096:
-      return tile == 4;
-    default:
-      return false;
-  }
+  if (ept != 8 && ept != 16) return false;
+  if (n < 16 || n > 4096 || (n & (n - 1))) return false;
+  if (n == 4096 && tile == 8) return false;
+  if (ept > n) return false;
+  return (n / ept) * tile <= 1024;
 }
 
 #define SPK_SIZE_SWITCH(FN, ...)                         \
@@ -646,32 +812,36 @@ bool size_supported(int n, int tile) {
     default: return hipErrorInvalidValue;                \
   }
 
-hipError_t launch_colpass(int n, int tile, int mode, const PassArgs& a, int frames,
+hipError_t launch_colpass(int n, int tile, int ept, int mode, const PassArgs& a, int frames,
                           int zdim, hipStream_t st) {
-  SPK_SIZE_SWITCH(colpass, tile, mode, a, frames, zdim, st)
+  SPK_SIZE_SWITCH(colpass, tile, ept, mode, a, frames, zdim, st)
 }
 
-hipError_t launch_rowpass(int n, int tile, bool use_t, bool emit_b, const PassArgs& a,
+hipError_t launch_rowpass(int n, int tile, int ept, bool use_t, bool emit_b, const PassArgs& a,
                           int frames, hipStream_t st) {
-  SPK_SIZE_SWITCH(rowpass, tile, use_t, emit_b, a, frames, st)
+  SPK_SIZE_SWITCH(rowpass, tile, ept, use_t, emit_b, a, frames, st)
 }
 
-hipError_t launch_rowfft(int n, int tile, int mode, const c32* in, c32* out,
+hipError_t launch_rowfft(int n, int tile, int ept, int mode, const c32* in, c32* out,
                          const float* sup, long long sstride, float g, const c32* tw,
-                         int frames, hipStream_t st) {
-  SPK_SIZE_SWITCH(rowfft, tile, mode, in, out, sup, sstride, g, tw, frames, st)
+                         const int* fmap, int frames, hipStream_t st) {
+  SPK_SIZE_SWITCH(rowfft, tile, ept, mode, in, out, sup, sstride, g, tw, fmap, frames, st)
 }
 
 hipError_t launch_shrinkwrap(int n, int tile, const c32* psi, float* sup, long long sstride,
-                             float* tmp, float* blur, float* maxp, const BlurW& w,
-                             float tau, int frame0, int frames, hipStream_t st) {
-  const size_t fo = (size_t)frame0 * n * n;
-  psi += fo;
-  sup += (size_t)frame0 * sstride;
-  tmp += fo;
-  blur += fo;
-  maxp += (size_t)frame0 * (n / tile);
-  SPK_SIZE_SWITCH(shrinkwrap, tile, psi, sup, sstride, tmp, blur, maxp, w, tau, frames, st)
+                             float* tmp, float* blur, float* maxp, const SwParam* tab,
+                             const SwParam& p0, const int* fmap, int frame0, int frames,
+                             hipStream_t st) {
+  if (!fmap) {  // a contiguous range of frames starting at frame0
+    const size_t fo = (size_t)frame0 * n * n;
+    psi += fo;
+    sup += (size_t)frame0 * sstride;
+    tmp += fo;
+    blur += fo;
+    maxp += (size_t)frame0 * (n / tile);
+  }
+  SPK_SIZE_SWITCH(shrinkwrap, tile, psi, sup, sstride, tmp, blur, maxp, tab, p0, fmap, frames,
+                  st)
 }
 
 hipError_t launch_pack_c64(int n, int tile, c32* dst, const c32* src, const float* sup,
@@ -701,6 +871,16 @@ hipError_t launch_pack_f32(int n, int tile, float* dst, const float* src, int fr
     hipLaunchKernelGGL(k_pack_f32<8>, grid, dim3(256), 0, st, dst, src, n);
   else
     hipLaunchKernelGGL(k_pack_f32<4>, grid, dim3(256), 0, st, dst, src, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_pack_mod(int n, int tile, float* dst, const float* src, int frames,
+                          hipStream_t st) {
+  dim3 grid((unsigned)((size_t)n * n / 256), frames);
+  if (tile == 8)
+    hipLaunchKernelGGL(k_pack_mod<8>, grid, dim3(256), 0, st, dst, src, n);
+  else
+    hipLaunchKernelGGL(k_pack_mod<4>, grid, dim3(256), 0, st, dst, src, n);
   return hipGetLastError();
 }
 
@@ -750,6 +930,14 @@ hipError_t launch_align(c32* data, int frames, long long per_frame, double* part
                      partial, nblk);
   hipLaunchKernelGGL(k_align_apply, dim3(nblk, frames), dim3(256), 0, st, data, per_frame,
                      partial, nblk);
+  return hipGetLastError();
+}
+
+hipError_t launch_stack_mean(const c32* data, long long per_frame, int frames, c32* out,
+                             hipStream_t st) {
+  const long long blocks = std::min<long long>(4096, (per_frame + 255) / 256);
+  hipLaunchKernelGGL(k_stack_mean, dim3((unsigned)blocks), dim3(256), 0, st, data, per_frame,
+                     frames, out);
   return hipGetLastError();
 }
 

@@ -81,6 +81,14 @@ class Context:
         check(self.lib.spk_align_global_phase(self.handle, _ptr(data), f, r, c))
         return data
 
+    def stack_mean(self, data: torch.Tensor) -> torch.Tensor:
+        """Mean over the first axis of a (frames, rows, cols) complex64 tensor."""
+        f, r, c = data.shape
+        _expect(data, torch.complex64, (f, r, c), self.device, "data")
+        out = torch.empty((r, c), dtype=torch.complex64, device=self.device)
+        check(self.lib.spk_stack_mean(self.handle, _ptr(data), f, r, c, _ptr(out)))
+        return out
+
     def copy_bytes(self, dst: torch.Tensor, src: torch.Tensor) -> None:
         nb = src.numel() * src.element_size()
         check(self.lib.spk_copy_bytes(self.handle, _ptr(dst), _ptr(src), nb))
@@ -109,11 +117,12 @@ class Plan:
     """``batch`` resident frames of n x n (see spk_plan_create)."""
 
     def __init__(self, ctx: Context, n: int, batch: int = 1, shared: bool = True,
-                 tile: int = 0):
+                 tile: int = 0, ept: int = 0):
         self.ctx, self.lib = ctx, ctx.lib
         self.n, self.batch, self.shared = int(n), int(batch), bool(shared)
         tune = _capi.SpkTune()
         tune.tile = int(tile)
+        tune.ept = int(ept)
         h = c_void_p()
         check(self.lib.spk_plan_create(ctx.handle, self.n, self.batch, int(self.shared),
                                        byref(tune), byref(h)))

@@ -1,20 +1,31 @@
 """Benchmark of the phase-retrieval hot path on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--cpu-iters I] [--no-cpu]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--cpu-iters I]
+                    [--no-cpu] [--no-batched] [--no-serving] [--requests R]
 
-Workload (single reconstruction, 1024 x 1024 complex64): synthetic frame
-(random star polygon, ~15 % fill, amplitude U[0.5, 1], phase U[-pi, pi],
-Poisson 1e8 photons, support dilated 3 px), 1000 iterations cycling 90 HIO /
-10 ER (beta 0.8, numpy semantics), in-loop shrinkwrap after every 20
-iterations (sigma 3 -> 1.5 px, tau 0.2), FP64 modulus-error trace, ROI copy
-into the savebuffer.  One step = one whole reconstruction (seed from a
-device-resident guess, reset support, 1000 iterations, ROI copy); inputs are
-resident in HBM before the timed region.  value = iterations per second.
+Headline workload (single reconstruction, 1024 x 1024 complex64): synthetic
+frame (random star polygon, ~15 % fill, amplitude U[0.5, 1], phase
+U[-pi, pi], Poisson 1e8 photons, support dilated 3 px), 1000 iterations
+cycling 90 HIO / 10 ER (beta 0.8, numpy semantics), in-loop shrinkwrap after
+every 20 iterations (sigma 3 -> 1.5 px, tau 0.2), FP64 modulus-error trace,
+ROI copy into the savebuffer.  One step = one whole reconstruction (seed from
+a device-resident guess, reset support, 1000 iterations, ROI copy); inputs
+are resident in HBM before the timed region.  value = iterations per second.
 
-Also reported: the dominant kernel's HBM roofline (algorithmic bytes per
-launch / average launch time, timed with HIP events on the library stream
-during one profiled reconstruction) and the restated reference numpy path
-timed on the host cores (bounded sample).
+Reported beside it in the same JSON line:
+  roofline      dominant kernel of the headline workload: algorithmic bytes
+                per launch / average launch time (HIP events around each
+                launch on the library stream, one profiled reconstruction)
+  batched       64 trials of one 512^2 frame (RAAR beta 0.87 and DM beta 0.7,
+                500 iterations each, aligned and averaged on the GPU), and 16
+                trials of the headline frame and schedule in one plan
+                (recons/s and the ratio to the single-reconstruction form)
+  serving       Poisson arrivals of mixed sizes {256: 40 %, 512: 30 %,
+                1024: 20 %, 2048: 10 %}, 500 HIO/ER iterations with shrinkwrap
+                each, continuous batching under an HBM budget: p50/p99
+                latency and recons/s
+  cpu_baseline  the restated reference numpy path on the host cores
+                (bounded sample of the headline frame)
 """
 from __future__ import annotations
 
@@ -42,7 +53,24 @@ def parse():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--cpu-iters", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--requests", type=int, default=2000)
+    ap.add_argument("--budget-gb", type=float, default=48.0)
     return ap.parse_args()
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def headline_steps(iters):
+    from als_speckle_amd import _capi
+    from als_speckle_amd.phasing import expand_order, shrinkwrap_schedule
+    algs = expand_order((("hio", 90), ("er", 10)), iters)
+    sw = shrinkwrap_schedule(iters, every=20, sigma0=3.0, sigma1=1.5, tau=0.2)
+    return [(_capi.ALG_CODES[x], _capi.SEM_NUMPY, 0.8, *sw.get(k, (0.0, 0.0)))
+            for k, x in enumerate(algs)], len(sw)
 
 
 def cpu_baseline(frame, iters):
@@ -57,42 +85,31 @@ def cpu_baseline(frame, iters):
     r.iterate(algs)
     dt = time.perf_counter() - t0
     cores = len(os.sched_getaffinity(0))
-    return {"value": iters / dt, "unit": "iterations/s", "cores": 1, "kind": "port",
+    return {"value": round(iters / dt, 3), "unit": "iterations/s", "cores": 1, "kind": "port",
             "sample": f"{iters} iterations (90 HIO/10 ER, beta 0.8) of the 1024x1024 frame, "
                       f"numpy {np.__version__} complex64 path with the reference's redundant "
                       f"FFT per _fft2 call, single-threaded ({cores} host cores visible)"}
 
 
-def main():
-    a = parse()
-    if a.gpus != 1:
-        print(f"bench.py measures one GPU only; --gpus {a.gpus} is not supported "
-              "(the path runs on a single MI355X; see DESIGN.md for how it would shard)",
-              file=sys.stderr)
-        sys.exit(2)
+def bench_single(ctx, a):
     import torch
-    from als_speckle_amd import _capi, synth
-    from als_speckle_amd.device import Context, Plan
-    from als_speckle_amd.phasing import expand_order, shrinkwrap_schedule
+    from als_speckle_amd.device import Plan
+    from als_speckle_amd.masking import bounding_box
+    from als_speckle_amd import synth
 
     n, iters = 1024, 1000
     frame = synth.config_frame("single_1024", 0)
-    ctx = Context(0)
     dev = ctx.device
     m_d = torch.from_numpy(frame.modulus).to(dev)
     s_d = torch.from_numpy(frame.support).to(dev)
     np.random.seed(1)
     guess = (np.random.rand(n, n) + 1j * np.random.rand(n, n)).astype(np.complex64)
     g_d = torch.from_numpy(guess).to(dev)
-    r0, r1, c0, c1 = (int(v) for v in __import__("als_speckle_amd.masking", fromlist=["x"]).bounding_box(frame.support, pad=10))
-    savebuf = torch.zeros((a.steps + a.warmup + 2, r1 - r0, c1 - c0), dtype=torch.complex64, device=dev)
+    r0, r1, c0, c1 = (int(v) for v in bounding_box(frame.support, pad=10))
+    savebuf = torch.zeros((a.steps + a.warmup + 2, r1 - r0, c1 - c0), dtype=torch.complex64,
+                          device=dev)
     err = torch.empty((iters, 1), dtype=torch.float64, device=dev)
-
-    algs = expand_order((("hio", 90), ("er", 10)), iters)
-    sw = shrinkwrap_schedule(iters, every=20, sigma0=3.0, sigma1=1.5, tau=0.2)
-    steps = [(_capi.ALG_CODES[x], _capi.SEM_NUMPY, 0.8, *sw.get(k, (0.0, 0.0)))
-             for k, x in enumerate(algs)]
-
+    steps, nsw = headline_steps(iters)
     plan = Plan(ctx, n, batch=1, shared=True)
     plan.set_modulus(m_d)
 
@@ -112,22 +129,177 @@ def main():
     ctx.sync()
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
-    its_per_s = iters * a.steps / dt
     final_err = float(err[-1, 0].item())
 
-    # dominant kernel: one profiled reconstruction, HIP events around each pass
     plan.profile(True)
     reconstruction(a.warmup + a.steps)
     ncol, tcol = plan.profile_read(0)
     nrow, trow = plan.profile_read(1)
     plan.profile(False)
-    avg_col, avg_row = tcol / ncol * 1e-3, trow / nrow * 1e-3     # seconds
-    bytes_col, bytes_row = B_COL * n * n, B_ROW * n * n
-    passes = {"column": (avg_col, bytes_col), "row": (avg_row, bytes_row)}
-    dom = max(passes, key=lambda k: passes[k][0] * (ncol if k == "column" else nrow))
-    d_avg, d_bytes = passes[dom]
+    avg = {"column": tcol / ncol * 1e-3, "row": trow / nrow * 1e-3}
+    byts = {"column": B_COL * n * n, "row": B_ROW * n * n}
+    dom = max(avg, key=lambda k: avg[k])
+    model = iters * (B_COL + B_ROW) * n * n + nsw * B_SHRINK * n * n + B_ONEOFF * n * n
+    return dict(frame=frame, n=n, iters=iters, dt=dt, final_err=final_err, avg=avg, byts=byts,
+                dom=dom, model=model)
+
+
+def bench_trials_512(ctx):
+    """64 random-start trials of one 512^2 frame, RAAR then DM, 500 iterations."""
+    import torch
+    from als_speckle_amd import synth
+    from als_speckle_amd.batch import phasing_batch
+    n, T, iters = 512, 64, 500
+    f = synth.config_frame("trials_512", 0)
+    b = phasing_batch(T, gpu_info=ctx)
+    b.load(f.modulus, f.support)
+    g = np.empty((T, n, n), np.complex64)
+    for t in range(T):
+        np.random.seed(t)
+        g[t] = (np.random.rand(n, n) + 1j * np.random.rand(n, n)).astype(np.complex64)
+    g_d = torch.from_numpy(g).to(ctx.device)
+    out = {}
+    for alg, beta, bpx in (("raar", 0.87, B_COL + B_ROW), ("dm", 0.7, 92)):
+        b.seed(guesses=g_d)
+        b.iterate(10, order=((alg, 1),), beta=beta)           # warm-up
+        ctx.sync()
+        t0 = time.perf_counter()
+        b.seed(guesses=g_d)
+        b.iterate(iters, order=((alg, 1),), beta=beta)
+        aligned, avg = b.aligned_average()
+        ctx.sync()
+        dt = time.perf_counter() - t0
+        tr = b.get_error_trace()
+        # m and s are shared by the trials: charged once per batched iteration
+        shared_px = 8 if alg == "raar" else 12
+        model = iters * n * n * ((bpx - shared_px) * T + shared_px)
+        out[alg] = {"recons_per_s": round(T / dt, 2), "ms_per_batch": round(dt * 1e3, 2),
+                    "us_per_batched_iteration": round(dt / iters * 1e6, 2),
+                    "model_hbm_frac": round(model / dt / HBM_PEAK, 4),
+                    "final_error_mean": float(np.mean(tr[-1])),
+                    "final_error_min": float(np.min(tr[-1]))}
+    return out
+
+
+def bench_batched_1024(ctx, frame, trials=16):
+    """The headline frame and schedule, `trials` random starts in one plan."""
+    import torch
+    from als_speckle_amd.device import Plan
+    n, iters = 1024, 1000
+    steps, _ = headline_steps(iters)
+    dev = ctx.device
+    plan = Plan(ctx, n, batch=trials, shared=False)
+    m = torch.from_numpy(np.stack([frame.modulus] * trials)).to(dev)
+    s = torch.from_numpy(np.stack([frame.support] * trials)).to(dev)
+    g = np.empty((trials, n, n), np.complex64)
+    for t in range(trials):
+        np.random.seed(1000 + t)
+        g[t] = (np.random.rand(n, n) + 1j * np.random.rand(n, n)).astype(np.complex64)
+    g = torch.from_numpy(g).to(dev)
+    plan.set_modulus(m)
+    err = torch.empty((iters, trials), dtype=torch.float64, device=dev)
+
+    def run():
+        plan.set_support(s)
+        plan.set_psi(g, mult_support=True)
+        plan.iterate(steps, err_out=err)
+
+    run()
+    ctx.sync()
+    t0 = time.perf_counter()
+    run()
+    ctx.sync()
+    dt = time.perf_counter() - t0
+    fe = err[-1].cpu().numpy()
+    return {"trials": trials, "recons_per_s": round(trials / dt, 3),
+            "iterations_per_s": round(trials * iters / dt, 1),
+            "final_error_mean": float(fe.mean()), "final_error_min": float(fe.min())}
+
+
+def bench_serving(a):
+    """Poisson arrivals of mixed frame sizes through the continuous-batching server."""
+    import torch
+    from als_speckle_amd import synth
+    from als_speckle_amd.serve import Request, Server
+    mix = {256: 0.4, 512: 0.3, 1024: 0.2, 2048: 0.1}
+    rng = np.random.Generator(np.random.PCG64(7))
+    dev = torch.device("cuda", 0)
+    bank = {}
+    for n in mix:
+        bank[n] = []
+        for k in range(2):
+            f = synth.make_frame(n, 100 + 10 * k + n, shape="polygon",
+                                 fill=float(rng.uniform(0.10, 0.15)), photons=1e7)
+            bank[n].append((torch.from_numpy(f.modulus).to(dev),
+                            torch.from_numpy(f.support).to(dev)))
+    sizes = list(mix)
+    probs = np.array([mix[n] for n in sizes])
+
+    def make_requests(count, arrivals):
+        reqs = []
+        gen = torch.Generator(device=dev)
+        gen.manual_seed(11)
+        for i in range(count):
+            n = int(sizes[rng.choice(len(sizes), p=probs)])
+            m, s = bank[n][int(rng.integers(0, len(bank[n])))]
+            re = torch.rand((n, n), device=dev, generator=gen)
+            im = torch.rand((n, n), device=dev, generator=gen)
+            g = torch.complex(re, im)
+            reqs.append(Request(i, m, s, g, iterations=500, shrinkwrap=True,
+                                arrival=float(arrivals[i])))
+        return reqs
+
+    budget = int(a.budget_gb * (1 << 30))
+    srv = Server(device=0, budget_bytes=budget, mix=mix, block=20)
+    slots = {p.n: p.slots for p in srv.pools.values()}
+    # capacity: a saturated burst
+    burst = make_requests(min(200, a.requests), np.zeros(a.requests))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    srv.run(burst)
+    cap = len(burst) / (time.perf_counter() - t0)
+    log(f"serving capacity {cap:.1f} req/s with slots {slots}")
+    lam = 0.8 * cap
+    arrivals = np.cumsum(rng.exponential(1.0 / lam, a.requests))
+    reqs = make_requests(a.requests, arrivals)
+    torch.cuda.synchronize()
+    srv2 = Server(device=0, slots=slots, block=20)
+    t0 = time.perf_counter()
+    res = srv2.run(reqs)
+    span = time.perf_counter() - t0
+    lat = np.array([r.latency for r in res]) * 1e3
+    by = {}
+    for n in sizes:
+        ln = np.array([r.latency for r in res if r.n == n]) * 1e3
+        if ln.size:
+            by[str(n)] = {"count": int(ln.size), "p50_ms": round(float(np.percentile(ln, 50)), 2),
+                          "p99_ms": round(float(np.percentile(ln, 99)), 2)}
+    fin = np.array([r.trace[-1] for r in res])
+    return {"requests": len(res), "arrival_rate_per_s": round(lam, 2),
+            "capacity_req_per_s": round(cap, 2), "recons_per_s": round(len(res) / span, 2),
+            "p50_ms": round(float(np.percentile(lat, 50)), 2),
+            "p99_ms": round(float(np.percentile(lat, 99)), 2), "by_size": by,
+            "slots": {str(k): v for k, v in slots.items()}, "hbm_budget_gb": a.budget_gb,
+            "final_error_median": float(np.median(fin))}
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {a.gpus} is not supported "
+              "(the path runs on a single MI355X; DESIGN.md describes how it would shard)",
+              file=sys.stderr)
+        sys.exit(2)
+    from als_speckle_amd.device import Context
+    ctx = Context(0)
+    log("single 1024^2 reconstructions")
+    s = bench_single(ctx, a)
+    n, iters, dt = s["n"], s["iters"], s["dt"]
+    its_per_s = iters * a.steps / dt
+    plain_rps = a.steps / dt
+    dom = s["dom"]
+    d_avg, d_bytes = s["avg"][dom], s["byts"][dom]
     achieved = d_bytes / d_avg
-    model_bytes = iters * (B_COL + B_ROW) * n * n + len(sw) * B_SHRINK * n * n + B_ONEOFF * n * n
 
     traffic = None
     tfile = os.path.join(ROOT, "profiles", f"traffic_{dom}pass_1024.json")
@@ -154,11 +326,12 @@ def main():
             "workload": "single_1024: 1024x1024 complex64, 1000 iterations 90 HIO/10 ER "
                         "(beta 0.8, numpy semantics), shrinkwrap every 20 (sigma 3->1.5, "
                         "tau 0.2), FP64 error trace, polygon support 15% fill, Poisson 1e8",
-            "n": n, "iterations": iters, "recons_per_s": round(a.steps / dt, 3),
-            "final_modulus_error": final_err,
-            "model_bytes_per_recon": model_bytes,
-            "model_hbm_frac_whole_recon": round(model_bytes * a.steps / dt / HBM_PEAK, 4),
-            "column_pass_us": round(avg_col * 1e6, 3), "row_pass_us": round(avg_row * 1e6, 3),
+            "n": n, "iterations": iters, "recons_per_s": round(plain_rps, 3),
+            "final_modulus_error": s["final_err"],
+            "model_bytes_per_recon": s["model"],
+            "model_hbm_frac_whole_recon": round(s["model"] * a.steps / dt / HBM_PEAK, 4),
+            "column_pass_us": round(s["avg"]["column"] * 1e6, 3),
+            "row_pass_us": round(s["avg"]["row"] * 1e6, 3),
         },
         "roofline": {
             "bound": "hbm", "kernel": f"k_{'colpass' if dom == 'column' else 'rowpass'}<1024>",
@@ -168,8 +341,19 @@ def main():
             "avg_launch_us": round(d_avg * 1e6, 3),
         },
     }
+    if not a.no_batched:
+        log("batched trials 64 x 512^2 (RAAR, DM)")
+        out["batched"] = {"trials_512": bench_trials_512(ctx)}
+        log("batched 16 x 1024^2 headline schedule")
+        b16 = bench_batched_1024(ctx, s["frame"])
+        b16["speedup_vs_single"] = round(b16["recons_per_s"] / plain_rps, 3)
+        out["batched"]["trials_1024"] = b16
+    if not a.no_serving:
+        log(f"serving {a.requests} requests")
+        out["serving"] = bench_serving(a)
     if not a.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(frame, a.cpu_iters)
+        log("cpu baseline")
+        out["cpu_baseline"] = cpu_baseline(s["frame"], a.cpu_iters)
     print(json.dumps(out), flush=True)
 
 

@@ -63,7 +63,8 @@ typedef struct spk_step {
 /* Kernel shape choices for a plan (autotuner output).  Zero = library default. */
 typedef struct spk_tune {
   int32_t tile;       /* tile edge of the internal layout: 4 or 8            */
-  int32_t reserved[7];
+  int32_t ept;        /* values per thread of one transform: 8 or 16         */
+  int32_t reserved[6];
 } spk_tune;
 
 /* -------------------------------------------------------------------------
@@ -160,6 +161,12 @@ int spk_fft2(spk_plan* plan, const float* in_c64, float* out_c64, int count,
 /* align_global_phase (phasing.py:1355-1404), closed form, FP64 sums */
 int spk_align_global_phase(spk_ctx* ctx, float* data_c64, int frames,
                            int rows, int cols);
+
+/* trial average: out (rows, cols) c64 = mean over frames of data; the
+ * reference averages aligned trials with np.mean / np.sum on the host
+ * (phasing.py:960-961, basic_phasing_example.py:43); FP64 sums */
+int spk_stack_mean(spk_ctx* ctx, const float* data_c64, int frames, int rows,
+                   int cols, float* out_c64);
 
 /* -------------------------------------------------------------------------
  * Measurement helpers.

@@ -26,7 +26,7 @@ def emu_lib(tmp_path_factory):
                    check=True)
     lib = ctypes.CDLL(so)
     lib.emu_fft2.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
-                             ctypes.c_void_p]
+                             ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
     lib.emu_fft2.restype = ctypes.c_int
     lib.emu_stockham_out.argtypes = [ctypes.c_int] * 4
     lib.emu_stockham_out.restype = ctypes.c_int

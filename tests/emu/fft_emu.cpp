@@ -26,9 +26,8 @@ std::vector<c32> twiddles(int n) {
 }
 
 // all threads (t, b) of one workgroup: x[(t*TT + b)][E]
-template <int N, int TT, bool INV, int S>
-void emu_stages(std::vector<std::array<c32, ept<N>()>>& x, c32* lds, const c32* tw) {
-  constexpr int E = ept<N>();
+template <int N, int TT, int E, bool INV, int S, class TW>
+void emu_stages(std::vector<std::array<c32, E>>& x, c32* lds, const TW& tw) {
   constexpr int T = N / E;
   using P = RadixPlan<N, E>;
   for (int tid = 0; tid < T * TT; ++tid) {
@@ -40,15 +39,13 @@ void emu_stages(std::vector<std::array<c32, ept<N>()>>& x, c32* lds, const c32* 
       c32(&xr)[E] = *reinterpret_cast<c32(*)[E]>(x[tid].data());
       stage_gather<N, E, TT>(xr, tid / TT, tid % TT, lds);
     }
-    emu_stages<N, TT, INV, S + 1>(x, lds, tw);
+    emu_stages<N, TT, E, INV, S + 1, TW>(x, lds, tw);
   }
 }
 
-template <int N, int TT>
-int fft2_tiled(int inverse, const c32* in, c32* out) {
-  constexpr int E = ept<N>();
+template <int N, int TT, int E, class TW>
+int fft2_tiled_tw(int inverse, const c32* in, c32* out, const TW& tw) {
   constexpr int T = N / E;
-  const std::vector<c32> tw = twiddles(N);
   std::vector<c32> tiled((size_t)N * N), lds(lds_elems<N, TT>());
   for (int p = 0; p < N * N; ++p) {
     int r, c;
@@ -59,24 +56,39 @@ int fft2_tiled(int inverse, const c32* in, c32* out) {
   // row transforms, one band of TT rows per workgroup
   for (int band = 0; band < N / TT; ++band) {
     for (int tid = 0; tid < T * TT; ++tid)
-      for (int i = 0; i < E; ++i) x[tid][i] = tiled[row_pos<N, TT>(band, tid % TT, tid / TT, i)];
-    if (inverse) emu_stages<N, TT, true, 0>(x, lds.data(), tw.data());
-    else emu_stages<N, TT, false, 0>(x, lds.data(), tw.data());
+      for (int i = 0; i < E; ++i) x[tid][i] = tiled[row_pos<N, TT, E>(band, tid % TT, tid / TT, i)];
+    if (inverse) emu_stages<N, TT, E, true, 0>(x, lds.data(), tw);
+    else emu_stages<N, TT, E, false, 0>(x, lds.data(), tw);
     for (int tid = 0; tid < T * TT; ++tid)
-      for (int i = 0; i < E; ++i) tiled[row_pos<N, TT>(band, tid % TT, tid / TT, i)] = x[tid][i];
+      for (int i = 0; i < E; ++i) tiled[row_pos<N, TT, E>(band, tid % TT, tid / TT, i)] = x[tid][i];
   }
   // column transforms, one band of TT columns per workgroup
   for (int band = 0; band < N / TT; ++band) {
     for (int tid = 0; tid < T * TT; ++tid)
-      for (int i = 0; i < E; ++i) x[tid][i] = tiled[col_pos<N, TT>(band, tid % TT, tid / TT, i)];
-    if (inverse) emu_stages<N, TT, true, 0>(x, lds.data(), tw.data());
-    else emu_stages<N, TT, false, 0>(x, lds.data(), tw.data());
+      for (int i = 0; i < E; ++i) x[tid][i] = tiled[col_pos<N, TT, E>(band, tid % TT, tid / TT, i)];
+    if (inverse) emu_stages<N, TT, E, true, 0>(x, lds.data(), tw);
+    else emu_stages<N, TT, E, false, 0>(x, lds.data(), tw);
     for (int tid = 0; tid < T * TT; ++tid)
-      for (int i = 0; i < E; ++i) tiled[col_pos<N, TT>(band, tid % TT, tid / TT, i)] = x[tid][i];
+      for (int i = 0; i < E; ++i) tiled[col_pos<N, TT, E>(band, tid % TT, tid / TT, i)] = x[tid][i];
   }
   for (int r = 0; r < N; ++r)
     for (int c = 0; c < N; ++c) out[(size_t)r * N + c] = tiled[tpos<TT>(N, r, c)];
   return 0;
+}
+
+// twiddles as the kernels use them: the full table (N <= 2048) or the
+// two-level table (hi = W^(64 i), lo = W^j) where LDS is short (N = 4096)
+template <int N, int TT, int E>
+int fft2_tiled(int inverse, const c32* in, c32* out, int two_level) {
+  const std::vector<c32> tw = twiddles(N);
+  if (!two_level) return fft2_tiled_tw<N, TT, E>(inverse, in, out, TwTable{tw.data()});
+  if constexpr (N >= 64) {
+    std::vector<c32> hl((N >> 6) + 64);
+    for (int i = 0; i < (N >> 6); ++i) hl[i] = tw[i * 64];
+    for (int j = 0; j < 64; ++j) hl[(N >> 6) + j] = tw[j];
+    return fft2_tiled_tw<N, TT, E>(inverse, in, out, TwTwoLevel<N>{hl.data()});
+  }
+  return -1;
 }
 
 }  // namespace
@@ -84,13 +96,16 @@ int fft2_tiled(int inverse, const c32* in, c32* out) {
 extern "C" {
 
 // 2-D transform of an n x n complex64 array through the tiled layout
-int emu_fft2(int n, int tile, int inverse, const float* in, float* out) {
+int emu_fft2(int n, int tile, int inverse, const float* in, float* out, int two_level,
+             int ept) {
   const c32* i = reinterpret_cast<const c32*>(in);
   c32* o = reinterpret_cast<c32*>(out);
 #define EMU_CASE(NN)                                            \
   if (n == NN) {                                                \
-    if (tile == 8 && NN != 4096) return fft2_tiled<NN, (NN == 4096 ? 4 : 8)>(inverse, i, o); \
-    if (tile == 4) return fft2_tiled<NN, 4>(inverse, i, o);      \
+    if (ept == 16 && tile == 8) return fft2_tiled<NN, 8, 16>(inverse, i, o, two_level); \
+    if (ept == 16 && tile == 4) return fft2_tiled<NN, 4, 16>(inverse, i, o, two_level); \
+    if (ept == 8 && tile == 8) return fft2_tiled<NN, 8, 8>(inverse, i, o, two_level);   \
+    if (ept == 8 && tile == 4) return fft2_tiled<NN, 4, 8>(inverse, i, o, two_level);   \
     return -1;                                                  \
   }
   EMU_CASE(16) EMU_CASE(32) EMU_CASE(64) EMU_CASE(128) EMU_CASE(256) EMU_CASE(512)

@@ -13,15 +13,23 @@ SIZES = [16, 32, 64, 128, 256, 512, 1024, 2048, 4096]
 @pytest.mark.parametrize("n", SIZES)
 @pytest.mark.parametrize("tile", [4, 8])
 @pytest.mark.parametrize("inverse", [0, 1])
-def test_emulated_fft2_matches_numpy(emu_lib, n, tile, inverse):
+@pytest.mark.parametrize("two_level", [0, 1])
+@pytest.mark.parametrize("ept", [8, 16])
+def test_emulated_fft2_matches_numpy(emu_lib, n, tile, inverse, two_level, ept):
     if n == 4096 and tile == 8:
         pytest.skip("tile 8 is not built for 4096")
+    if (n // ept) * tile > 1024:
+        pytest.skip("workgroup would exceed 1024 threads")
+    if ept == 8 and (n >= 2048 or two_level):
+        pytest.skip("8 values per thread: small and medium frames, full twiddle table")
     if n == 4096 and inverse:
         pytest.skip("one direction is enough at 4096 (2 s each)")
+    if two_level and (n < 64 or (n < 1024 and tile == 8)):
+        pytest.skip("two-level twiddles are used for large frames only")
     rng = np.random.default_rng(n + tile)
     x = (rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))).astype(np.complex64)
     out = np.zeros_like(x)
-    assert emu_lib.emu_fft2(n, tile, inverse, x.ctypes.data, out.ctypes.data) == 0
+    assert emu_lib.emu_fft2(n, tile, inverse, x.ctypes.data, out.ctypes.data, two_level, ept) == 0
     xd = x.astype(np.complex128)
     ref = np.fft.ifft2(xd) * n * n if inverse else np.fft.fft2(xd)
     rel = np.linalg.norm(out - ref) / np.linalg.norm(ref)

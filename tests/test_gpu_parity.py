@@ -43,8 +43,8 @@ def start_psi(n, s, seed=5):
     return (g * s).astype(np.complex64)
 
 
-def load_plan(ctx, f, psi, batch=1):
-    p = Plan(ctx, f.modulus.shape[0], batch=batch, shared=True)
+def load_plan(ctx, f, psi, batch=1, tile=0, ept=0):
+    p = Plan(ctx, f.modulus.shape[0], batch=batch, shared=True, tile=tile, ept=ept)
     p.set_modulus(dev(f.modulus, torch.float32))
     p.set_support(dev(f.support, torch.float32))
     if batch == 1:
@@ -61,14 +61,17 @@ def step(alg, beta=0.9, sem="numpy", sigma=0.0, tau=0.0):
 # ------------------------------------------------------------------ FFT
 @pytest.mark.parametrize("n", [16, 32, 64, 128, 256, 512, 1024, 2048, 4096])
 @pytest.mark.parametrize("tile", [4, 8])
-def test_fft2_matches_numpy(gpu_ctx, n, tile):
+@pytest.mark.parametrize("ept", [16, 8])
+def test_fft2_matches_numpy(gpu_ctx, n, tile, ept):
     if n == 4096 and tile == 8:
         pytest.skip("tile 8 not built at 4096")
-    print(f"[fft2 n={n} tile={tile}]", flush=True)
+    if (n // ept) * tile > 1024:
+        pytest.skip("shape not built (over 1024 threads)")
+    print(f"[fft2 n={n} tile={tile} ept={ept}]", flush=True)
     rng = np.random.default_rng(n)
     cnt = 2 if n <= 1024 else 1
     x = (rng.standard_normal((cnt, n, n)) + 1j * rng.standard_normal((cnt, n, n))).astype(np.complex64)
-    p = Plan(gpu_ctx, n, batch=1, tile=tile)
+    p = Plan(gpu_ctx, n, batch=1, tile=tile, ept=ept)
     fx = p.fft2(dev(x, torch.complex64)).cpu().numpy()
     ix = p.fft2(dev(x, torch.complex64), inverse=True).cpu().numpy()
     xd = x.astype(np.complex128)
@@ -81,12 +84,13 @@ def test_fft2_matches_numpy(gpu_ctx, n, tile):
 @pytest.mark.parametrize("alg,sem", [("hio", "numpy"), ("er", "numpy"), ("er", "opencl"),
                                      ("raar", "numpy"), ("sf", "numpy"), ("sf", "opencl"),
                                      ("dm", "numpy")])
-def test_one_step_matches_oracle(gpu_ctx, n, alg, sem):
-    print(f"[one-step n={n} {alg}/{sem}]", flush=True)
+@pytest.mark.parametrize("tile,ept", [(4, 16), (8, 16), (4, 8)])
+def test_one_step_matches_oracle(gpu_ctx, n, alg, sem, tile, ept):
+    print(f"[one-step n={n} {alg}/{sem} tile={tile} ept={ept}]", flush=True)
     f = frame(n)
     psi = start_psi(n, f.support)
     beta = 0.7 if alg == "dm" else 0.9
-    p = load_plan(gpu_ctx, f, psi)
+    p = load_plan(gpu_ctx, f, psi, tile=tile, ept=ept)
     err = p.iterate([step(alg, beta, sem)]).cpu().numpy()[0, 0]
     got = p.get_psi().cpu().numpy()
     want, _, e = O.iteration(psi.astype(np.complex128), f.modulus, f.support, alg, beta, sem)
@@ -159,17 +163,18 @@ def test_dm_doubles_solution(gpu_ctx):
     assert rel(p.get_psi().cpu().numpy(), 2 * f.obj) < 2e-6
 
 
-def test_dm_multi_step_lockstep(gpu_ctx):
+@pytest.mark.parametrize("tile,ept", [(4, 16), (8, 8)])
+def test_dm_multi_step_lockstep(gpu_ctx, tile, ept):
     """DM over several iterations uses the row pass's phib emission; the
     result equals step-by-step calls (phib rebuilt each call) and the oracle
     per step."""
     n = 128
     f = frame(n, seed=6)
     psi = start_psi(n, f.support, seed=8)
-    p = load_plan(gpu_ctx, f, psi)
+    p = load_plan(gpu_ctx, f, psi, tile=tile, ept=ept)
     tr_all = p.iterate([step("dm", 0.7)] * 8).cpu().numpy()[:, 0]
     end_all = p.get_psi().cpu().numpy()
-    q = load_plan(gpu_ctx, f, psi)
+    q = load_plan(gpu_ctx, f, psi, tile=tile, ept=ept)
     cur = psi.astype(np.complex128)
     for k in range(8):
         e = q.iterate([step("dm", 0.7)]).cpu().numpy()[0, 0]

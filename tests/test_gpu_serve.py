@@ -1,0 +1,95 @@
+"""Batched trials and the continuous-batching server on the GPU.
+
+The identity checks are bitwise: a frame's kernels run the same code on the
+same inputs whatever else is resident, and no reduction spans frames.
+"""
+import numpy as np
+import pytest
+import torch
+
+from als_speckle_amd import synth
+from als_speckle_amd.batch import phasing_batch
+from als_speckle_amd.phasing import phasing
+from als_speckle_amd.serve import Request, Server
+from oracle import phasing_c128 as O
+
+pytestmark = pytest.mark.gpu
+
+
+def test_batch_trials_equal_sequential_trials():
+    f = synth.make_frame(128, 2, shape="polygon", fill=0.1, photons=1e7)
+    seeds = [11, 12, 13, 14]
+    b = phasing_batch(4)
+    b.load(f.modulus, f.support)
+    b.seed(seeds)
+    b.iterate(30, order=(("raar", 1),), beta=0.87)
+    tr = b.get_error_trace()
+    stack = b.get(b.roi_stack())
+    r = phasing()
+    r.load(modulus=f.modulus, support=f.support, numtrials=4)
+    for t, sd in enumerate(seeds):
+        r.seed(sd)
+        r.iterate(30, order=(("raar", 1),), beta=0.87)
+        assert np.array_equal(r.get_error_trace(), tr[:, t])
+    buf = r.get(r.savebuffer)
+    assert np.array_equal(buf[:4], stack)
+    aligned, avg = b.aligned_average()
+    want = O.align_global_phase(stack.astype(np.complex128))
+    got = b.get(aligned)
+    assert np.linalg.norm(got - want) / np.linalg.norm(want) < 1e-5
+    assert np.allclose(b.get(avg), want.mean(0), atol=1e-5 * np.abs(want).max())
+
+
+def test_batch_dm_runs_and_matches_single():
+    f = synth.make_frame(64, 3, shape="disk", radius=10, photons=1e6)
+    b = phasing_batch(3)
+    b.load(f.modulus, f.support)
+    b.seed([1, 2, 3])
+    b.iterate(12, order=(("dm", 1),), beta=0.7)
+    tr = b.get_error_trace()
+    r = phasing()
+    r.load(modulus=f.modulus, support=f.support, numtrials=1)
+    r.seed(2)
+    r.iterate(12, order=(("dm", 1),), beta=0.7)
+    assert np.array_equal(r.get_error_trace(), tr[:, 1])
+
+
+def _requests(specs):
+    reqs = []
+    for rid, (n, seed, arrival) in enumerate(specs):
+        f = synth.make_frame(n, seed, shape="polygon", fill=0.12, photons=1e7)
+        rng = np.random.default_rng(100 + rid)
+        g = (rng.random((n, n)) + 1j * rng.random((n, n))).astype(np.complex64)
+        reqs.append(Request(rid, f.modulus, f.support, g, iterations=60,
+                            shrinkwrap=True, arrival=arrival))
+    return reqs
+
+
+def test_server_result_same_alone_and_in_mix():
+    specs = [(64, 1, 0.0), (128, 2, 0.0), (64, 3, 0.0), (64, 4, 0.0), (128, 5, 0.0),
+             (64, 6, 0.0)]
+    mix = Server(slots={64: 2, 128: 1}, block=20)
+    res = {r.rid: r for r in mix.run(_requests(specs))}
+    assert sorted(res) == list(range(len(specs)))
+    # queueing happened: more 64^2 requests than slots
+    assert max(r.admitted for r in res.values()) > 0
+    for rid in (2, 4):
+        alone = Server(slots={64: 2, 128: 1}, block=20)
+        one = alone.run([_requests(specs)[rid]])[0]
+        assert np.array_equal(one.psi.cpu().numpy(), res[rid].psi.cpu().numpy())
+        assert np.array_equal(one.support.cpu().numpy(), res[rid].support.cpu().numpy())
+        assert np.array_equal(one.trace, res[rid].trace)
+
+
+def test_server_matches_dropin_object():
+    specs = [(64, 7, 0.0)]
+    req = _requests(specs)[0]
+    res = Server(slots={64: 3}, block=20).run([req])[0]
+    r = phasing()
+    r.load(modulus=req.modulus, support=req.support, numtrials=1)
+    # the drop-in seeds from numpy; load the same start through the C ABI plan
+    r._plan.set_psi(torch.from_numpy(req.guess).cuda(), -1, mult_support=True)
+    r._seeded = True
+    r.iterate(60, order=req.order, beta=req.beta, shrinkwrap=True)
+    assert np.array_equal(r.get(r.psi_in), res.psi.cpu().numpy())
+    assert np.array_equal(r.get_error_trace(), res.trace)

@@ -114,3 +114,15 @@ def test_phasing_has_no_cpu_path():
     if not torch.cuda.is_available():
         with pytest.raises(RuntimeError, match="GPU"):
             P.phasing()
+
+
+def test_server_budget_sizing():
+    from als_speckle_amd.device import plan_bytes
+    from als_speckle_amd.serve import Server
+    mix = {256: 0.4, 512: 0.3, 1024: 0.2, 2048: 0.1}
+    slots = Server.size_slots(8 << 30, mix)
+    used = sum(plan_bytes(n, k, False) for n, k in slots.items())
+    assert used <= (8 << 30) * 1.01
+    assert slots[2048] >= 1 and slots[256] >= 1
+    # larger frames get more of the budget, smaller frames more slots
+    assert slots[256] >= slots[2048]
