@@ -63,6 +63,10 @@ SIGNATURES = {
     "spk_stack_mean": (c_int, [c_void_p, c_void_p, c_int, c_int, c_int, c_void_p]),
     "spk_time_passes": (c_int, [c_void_p, c_int, c_int, POINTER(SpkStep), POINTER(c_float)]),
     "spk_plan_profile": (c_int, [c_void_p, c_int]),
+    "spk_graph_begin": (c_int, [c_void_p]),
+    "spk_graph_end": (c_int, [c_void_p, POINTER(c_void_p)]),
+    "spk_graph_launch": (c_int, [c_void_p]),
+    "spk_graph_destroy": (c_int, [c_void_p]),
     "spk_plan_profile_read": (c_int, [c_void_p, c_int, POINTER(c_int), POINTER(c_double)]),
 }
 

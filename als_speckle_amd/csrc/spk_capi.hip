@@ -2,6 +2,7 @@
 // argument checking, plan state, and the launch sequence of an iteration.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -260,13 +261,17 @@ int upload_stage(spk_plan* p, const Stage& stg) {
   return SPK_OK;
 }
 
-// bracket one pass launch with events when profiling
-int prof_mark(spk_plan* p, int pass) {
+// when profiling, give the next pass launch a start/stop event pair, recorded
+// by the dispatch itself (hipExtLaunchKernelGGL)
+int prof_arm(spk_plan* p, int pass) {
   if (!p->profiling) return SPK_OK;
-  hipEvent_t e;
-  SPK_HIP(hipEventCreate(&e));
-  SPK_HIP(hipEventRecord(e, p->ctx->stream));
-  p->prof_ev[pass].push_back(e);
+  hipEvent_t e0, e1;
+  SPK_HIP(hipEventCreate(&e0));
+  SPK_HIP(hipEventCreate(&e1));
+  p->prof_ev[pass].push_back(e0);
+  p->prof_ev[pass].push_back(e1);
+  g_launch_events[0] = e0;
+  g_launch_events[1] = e1;
   return SPK_OK;
 }
 
@@ -292,7 +297,7 @@ int do_shrinkwrap(spk_plan* p, int frame0, int frames, float sigma, float tau) {
   if ((rc = ensure_sw(p))) return rc;
   SPK_HIP(launch_shrinkwrap(p->n, p->tile, p->psi, p->sup, p->mstride(), p->sw_tmp,
                             p->sw_blur, p->sw_maxp, nullptr, sp, nullptr, frame0, frames,
-                            p->ctx->stream));
+                            sp.w.radius, p->ctx->stream));
   p->phib_fresh = false;
   return SPK_OK;
 }
@@ -573,7 +578,7 @@ int spk_iterate(spk_plan* p, const spk_step* steps, int nsteps, int per_frame,
   // a shrinkwrap and their parameters.
   Stage stg;
   std::vector<size_t> map_off(nsteps), dm_off(nsteps), sw_off(nsteps), swp_off(nsteps);
-  std::vector<int> map_n(nsteps, B), dm_n(nsteps, 0), sw_n(nsteps, 0);
+  std::vector<int> map_n(nsteps, B), dm_n(nsteps, 0), sw_n(nsteps, 0), sw_r(nsteps, 0);
   std::vector<float> dm_g(nsteps, 0.0f);
   std::vector<SwParam> sw_first(nsteps);
   size_t coef_off = 0, act_off = 0;
@@ -612,6 +617,7 @@ int spk_iterate(spk_plan* p, const spk_step* steps, int nsteps, int per_frame,
           if ((rc = make_swparam(p->n, sp.sw_sigma, sp.sw_tau, &q))) return rc;
           lst.push_back(f);
           prm.push_back(q);
+          sw_r[k] = std::max(sw_r[k], q.w.radius);
         }
         sw_n[k] = (int)lst.size();
         sw_off[k] = stg.add(lst.data(), lst.size());
@@ -622,9 +628,11 @@ int spk_iterate(spk_plan* p, const spk_step* steps, int nsteps, int per_frame,
   } else {
     for (int k = 0; k < nsteps; ++k) {
       if (any_dm[k]) dm_g[k] = 1.0f / steps[k].beta;
-      if (any_sw[k] &&
-          (rc = make_swparam(p->n, steps[k].sw_sigma, steps[k].sw_tau, &sw_first[k])))
-        return rc;
+      if (any_sw[k]) {
+        if ((rc = make_swparam(p->n, steps[k].sw_sigma, steps[k].sw_tau, &sw_first[k])))
+          return rc;
+        sw_r[k] = sw_first[k].w.radius;
+      }
     }
   }
   unsigned char* dbase = p->stage_d;
@@ -662,25 +670,24 @@ int spk_iterate(spk_plan* p, const spk_step* steps, int nsteps, int per_frame,
                             per_frame ? dm_n[k] : B, st));
     }
     a.errp = err_trace ? p->errp + (size_t)k * B * p->bands : nullptr;
-    if ((rc = prof_mark(p, 0))) return rc;
+    if ((rc = prof_arm(p, 0))) return rc;
     SPK_HIP(launch_colpass(p->n, p->tile, p->ept, 0, a, nact, any_dm[k] ? 2 : 1, st));
-    if ((rc = prof_mark(p, 0))) return rc;
     bool emit = false;
     for (int f = 0; f < W; ++f) emit |= (coefs[row + f].flags & kEmitB) != 0;
     a.errp = nullptr;
-    if ((rc = prof_mark(p, 1))) return rc;
+    if ((rc = prof_arm(p, 1))) return rc;
     SPK_HIP(launch_rowpass(p->n, p->tile, p->ept, any_dm[k] != 0, emit, a, nact, st));
-    if ((rc = prof_mark(p, 1))) return rc;
     p->phib_fresh = emit;
     if (any_sw[k]) {
       if (per_frame) {
         SPK_HIP(launch_shrinkwrap(p->n, p->tile, p->psi, p->sup, p->mstride(), p->sw_tmp,
                                   p->sw_blur, p->sw_maxp, (const SwParam*)dptr(swp_off[k]),
-                                  SwParam{}, (const int*)dptr(sw_off[k]), 0, sw_n[k], st));
+                                  SwParam{}, (const int*)dptr(sw_off[k]), 0, sw_n[k], sw_r[k],
+                                  st));
       } else {
         SPK_HIP(launch_shrinkwrap(p->n, p->tile, p->psi, p->sup, p->mstride(), p->sw_tmp,
                                   p->sw_blur, p->sw_maxp, nullptr, sw_first[k], nullptr, 0, B,
-                                  st));
+                                  sw_r[k], st));
       }
       p->phib_fresh = false;
     }
@@ -779,6 +786,63 @@ int spk_copy_bytes(spk_ctx* ctx, void* dst, const void* src, size_t bytes) {
   int rc = set_device(ctx);
   if (rc) return rc;
   SPK_HIP(launch_copy(dst, src, bytes, ctx->stream));
+  return SPK_OK;
+}
+
+struct spk_graph {
+  spk_ctx* ctx = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+int spk_graph_begin(spk_plan* p) {
+  SPK_REQUIRE(p, SPK_EINVAL, "null plan");
+  int rc = set_device(p->ctx);
+  if (rc) return rc;
+  SPK_REQUIRE(p->ctx->stream != nullptr, SPK_EINVAL,
+              "graph capture needs a context created on a non-null stream");
+  SPK_REQUIRE(!p->profiling, SPK_ESTATE, "disable profiling before capturing");
+  SPK_HIP(hipStreamBeginCapture(p->ctx->stream, hipStreamCaptureModeThreadLocal));
+  return SPK_OK;
+}
+
+int spk_graph_end(spk_plan* p, spk_graph** out) {
+  SPK_REQUIRE(p && out, SPK_EINVAL, "null argument");
+  *out = nullptr;
+  int rc = set_device(p->ctx);
+  if (rc) return rc;
+  hipGraph_t g = nullptr;
+  SPK_HIP(hipStreamEndCapture(p->ctx->stream, &g));
+  spk_graph* h = new spk_graph;
+  h->ctx = p->ctx;
+  h->graph = g;
+  hipError_t e = hipGraphInstantiate(&h->exec, g, nullptr, nullptr, 0);
+  if (e != hipSuccess) {
+    (void)hipGraphDestroy(g);
+    delete h;
+    return fail(SPK_EHIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+  }
+  p->ctx->refs++;
+  *out = h;
+  return SPK_OK;
+}
+
+int spk_graph_launch(spk_graph* h) {
+  SPK_REQUIRE(h, SPK_EINVAL, "null graph");
+  int rc = set_device(h->ctx);
+  if (rc) return rc;
+  SPK_HIP(hipGraphLaunch(h->exec, h->ctx->stream));
+  return SPK_OK;
+}
+
+int spk_graph_destroy(spk_graph* h) {
+  if (!h) return SPK_OK;
+  (void)hipSetDevice(h->ctx->device);
+  (void)hipStreamSynchronize(h->ctx->stream);
+  if (h->exec) (void)hipGraphExecDestroy(h->exec);
+  if (h->graph) (void)hipGraphDestroy(h->graph);
+  ctx_release(h->ctx);
+  delete h;
   return SPK_OK;
 }
 

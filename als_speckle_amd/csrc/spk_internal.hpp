@@ -54,6 +54,12 @@ struct SwParam {  // one frame's shrinkwrap: blur weights and threshold fraction
   float tau;
 };
 
+// Kernel timing: when set, the next column-pass or row-pass launch on this
+// thread is made with hipExtLaunchKernelGGL and records these start/stop
+// events with the dispatch itself (the kernel's own execution interval, as a
+// profiler reports it); the launch clears them.
+extern thread_local hipEvent_t g_launch_events[2];
+
 // Kernel launchers (spk_kernels.hip).  All return hipError_t of the launch.
 // `tile` is the internal layout tile edge (4 or 8).
 // `ept` is the number of values per thread of one transform (8 or 16).
@@ -87,10 +93,11 @@ hipError_t launch_err_finalize(const double* errp, int nsteps, int frames,
                                const int* active, hipStream_t st);
 // frames: fmap[0..frames) with parameters tab[0..frames), or (fmap null)
 // the range frame0.. with p0 (tab null) / tab
+// rmax: the largest blur radius among the frames (<= 47)
 hipError_t launch_shrinkwrap(int n, int tile, const c32* psi, float* sup,
                              long long sstride, float* tmp, float* blur,
                              float* maxp, const SwParam* tab, const SwParam& p0,
-                             const int* fmap, int frame0, int frames,
+                             const int* fmap, int frame0, int frames, int rmax,
                              hipStream_t st);
 hipError_t launch_copy_roi(int n, int tile, const c32* psi, c32* dst, int r0,
                            int c0, int rows, int cols, hipStream_t st);

@@ -15,6 +15,7 @@
 //                new psi_in -> phi for the next iteration
 // This replaces, per iteration, the reference's pyfft forward + abs_f2_f2 +
 // fourier + pyfft inverse + update kernel (phasing.py:1187-1353).
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 #include <math.h>
 
@@ -440,63 +441,99 @@ static __global__ void k_err_finalize(const double* errp, int nsteps, int frames
 // grid.y walks the frames to process: frame = fmap[y] (or y), parameters
 // tab[y] (or p0).
 // ---------------------------------------------------------------------------
-template <int N, int TT>
+// LDS slot of logical index i in a blur line: one pad word per 8 so that
+// threads starting 8 apart hit different banks
+__device__ __forceinline__ int blur_slot(int i) { return i + (i >> 3); }
+
+// Sliding-window convolution of one line held in LDS: outputs
+// out[c0 .. c0+K) = sum_{|d| <= R} w[|d|] line[(c0 + k + d) mod N], with the
+// window in registers (K + 2*RMAX loads) and weights w[d] = 0 for d > R.
+template <int N, int K, int RMAX>
+__device__ __forceinline__ void blur_window(const float* line, int c0, const float (&w)[RMAX + 1],
+                                            float (&acc)[K]) {
+  float win[K + 2 * RMAX];
+#pragma unroll
+  for (int j = 0; j < K + 2 * RMAX; ++j) win[j] = line[blur_slot((c0 - RMAX + j) & (N - 1))];
+#pragma unroll
+  for (int k = 0; k < K; ++k) acc[k] = w[0] * win[k + RMAX];
+#pragma unroll
+  for (int d = 1; d <= RMAX; ++d) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) acc[k] += w[d] * (win[k + RMAX + d] + win[k + RMAX - d]);
+  }
+}
+
+template <int RMAX>
+__device__ __forceinline__ void load_blur_weights(const SwParam* tab, const SwParam& p0,
+                                                  float* wsh, float (&w)[RMAX + 1]) {
+  const BlurW& bw = tab ? tab[blockIdx.y].w : p0.w;
+  if (threadIdx.x <= RMAX) wsh[threadIdx.x] = (int)threadIdx.x <= bw.radius ? bw.w[threadIdx.x] : 0.0f;
+  __syncthreads();
+#pragma unroll
+  for (int d = 0; d <= RMAX; ++d) w[d] = wsh[d];
+}
+
+template <int N, int TT, int RMAX>
 __global__ void __launch_bounds__(256) k_blur_rows(const c32* psi, float* tmp,
                                                    const SwParam* tab, SwParam p0,
                                                    const int* fmap) {
-  constexpr int LS = N + TT;  // padded LDS row stride
+  constexpr int K = 8;
+  constexpr int LS = N + N / 8 + 8;  // padded line stride
   __shared__ float sh[TT * LS];
+  __shared__ float wsh[RMAX + 1];
   const int band = blockIdx.x, f = fmap ? fmap[blockIdx.y] : blockIdx.y;
-  const BlurW& w = tab ? tab[blockIdx.y].w : p0.w;
   const size_t off = (size_t)f * N * N + (size_t)band * N * TT;
-  // the band is contiguous: element k -> tile k/TT^2, row (k/TT)%TT, col
+  // the band is contiguous: element k -> row (k/TT)%TT, column (k/TT^2)*TT + k%TT
   for (int k = threadIdx.x; k < N * TT; k += 256) {
     const int b = (k / TT) % TT;
     const int col = (k / (TT * TT)) * TT + k % TT;
     const c32 v = psi[off + k];
-    sh[b * LS + col] = sqrtf(v.x * v.x + v.y * v.y);
+    sh[b * LS + blur_slot(col)] = sqrtf(v.x * v.x + v.y * v.y);
   }
-  __syncthreads();
-  const int R = w.radius;
-  for (int k = threadIdx.x; k < N * TT; k += 256) {
-    const int b = (k / TT) % TT;
-    const int col = (k / (TT * TT)) * TT + k % TT;
-    const float* row = sh + b * LS;
-    float acc = w.w[0] * row[col];
-    for (int d = 1; d <= R; ++d)
-      acc += w.w[d] * (row[(col + d) & (N - 1)] + row[(col - d) & (N - 1)]);
-    tmp[off + k] = acc;
+  float w[RMAX + 1];
+  load_blur_weights<RMAX>(tab, p0, wsh, w);   // includes the barrier
+  for (int seg = threadIdx.x; seg < N * TT / K; seg += 256) {
+    const int b = seg / (N / K), c0 = (seg % (N / K)) * K;
+    float acc[K];
+    blur_window<N, K, RMAX>(sh + b * LS, c0, w, acc);
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const int col = c0 + k;
+      tmp[off + (col / TT) * (TT * TT) + b * TT + (col % TT)] = acc[k];
+    }
   }
 }
 
-template <int N, int TT>
+template <int N, int TT, int RMAX>
 __global__ void __launch_bounds__(256) k_blur_cols(const float* tmp, float* blur,
                                                    float* maxp, const SwParam* tab,
                                                    SwParam p0, const int* fmap) {
   constexpr int NB = N / TT;
-  constexpr int LS = N + 8;
+  constexpr int K = 8;
+  constexpr int LS = N + N / 8 + 8;
   __shared__ float sh[TT * LS];
+  __shared__ float wsh[RMAX + 1];
   __shared__ float red[256];
   const int band = blockIdx.x, f = fmap ? fmap[blockIdx.y] : blockIdx.y;
-  const BlurW& w = tab ? tab[blockIdx.y].w : p0.w;
   const size_t fo = (size_t)f * N * N;
   for (int k = threadIdx.x; k < N * TT; k += 256) {
     const int tr = k / (TT * TT), within = k % (TT * TT);
     const int row = tr * TT + within / TT, ci = within % TT;
-    sh[ci * LS + row] = tmp[fo + (size_t)(tr * NB + band) * (TT * TT) + within];
+    sh[ci * LS + blur_slot(row)] = tmp[fo + (size_t)(tr * NB + band) * (TT * TT) + within];
   }
-  __syncthreads();
-  const int R = w.radius;
+  float w[RMAX + 1];
+  load_blur_weights<RMAX>(tab, p0, wsh, w);
   float mx = -INFINITY;
-  for (int k = threadIdx.x; k < N * TT; k += 256) {
-    const int tr = k / (TT * TT), within = k % (TT * TT);
-    const int row = tr * TT + within / TT, ci = within % TT;
-    const float* col = sh + ci * LS;
-    float acc = w.w[0] * col[row];
-    for (int d = 1; d <= R; ++d)
-      acc += w.w[d] * (col[(row + d) & (N - 1)] + col[(row - d) & (N - 1)]);
-    blur[fo + (size_t)(tr * NB + band) * (TT * TT) + within] = acc;
-    mx = fmaxf(mx, acc);
+  for (int seg = threadIdx.x; seg < N * TT / K; seg += 256) {
+    const int ci = seg / (N / K), r0 = (seg % (N / K)) * K;
+    float acc[K];
+    blur_window<N, K, RMAX>(sh + ci * LS, r0, w, acc);
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const int row = r0 + k;
+      blur[fo + (size_t)((row / TT) * NB + band) * (TT * TT) + (row % TT) * TT + ci] = acc[k];
+      mx = fmaxf(mx, acc[k]);
+    }
   }
   const float m = block_max(mx, red);
   if (threadIdx.x == 0) maxp[(size_t)f * NB + band] = m;
@@ -628,14 +665,26 @@ static __global__ void __launch_bounds__(256) k_copy16(uint4* dst, const uint4* 
 
 #define SPK_SIZES(X) X(16) X(32) X(64) X(128) X(256) X(512) X(1024) X(2048) X(4096)
 
+// launch a pass kernel, through hipExtLaunchKernelGGL when timing events are set
+template <class K>
+static void launch_pass(K kernel, dim3 grid, dim3 block, hipStream_t st, const PassArgs& a) {
+  if (g_launch_events[0]) {
+    hipExtLaunchKernelGGL(kernel, grid, block, 0, st, g_launch_events[0], g_launch_events[1], 0,
+                          a);
+    g_launch_events[0] = g_launch_events[1] = nullptr;
+  } else {
+    hipLaunchKernelGGL(kernel, grid, block, 0, st, a);
+  }
+}
+
 template <int N, int TT, int E>
 static hipError_t colpass_nt(int mode, const PassArgs& a, int frames, int zdim,
                              hipStream_t st) {
   dim3 grid(N / TT, frames, zdim), block((N / E) * TT);
   switch (mode) {
-    case 0: hipLaunchKernelGGL((k_colpass<N, TT, E, 0>), grid, block, 0, st, a); break;
-    case 2: hipLaunchKernelGGL((k_colpass<N, TT, E, 2>), grid, block, 0, st, a); break;
-    case 3: hipLaunchKernelGGL((k_colpass<N, TT, E, 3>), grid, block, 0, st, a); break;
+    case 0: launch_pass(k_colpass<N, TT, E, 0>, grid, block, st, a); break;
+    case 2: launch_pass(k_colpass<N, TT, E, 2>, grid, block, st, a); break;
+    case 3: launch_pass(k_colpass<N, TT, E, 3>, grid, block, st, a); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
@@ -646,13 +695,13 @@ static hipError_t rowpass_nt(bool use_t, bool emit_b, const PassArgs& a, int fra
                              hipStream_t st) {
   dim3 grid(N / TT, frames), block((N / E) * TT);
   if (use_t && emit_b)
-    hipLaunchKernelGGL((k_rowpass<N, TT, E, true, true>), grid, block, 0, st, a);
+    launch_pass(k_rowpass<N, TT, E, true, true>, grid, block, st, a);
   else if (use_t)
-    hipLaunchKernelGGL((k_rowpass<N, TT, E, true, false>), grid, block, 0, st, a);
+    launch_pass(k_rowpass<N, TT, E, true, false>, grid, block, st, a);
   else if (emit_b)
-    hipLaunchKernelGGL((k_rowpass<N, TT, E, false, true>), grid, block, 0, st, a);
+    launch_pass(k_rowpass<N, TT, E, false, true>, grid, block, st, a);
   else
-    hipLaunchKernelGGL((k_rowpass<N, TT, E, false, false>), grid, block, 0, st, a);
+    launch_pass(k_rowpass<N, TT, E, false, false>, grid, block, st, a);
   return hipGetLastError();
 }
 
@@ -673,11 +722,18 @@ static hipError_t rowfft_nt(int mode, const c32* in, c32* out, const float* sup,
 template <int N, int TT>
 static hipError_t shrinkwrap_nt(const c32* psi, float* sup, long long sstride, float* tmp,
                                 float* blur, float* maxp, const SwParam* tab, const SwParam& p0,
-                                const int* fmap, int frames, hipStream_t st) {
+                                const int* fmap, int frames, int rmax, hipStream_t st) {
   dim3 grid(N / TT, frames);
-  hipLaunchKernelGGL((k_blur_rows<N, TT>), grid, dim3(256), 0, st, psi, tmp, tab, p0, fmap);
-  hipLaunchKernelGGL((k_blur_cols<N, TT>), grid, dim3(256), 0, st, tmp, blur, maxp, tab, p0,
-                     fmap);
+  // register window sized for the largest radius among the frames
+  if (rmax <= 24) {
+    hipLaunchKernelGGL((k_blur_rows<N, TT, 24>), grid, dim3(256), 0, st, psi, tmp, tab, p0, fmap);
+    hipLaunchKernelGGL((k_blur_cols<N, TT, 24>), grid, dim3(256), 0, st, tmp, blur, maxp, tab,
+                       p0, fmap);
+  } else {
+    hipLaunchKernelGGL((k_blur_rows<N, TT, 48>), grid, dim3(256), 0, st, psi, tmp, tab, p0, fmap);
+    hipLaunchKernelGGL((k_blur_cols<N, TT, 48>), grid, dim3(256), 0, st, tmp, blur, maxp, tab,
+                       p0, fmap);
+  }
   const long long count = (long long)N * N;
   dim3 g2((unsigned)((count + 1023) / 1024), frames);
   hipLaunchKernelGGL(k_sw_threshold, g2, dim3(256), 0, st, sup, sstride, blur, maxp,
@@ -710,7 +766,8 @@ constexpr bool shape_ok() {
                         const int* fmap, int frames, hipStream_t st);                       \
   hipError_t shrinkwrap_##n(int tile, const c32* psi, float* sup, long long sstride,       \
                             float* tmp, float* blur, float* maxp, const SwParam* tab,      \
-                            const SwParam& p0, const int* fmap, int frames, hipStream_t st);
+                            const SwParam& p0, const int* fmap, int frames, int rmax,      \
+                            hipStream_t st);
 
 // a shape that is not built compiles to an error return, not a kernel
 #define SPK_CALL_COL(TT_, E_)                                                   \
@@ -748,12 +805,13 @@ constexpr bool shape_ok() {
   }                                                                                         \
   hipError_t shrinkwrap_##n(int tile, const c32* psi, float* sup, long long sstride,       \
                             float* tmp, float* blur, float* maxp, const SwParam* tab,      \
-                            const SwParam& p0, const int* fmap, int frames, hipStream_t st) { \
+                            const SwParam& p0, const int* fmap, int frames, int rmax,      \
+                            hipStream_t st) {                                               \
     if (tile == 4) return shrinkwrap_nt<n, 4>(psi, sup, sstride, tmp, blur, maxp, tab, p0, \
-                                              fmap, frames, st);                            \
+                                              fmap, frames, rmax, st);                      \
     if (tile == 8 && n != 4096)                                                             \
       return shrinkwrap_nt<n, (n == 4096 ? 4 : 8)>(psi, sup, sstride, tmp, blur, maxp, tab, \
-                                                   p0, fmap, frames, st);                   \
+                                                   p0, fmap, frames, rmax, st);             \
     return hipErrorInvalidValue;                                                            \
   }
 
@@ -788,6 +846,8 @@ SPK_SIZED_DEF(4096)
 #endif
 
 #if SPK_BUILD_COMMON
+
+thread_local hipEvent_t g_launch_events[2] = {nullptr, nullptr};
 
 bool size_supported(int n, int tile, int ept) {
   if (tile != 4 && tile != 8) return false;
@@ -831,7 +891,7 @@ hipError_t launch_rowfft(int n, int tile, int ept, int mode, const c32* in, c32*
 hipError_t launch_shrinkwrap(int n, int tile, const c32* psi, float* sup, long long sstride,
                              float* tmp, float* blur, float* maxp, const SwParam* tab,
                              const SwParam& p0, const int* fmap, int frame0, int frames,
-                             hipStream_t st) {
+                             int rmax, hipStream_t st) {
   if (!fmap) {  // a contiguous range of frames starting at frame0
     const size_t fo = (size_t)frame0 * n * n;
     psi += fo;
@@ -841,7 +901,7 @@ hipError_t launch_shrinkwrap(int n, int tile, const c32* psi, float* sup, long l
     maxp += (size_t)frame0 * (n / tile);
   }
   SPK_SIZE_SWITCH(shrinkwrap, tile, psi, sup, sstride, tmp, blur, maxp, tab, p0, fmap, frames,
-                  st)
+                  rmax, st)
 }
 
 hipError_t launch_pack_c64(int n, int tile, c32* dst, const c32* src, const float* sup,

@@ -200,6 +200,17 @@ class Plan:
         check(self.lib.spk_fft2(self.handle, _ptr(xx), _ptr(o), cnt, int(inverse)))
         return o[0] if squeeze else o
 
+    def capture(self, fn) -> "Graph":
+        """Record the plan calls made by fn() into a HIP graph (not run)."""
+        check(self.lib.spk_graph_begin(self.handle))
+        try:
+            fn()
+        finally:
+            h = c_void_p()
+            rc = self.lib.spk_graph_end(self.handle, byref(h))
+        check(rc)
+        return Graph(self.lib, h, self)
+
     def profile(self, enable: bool) -> None:
         check(self.lib.spk_plan_profile(self.handle, int(enable)))
 
@@ -219,6 +230,29 @@ class Plan:
     def close(self) -> None:
         if getattr(self, "handle", None):
             self.lib.spk_plan_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        if sys.is_finalizing():
+            return
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Graph:
+    """A captured sequence of plan calls; launch() replays it."""
+
+    def __init__(self, lib, handle, plan):
+        self.lib, self.handle, self.plan = lib, handle, plan
+
+    def launch(self) -> None:
+        check(self.lib.spk_graph_launch(self.handle))
+
+    def close(self) -> None:
+        if getattr(self, "handle", None):
+            self.lib.spk_graph_destroy(self.handle)
             self.handle = None
 
     def __del__(self):

@@ -202,7 +202,7 @@ class Server:
         self.queue = waiting
 
     def tick(self):
-        """Admit, launch one block on every busy pool, wait, retire."""
+        """Admit, launch one block on every busy pool, wait, retire (lockstep)."""
         now = self.now()
         self._admit(now)
         launched = [p for p in self.pools.values() if p.launch(self.block)]
@@ -212,22 +212,52 @@ class Server:
         out = []
         for p in launched:
             out.extend(p.retire(now))
+            p.event = None
         self.results.extend(out)
         return out
 
     def busy(self):
         return bool(self.queue) or any(p.active for p in self.pools.values())
 
-    def run(self, requests, clock_start=None):
-        """Serve requests (arrival times relative to now); returns results."""
+    def poll(self):
+        """One pass of the asynchronous loop: each pool whose block finished is
+        retired and relaunched on its own, so a size class never waits for
+        another.  Returns True if anything progressed."""
+        progressed = False
+        now = self.now()
+        self._admit(now)
+        for p in self.pools.values():
+            if p.event is not None:
+                if not p.event.query():
+                    continue
+                now = self.now()
+                self.results.extend(p.retire(now))
+                p.event = None
+                self._admit(now)
+                progressed = True
+            if p.active and p.launch(self.block):
+                progressed = True
+        return progressed
+
+    def run(self, requests, clock_start=None, lockstep=False):
+        """Serve requests (arrival times relative to the start); returns results."""
         self.t0 = time.perf_counter() if clock_start is None else clock_start
         for r in sorted(requests, key=lambda r: r.arrival):
             self.submit(r)
-        while self.busy():
-            if not any(p.active for p in self.pools.values()):
-                nxt = min(r.arrival for r in self.queue)
-                wait = nxt - self.now()
-                if wait > 0:
-                    time.sleep(min(wait, 0.01))
-            self.tick()
+        while self.busy() or any(p.event is not None for p in self.pools.values()):
+            if lockstep:
+                if not any(p.active for p in self.pools.values()) and self.queue:
+                    wait = min(r.arrival for r in self.queue) - self.now()
+                    if wait > 0:
+                        time.sleep(min(wait, 0.01))
+                self.tick()
+                continue
+            if not self.poll():
+                idle = all(p.event is None and not p.active for p in self.pools.values())
+                if idle and self.queue:
+                    wait = min(r.arrival for r in self.queue) - self.now()
+                    if wait > 0:
+                        time.sleep(min(wait, 0.005))
+                else:
+                    time.sleep(20e-6)
         return self.results

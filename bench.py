@@ -57,7 +57,34 @@ def parse():
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--requests", type=int, default=2000)
     ap.add_argument("--budget-gb", type=float, default=48.0)
+    ap.add_argument("--kernel-probe", action="store_true",
+                    help="only a 256 MiB device copy (HBM counter calibration) and 20 "
+                         "iterations of the headline frame, for rocprofv3 --pmc runs")
     return ap.parse_args()
+
+
+def kernel_probe(ctx):
+    """A copy of known size, then 20 headline iterations (single frame)."""
+    import torch
+    from als_speckle_amd.device import Plan
+    from als_speckle_amd import synth
+    nbytes = 256 << 20
+    src = torch.ones(nbytes // 4, dtype=torch.float32, device=ctx.device)
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        ctx.copy_bytes(dst, src)
+    frame = synth.config_frame("single_1024", 0)
+    plan = Plan(ctx, 1024, batch=1, shared=True)
+    plan.set_modulus(torch.from_numpy(frame.modulus).to(ctx.device))
+    plan.set_support(torch.from_numpy(frame.support).to(ctx.device))
+    np.random.seed(1)
+    g = (np.random.rand(1024, 1024) + 1j * np.random.rand(1024, 1024)).astype(np.complex64)
+    plan.set_psi(torch.from_numpy(g).to(ctx.device))
+    steps, _ = headline_steps(20)
+    plan.iterate(steps)
+    ctx.sync()
+    print(json.dumps({"kernel_probe": True, "copy_bytes": nbytes, "copies": 3,
+                      "iterations": 20, "n": 1024}), flush=True)
 
 
 def log(msg):
@@ -91,14 +118,18 @@ def cpu_baseline(frame, iters):
                       f"FFT per _fft2 call, single-threaded ({cores} host cores visible)"}
 
 
-def bench_single(ctx, a):
+def bench_single(a):
+    """Single 1024^2 reconstructions on a dedicated stream, eager and as a
+    replayed HIP graph of the whole reconstruction."""
     import torch
-    from als_speckle_amd.device import Plan
+    from als_speckle_amd.device import Context, Plan
     from als_speckle_amd.masking import bounding_box
     from als_speckle_amd import synth
 
     n, iters = 1024, 1000
     frame = synth.config_frame("single_1024", 0)
+    stream = torch.cuda.Stream()
+    ctx = Context(0, stream=stream)
     dev = ctx.device
     m_d = torch.from_numpy(frame.modulus).to(dev)
     s_d = torch.from_numpy(frame.support).to(dev)
@@ -109,6 +140,7 @@ def bench_single(ctx, a):
     savebuf = torch.zeros((a.steps + a.warmup + 2, r1 - r0, c1 - c0), dtype=torch.complex64,
                           device=dev)
     err = torch.empty((iters, 1), dtype=torch.float64, device=dev)
+    torch.cuda.synchronize()
     steps, nsw = headline_steps(iters)
     plan = Plan(ctx, n, batch=1, shared=True)
     plan.set_modulus(m_d)
@@ -119,29 +151,42 @@ def bench_single(ctx, a):
         plan.iterate(steps, err_out=err)
         plan.copy_roi(0, savebuf, slot, r0, c0, r1 - r0, c1 - c0)
 
-    for w in range(a.warmup):
-        reconstruction(w)
-    ctx.sync()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for k in range(a.steps):
-        reconstruction(a.warmup + k)
-    ctx.sync()
-    torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
-    final_err = float(err[-1, 0].item())
+    with torch.cuda.stream(stream):
+        for w in range(a.warmup):
+            reconstruction(w)
+        ctx.sync()
+        t0 = time.perf_counter()
+        for k in range(a.steps):
+            reconstruction(a.warmup + k)
+        ctx.sync()
+        dt_eager = time.perf_counter() - t0
+        final_err = float(err[-1, 0].item())
 
-    plan.profile(True)
-    reconstruction(a.warmup + a.steps)
-    ncol, tcol = plan.profile_read(0)
-    nrow, trow = plan.profile_read(1)
-    plan.profile(False)
+        graph = plan.capture(lambda: reconstruction(a.steps + a.warmup + 1))
+        graph.launch()
+        ctx.sync()
+        t0 = time.perf_counter()
+        for k in range(a.steps):
+            graph.launch()
+        ctx.sync()
+        dt_graph = time.perf_counter() - t0
+        graph_err = float(err[-1, 0].item())
+        graph.close()
+
+        plan.profile(True)
+        reconstruction(a.warmup + a.steps)
+        ncol, tcol = plan.profile_read(0)
+        nrow, trow = plan.profile_read(1)
+        plan.profile(False)
     avg = {"column": tcol / ncol * 1e-3, "row": trow / nrow * 1e-3}
     byts = {"column": B_COL * n * n, "row": B_ROW * n * n}
     dom = max(avg, key=lambda k: avg[k])
     model = iters * (B_COL + B_ROW) * n * n + nsw * B_SHRINK * n * n + B_ONEOFF * n * n
-    return dict(frame=frame, n=n, iters=iters, dt=dt, final_err=final_err, avg=avg, byts=byts,
-                dom=dom, model=model)
+    assert graph_err == final_err, "graph replay differs from the eager reconstruction"
+    dt = min(dt_eager, dt_graph)
+    return dict(frame=frame, n=n, iters=iters, dt=dt, dt_eager=dt_eager, dt_graph=dt_graph,
+                launch="hipGraph" if dt_graph < dt_eager else "eager",
+                final_err=final_err, avg=avg, byts=byts, dom=dom, model=model)
 
 
 def bench_trials_512(ctx):
@@ -292,8 +337,11 @@ def main():
         sys.exit(2)
     from als_speckle_amd.device import Context
     ctx = Context(0)
+    if a.kernel_probe:
+        kernel_probe(ctx)
+        return
     log("single 1024^2 reconstructions")
-    s = bench_single(ctx, a)
+    s = bench_single(a)
     n, iters, dt = s["n"], s["iters"], s["dt"]
     its_per_s = iters * a.steps / dt
     plain_rps = a.steps / dt
@@ -327,6 +375,9 @@ def main():
                         "(beta 0.8, numpy semantics), shrinkwrap every 20 (sigma 3->1.5, "
                         "tau 0.2), FP64 error trace, polygon support 15% fill, Poisson 1e8",
             "n": n, "iterations": iters, "recons_per_s": round(plain_rps, 3),
+            "launch": s["launch"],
+            "eager_iterations_per_s": round(iters * a.steps / s["dt_eager"], 1),
+            "graph_iterations_per_s": round(iters * a.steps / s["dt_graph"], 1),
             "final_modulus_error": s["final_err"],
             "model_bytes_per_recon": s["model"],
             "model_hbm_frac_whole_recon": round(s["model"] * a.steps / dt / HBM_PEAK, 4),

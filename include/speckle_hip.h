@@ -146,6 +146,23 @@ int spk_copy_roi(spk_plan* plan, int frame, float* dst_c64, int slots,
                  int slot, int r0, int c0, int rows, int cols);
 
 /* -------------------------------------------------------------------------
+ * HIP graphs.  spk_graph_begin starts stream capture on the plan's stream
+ * (which must not be the null stream); the plan calls made until
+ * spk_graph_end (spk_set_support / spk_set_psi / spk_iterate with
+ * per_frame = 0 / spk_copy_roi) are recorded instead of run, and
+ * spk_graph_end instantiates them.  spk_graph_launch replays the recorded
+ * work on the plan's stream with the same pointers and step coefficients.
+ * Workspace must already exist: run the same calls once before capturing.
+ * No counterpart in the reference (its launches block one by one,
+ * gpu.py:349).
+ * ---------------------------------------------------------------------- */
+typedef struct spk_graph spk_graph;
+int spk_graph_begin(spk_plan* plan);
+int spk_graph_end(spk_plan* plan, spk_graph** out);
+int spk_graph_launch(spk_graph* graph);
+int spk_graph_destroy(spk_graph* graph);
+
+/* -------------------------------------------------------------------------
  * Standalone transform.  Replaces pyfft.cl.Plan.execute(data_in, data_out,
  * inverse) (phasing.py:1074-1075): count frames of n x n c64, row-major,
  * out-of-place or in-place.  Forward unnormalised; inverse scaled by 1/n^2
@@ -174,10 +191,11 @@ int spk_stack_mean(spk_ctx* ctx, const float* data_c64, int frames, int rows,
 /* plain device copy of `bytes` (multiple of 16), for HBM counter calibration */
 int spk_copy_bytes(spk_ctx* ctx, void* dst, const void* src, size_t bytes);
 /* Per-pass timing inside spk_iterate: with enable != 0 every column-pass and
- * row-pass launch of this plan is bracketed by HIP events on the context's
- * stream.  spk_plan_profile_read synchronises, returns the number of launches
- * and their summed milliseconds for pass 0 (column) or 1 (row), and clears
- * that pass's record. */
+ * row-pass launch of this plan is made with hipExtLaunchKernelGGL and a
+ * start/stop HIP event pair recorded by the dispatch itself (the kernel's own
+ * execution interval).  spk_plan_profile_read synchronises, returns the number
+ * of launches and their summed milliseconds for pass 0 (column) or 1 (row),
+ * and clears that pass's record. */
 int spk_plan_profile(spk_plan* plan, int enable);
 int spk_plan_profile_read(spk_plan* plan, int pass, int* launches,
                           double* total_ms);

@@ -318,3 +318,43 @@ def test_context_outlives_plans():
     p.set_modulus(dev(f.modulus, torch.float32))
     p.close()
     q.close()
+
+
+def test_graph_replay_matches_eager():
+    """A captured reconstruction block replays to the eager result, every time,
+    with eager work and synchronisations between the replays."""
+    from als_speckle_amd.device import Context
+    n = 128
+    f = frame(n, seed=12)
+    psi = start_psi(n, f.support, seed=4)
+    stream = torch.cuda.Stream()
+    ctx = Context(0, stream=stream)
+    p = Plan(ctx, n)
+    m, s, g = dev(f.modulus, torch.float32), dev(f.support, torch.float32), dev(psi, torch.complex64)
+    err = torch.empty((30, 1), dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    sched = [step("hio", 0.9, "numpy", 2.0 if k == 19 else 0.0, 0.2) for k in range(25)] + \
+            [step("er", 0.9)] * 5
+
+    def block():
+        p.set_support(s)
+        p.set_psi(g, mult_support=False)
+        p.iterate(sched, err_out=err)
+
+    with torch.cuda.stream(stream):
+        p.set_modulus(m)
+        block()
+        ctx.sync()
+        want_psi = p.get_psi().cpu().numpy()
+        want_err = err.cpu().numpy().copy()
+        graph = p.capture(block)
+        for rep in range(3):
+            err.zero_()
+            graph.launch()
+            ctx.sync()
+            assert np.array_equal(p.get_psi().cpu().numpy(), want_psi), rep
+            assert np.array_equal(err.cpu().numpy(), want_err), rep
+            q = load_plan(ctx, f, psi)   # unrelated eager work in between
+            q.iterate([step("raar", 0.8)] * 3)
+            ctx.sync()
+    graph.close()
