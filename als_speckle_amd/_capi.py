@@ -61,6 +61,10 @@ SIGNATURES = {
     "spk_align_global_phase": (c_int, [c_void_p, c_void_p, c_int, c_int, c_int]),
     "spk_copy_bytes": (c_int, [c_void_p, c_void_p, c_void_p, c_size_t]),
     "spk_stack_mean": (c_int, [c_void_p, c_void_p, c_int, c_int, c_int, c_void_p]),
+    "spk_blur2d": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_int, c_float]),
+    "spk_prtf": (c_int, [c_void_p, c_void_p, c_int, c_int, c_int, c_void_p]),
+    "spk_rftf": (c_int, [c_void_p, c_void_p, c_int, c_int, c_void_p, c_int, c_void_p]),
+    "spk_rftf_stack": (c_int, [c_void_p, c_void_p, c_int, c_int, c_int, c_void_p, c_void_p]),
     "spk_time_passes": (c_int, [c_void_p, c_int, c_int, POINTER(SpkStep), POINTER(c_float)]),
     "spk_plan_profile": (c_int, [c_void_p, c_int]),
     "spk_graph_begin": (c_int, [c_void_p]),

@@ -87,8 +87,10 @@ struct spk_plan {
   spk_ctx* ctx = nullptr;
   int n = 0, batch = 0, shared = 0, tile = 8, ept = 16, bands = 0;
   c32* psi = nullptr;
-  c32* phi = nullptr;
+  c32* phi = nullptr;         // row transform, tiled (row pass input)
+  c32* phic = nullptr;        // row transform, column-band-major (column pass input)
   c32* phib = nullptr;
+  c32* phibc = nullptr;
   float* mod = nullptr;
   float* sup = nullptr;
   c32* tw = nullptr;
@@ -114,6 +116,10 @@ struct spk_plan {
   bool phib_fresh = false;    // phib == rowFFT(dm1(psi)) for the next DM step
   float phib_g = 0.0f;
   long long nn() const { return (long long)n * n; }
+  // Phi kept in two layouts (see spk_fft_core.hpp dual_layout): n >= 4096
+  bool dual() const { return n >= 4096; }
+  c32* phi_in() const { return dual() ? phic : phi; }    // column pass input
+  c32* phib_in() const { return dual() ? phibc : phib; }
   long long mstride() const { return shared ? 0 : nn(); }
 };
 
@@ -190,7 +196,9 @@ PassArgs base_args(spk_plan* p) {
   PassArgs a{};
   a.psi = p->psi;
   a.phi = p->phi;
+  a.phic = p->phi_in();
   a.phib = p->phib;
+  a.phibc = p->phib_in();
   a.mod = p->mod;
   a.sup = p->sup;
   a.mstride = p->mstride();
@@ -204,7 +212,9 @@ PassArgs base_args(spk_plan* p) {
 
 int ensure_phib(spk_plan* p) {
   if (p->phib) return SPK_OK;
-  return dalloc(&p->phib, (size_t)p->batch * p->nn());
+  int rc = dalloc(&p->phib, (size_t)p->batch * p->nn());
+  if (rc || !p->dual()) return rc;
+  return dalloc(&p->phibc, (size_t)p->batch * p->nn());
 }
 
 int ensure_sw(spk_plan* p) {
@@ -219,8 +229,8 @@ int ensure_sw(spk_plan* p) {
 // phi = rowFFT(psi) for frames [f0, f0 + nf)
 int rowfft_frames(spk_plan* p, int f0, int nf) {
   const size_t fo = (size_t)f0 * p->nn();
-  SPK_HIP(launch_rowfft(p->n, p->tile, p->ept, 0, p->psi + fo, p->phi + fo, nullptr, 0, 0.0f,
-                        p->tw, nullptr, nf, p->ctx->stream));
+  SPK_HIP(launch_rowfft(p->n, p->tile, p->ept, 0, p->psi + fo, p->phi_in() + fo, nullptr, 0,
+                        0.0f, p->tw, nullptr, nf, p->ctx->stream));
   return SPK_OK;
 }
 
@@ -358,8 +368,9 @@ int spk_plan_bytes(int n, int batch, int shared, int with_dm, size_t* bytes) {
   const size_t nn = (size_t)n * n;
   const size_t frames_ms = shared ? 1 : (size_t)batch;
   size_t b = 0;
-  b += 2 * nn * batch * sizeof(c32);             // psi, phi
-  if (with_dm) b += nn * batch * sizeof(c32);    // phib
+  const size_t lay = n >= 4096 ? 2 : 1;          // phi layouts kept
+  b += (1 + lay) * nn * batch * sizeof(c32);     // psi, phi
+  if (with_dm) b += lay * nn * batch * sizeof(c32);  // phib
   b += 2 * nn * frames_ms * sizeof(float);       // mod, sup
   b += n * sizeof(c32) + 2 * 1024 * batch * sizeof(double);
   *bytes = b;
@@ -393,6 +404,7 @@ int spk_plan_create(spk_ctx* ctx, int n, int batch, int shared, const spk_tune* 
   const size_t nn = (size_t)n * n;
   const size_t fms = shared ? 1 : (size_t)batch;
   if ((rc = dalloc(&p->psi, nn * batch)) || (rc = dalloc(&p->phi, nn * batch)) ||
+      (p->dual() && (rc = dalloc(&p->phic, nn * batch))) ||
       (rc = dalloc(&p->mod, nn * fms)) || (rc = dalloc(&p->sup, nn * fms)) ||
       (rc = dalloc(&p->tw, (size_t)n)) || (rc = dalloc(&p->summ2, (size_t)batch)) ||
       (rc = dalloc(&p->sumpart, (size_t)64 * batch))) {
@@ -422,7 +434,8 @@ int spk_plan_destroy(spk_plan* p) {
     if (p->ctx->stream) (void)hipStreamSynchronize(p->ctx->stream);
     else (void)hipDeviceSynchronize();
   }
-  dfree(p->psi); dfree(p->phi); dfree(p->phib); dfree(p->mod); dfree(p->sup);
+  dfree(p->psi); dfree(p->phi); dfree(p->phic); dfree(p->phib); dfree(p->phibc);
+  dfree(p->mod); dfree(p->sup);
   dfree(p->tw); dfree(p->summ2); dfree(p->sumpart); dfree(p->errp);
   dfree(p->sw_tmp); dfree(p->sw_blur); dfree(p->sw_maxp); dfree(p->fftbuf);
   dfree(p->stage_d);
@@ -665,7 +678,7 @@ int spk_iterate(spk_plan* p, const spk_step* steps, int nsteps, int per_frame,
     const int nact = map_n[k];
     if (any_dm[k] && !p->phib_fresh) {
       // phib = rowFFT((s - gamma_m (1-s)) psi), dm1 (phasing_dm1.cl:16)
-      SPK_HIP(launch_rowfft(p->n, p->tile, p->ept, 2, p->psi, p->phib, p->sup, p->mstride(),
+      SPK_HIP(launch_rowfft(p->n, p->tile, p->ept, 2, p->psi, p->phib_in(), p->sup, p->mstride(),
                             dm_g[k], p->tw, per_frame ? (const int*)dptr(dm_off[k]) : nullptr,
                             per_frame ? dm_n[k] : B, st));
     }
@@ -732,23 +745,27 @@ int spk_fft2(spk_plan* p, const float* in, float* out, int count, int inverse) {
   SPK_REQUIRE(count >= 1, SPK_EINVAL, "count must be >= 1");
   int rc = set_device(p->ctx);
   if (rc) return rc;
+  // two work areas: tiled, then column-band-major after the row transform
   if ((size_t)count > p->fftbuf_frames) {
     SPK_HIP(hipStreamSynchronize(p->ctx->stream));
     dfree(p->fftbuf);
     p->fftbuf_frames = 0;
-    if ((rc = dalloc(&p->fftbuf, (size_t)count * p->nn()))) return rc;
+    if ((rc = dalloc(&p->fftbuf, 2 * (size_t)count * p->nn()))) return rc;
     p->fftbuf_frames = count;
   }
   hipStream_t st = p->ctx->stream;
-  SPK_HIP(launch_pack_c64(p->n, p->tile, p->fftbuf, (const c32*)in, nullptr, 0, count, st));
-  SPK_HIP(launch_rowfft(p->n, p->tile, p->ept, inverse ? 1 : 0, p->fftbuf, p->fftbuf, nullptr,
-                        0, 0.0f, p->tw, nullptr, count, st));
+  c32* tiled = p->fftbuf;
+  c32* colband = p->dual() ? p->fftbuf + (size_t)count * p->nn() : tiled;
+  SPK_HIP(launch_pack_c64(p->n, p->tile, tiled, (const c32*)in, nullptr, 0, count, st));
+  SPK_HIP(launch_rowfft(p->n, p->tile, p->ept, inverse ? 1 : 0, tiled, colband, nullptr, 0, 0.0f,
+                        p->tw, nullptr, count, st));
   PassArgs a = base_args(p);
-  a.phi = p->fftbuf;
+  a.phic = colband;
+  a.phi = tiled;
   a.c.flags = kActive;
   SPK_HIP(launch_colpass(p->n, p->tile, p->ept, inverse ? 3 : 2, a, count, 1, st));
   const float scl = inverse ? (float)(1.0 / ((double)p->n * p->n)) : 1.0f;
-  SPK_HIP(launch_unpack_c64(p->n, p->tile, (c32*)out, p->fftbuf, scl, count, st));
+  SPK_HIP(launch_unpack_c64(p->n, p->tile, (c32*)out, tiled, scl, count, st));
   return SPK_OK;
 }
 
@@ -768,6 +785,134 @@ int spk_align_global_phase(spk_ctx* ctx, float* data, int frames, int rows, int 
     ctx->scratch_cap = need;
   }
   SPK_HIP(launch_align((c32*)data, frames, per, ctx->scratch, nblk, ctx->stream));
+  return SPK_OK;
+}
+
+namespace {
+// device scratch of a context, grown on demand (queued work is drained first)
+int ctx_scratch(spk_ctx* ctx, size_t bytes, void** out) {
+  const size_t need = (bytes + 7) / 8;
+  if (need > ctx->scratch_cap) {
+    if (ctx->stream) SPK_HIP(hipStreamSynchronize(ctx->stream));
+    else SPK_HIP(hipDeviceSynchronize());
+    dfree(ctx->scratch);
+    ctx->scratch_cap = 0;
+    int rc = dalloc(&ctx->scratch, need);
+    if (rc) return rc;
+    ctx->scratch_cap = need;
+  }
+  *out = ctx->scratch;
+  return SPK_OK;
+}
+
+// taps of the reference's blur kernel along one axis of length n:
+// K[d] = exp(-(((d - h) mod n) - h)^2 / (2 sigma^2)), h = n // 2 (the centred
+// Gaussian of shape.gaussian after np.fft.fftshift), normalised over the axis
+void blur_taps(int n, double sigma, std::vector<int>& off, std::vector<float>& w) {
+  const int h = n / 2;
+  std::vector<double> k(n);
+  double tot = 0.0, mx = 0.0;
+  for (int d = 0; d < n; ++d) {
+    const double x = (double)(((d - h) % n + n) % n - h);
+    k[d] = std::exp(-x * x / (2.0 * sigma * sigma));
+    tot += k[d];
+    mx = std::max(mx, k[d]);
+  }
+  off.clear();
+  w.clear();
+  for (int d = 0; d < n; ++d)
+    if (k[d] > 1e-16 * mx) {
+      off.push_back(d);
+      w.push_back((float)(k[d] / tot));
+    }
+}
+}  // namespace
+
+int spk_blur2d(spk_ctx* ctx, const float* in, float* out, int rows, int cols, float sigma) {
+  SPK_REQUIRE(in && out && rows >= 1 && cols >= 1, SPK_EINVAL, "bad image");
+  SPK_REQUIRE(sigma > 0.0f && std::isfinite(sigma), SPK_EINVAL, "blur sigma must be > 0");
+  int rc = set_device(ctx);
+  if (rc) return rc;
+  std::vector<int> orow, ocol;
+  std::vector<float> wrow, wcol;
+  blur_taps(rows, sigma, orow, wrow);
+  blur_taps(cols, sigma, ocol, wcol);
+  const size_t img = (size_t)rows * cols * sizeof(float);
+  const size_t tabs = (orow.size() + ocol.size()) * 8;
+  void* scr;
+  if ((rc = ctx_scratch(ctx, img + tabs + 64, &scr))) return rc;
+  float* tmp = (float*)scr;
+  int* d_orow = (int*)((char*)scr + img);
+  int* d_ocol = d_orow + orow.size();
+  float* d_wrow = (float*)(d_ocol + ocol.size());
+  float* d_wcol = d_wrow + wrow.size();
+  std::vector<unsigned char> host(tabs);
+  unsigned char* h = host.data();
+  std::memcpy(h, orow.data(), orow.size() * 4);
+  std::memcpy(h + orow.size() * 4, ocol.data(), ocol.size() * 4);
+  std::memcpy(h + (orow.size() + ocol.size()) * 4, wrow.data(), wrow.size() * 4);
+  std::memcpy(h + (orow.size() + ocol.size() + wrow.size()) * 4, wcol.data(), wcol.size() * 4);
+  // synchronous copy: the host vector goes out of scope at return
+  SPK_HIP(hipMemcpyAsync(d_orow, h, tabs, hipMemcpyHostToDevice, ctx->stream));
+  SPK_HIP(hipStreamSynchronize(ctx->stream));
+  SPK_HIP(launch_conv_axis(in, tmp, rows, cols, 1, d_ocol, d_wcol, (int)ocol.size(), ctx->stream));
+  SPK_HIP(launch_conv_axis(tmp, out, rows, cols, 0, d_orow, d_wrow, (int)orow.size(), ctx->stream));
+  return SPK_OK;
+}
+
+int spk_prtf(spk_plan* p, const float* stack, int frames, int rows, int cols, float* out) {
+  SPK_REQUIRE(p && stack && out && frames >= 1, SPK_EINVAL, "bad arguments");
+  SPK_REQUIRE(rows >= 1 && cols >= 1 && rows <= p->n && cols <= p->n, SPK_EINVAL,
+              "estimates larger than the plan's frame");
+  int rc = set_device(p->ctx);
+  if (rc) return rc;
+  const size_t nn = (size_t)p->nn();
+  void* scr;
+  if ((rc = ctx_scratch(p->ctx, 2 * frames * nn * sizeof(c32), &scr))) return rc;
+  c32* pad = (c32*)scr;
+  c32* F = pad + frames * nn;
+  SPK_HIP(launch_embed((const c32*)stack, pad, rows, cols, p->n, frames, p->ctx->stream));
+  if ((rc = spk_fft2(p, (const float*)pad, (float*)F, frames, 0))) return rc;
+  SPK_HIP(launch_prtf(F, frames, p->n, out, p->ctx->stream));
+  return SPK_OK;
+}
+
+int spk_rftf(spk_plan* p, const float* est, int rows, int cols, const float* goal, int scale,
+             float* out) {
+  SPK_REQUIRE(p && est && goal && out, SPK_EINVAL, "bad arguments");
+  SPK_REQUIRE(rows >= 1 && cols >= 1 && rows <= p->n && cols <= p->n, SPK_EINVAL,
+              "estimate larger than the goal modulus");
+  int rc = set_device(p->ctx);
+  if (rc) return rc;
+  const size_t nn = (size_t)p->nn();
+  const int nblk = 64;
+  void* scr;
+  if ((rc = ctx_scratch(p->ctx, 2 * nn * sizeof(c32) + 2 * nblk * sizeof(double), &scr)))
+    return rc;
+  c32* pad = (c32*)scr;
+  c32* F = pad + nn;
+  double* part = (double*)(F + nn);
+  SPK_HIP(launch_embed((const c32*)est, pad, rows, cols, p->n, 1, p->ctx->stream));
+  if ((rc = spk_fft2(p, (const float*)pad, (float*)F, 1, 0))) return rc;
+  SPK_HIP(launch_rftf(F, goal, p->n, part, nblk, scale, out, p->ctx->stream));
+  return SPK_OK;
+}
+
+int spk_rftf_stack(spk_plan* p, const float* stack, int frames, int rows, int cols,
+                   const float* goal, float* out) {
+  SPK_REQUIRE(p && stack && goal && out && frames >= 1, SPK_EINVAL, "bad arguments");
+  SPK_REQUIRE(rows >= 1 && cols >= 1 && rows <= p->n && cols <= p->n, SPK_EINVAL,
+              "estimates larger than the goal modulus");
+  int rc = set_device(p->ctx);
+  if (rc) return rc;
+  const size_t nn = (size_t)p->nn();
+  void* scr;
+  if ((rc = ctx_scratch(p->ctx, 2 * frames * nn * sizeof(c32), &scr))) return rc;
+  c32* pad = (c32*)scr;
+  c32* F = pad + frames * nn;
+  SPK_HIP(launch_embed((const c32*)stack, pad, rows, cols, p->n, frames, p->ctx->stream));
+  if ((rc = spk_fft2(p, (const float*)pad, (float*)F, frames, 0))) return rc;
+  SPK_HIP(launch_rftf_stack(F, goal, frames, p->n, out, p->ctx->stream));
   return SPK_OK;
 }
 

@@ -228,17 +228,38 @@ SPK_HD int col_pos(int band, int b, int t, int i) {
   return ((row / TT) * (N / TT) + band) * (TT * TT) + (row % TT) * TT + b;
 }
 
-// Modulus layout (read only by the column pass): column-band-major, a band
-// of TT columns is n*TT contiguous floats, row-major inside the band, so a
-// wavefront of the column pass reads it linearly.
+// Column-band-major layout: a band of TT columns is n*TT contiguous values,
+// row-major inside the band.  The modulus is always stored this way (only
+// the column pass reads it).  For Phi (the row transform) two schemes exist:
+//  - single layout (n <= 2048): Phi stays tiled; the row pass reads and
+//    writes whole contiguous row bands, the column pass reads and writes
+//    TT x TT tiles in place;
+//  - dual layout (n = 4096): the row pass writes Phi column-band-major and
+//    the column pass writes it back tiled, so every pass *reads* contiguous
+//    memory (at 4096 the tile stride of the single layout is 128 KB).
+// Measured on MI355X: strided writes cost more than strided reads up to
+// 2048, so the dual layout only pays at 4096.
+template <int N>
+constexpr bool dual_layout() {
+  return N >= 4096;
+}
 template <int TT>
 SPK_HD int mcb_pos(int n, int r, int c) {
   return (c / TT) * n * TT + r * TT + (c % TT);
 }
+// column pass: element i of thread t on column b of column-band `band`
 template <int N, int TT, int E = ept<N, TT>()>
 SPK_HD int mcol_pos(int band, int b, int t, int i) {
   constexpr int T = N / E;
   return band * N * TT + (t + i * T) * TT + b;
+}
+// row pass: element i of thread t on row b of row-band `band`, in the
+// column-band-major layout (where the row pass writes Phi)
+template <int N, int TT, int E = ept<N, TT>()>
+SPK_HD int cbrow_pos(int band, int b, int t, int i) {
+  constexpr int T = N / E;
+  const int col = t + i * T;
+  return (col / TT) * N * TT + (band * TT + b) * TT + (col % TT);
 }
 
 // LDS slot of element p of transform b (TT transforms per workgroup,

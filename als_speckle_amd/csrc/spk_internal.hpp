@@ -30,8 +30,10 @@ struct Coef {
 
 struct PassArgs {
   c32* psi;               // tiled iterate, frame stride n*n
-  c32* phi;               // tiled row-transform of psi
-  c32* phib;              // tiled row-transform of the DM modified estimate
+  c32* phi;               // Phi in the tiled layout: column pass writes, row pass reads
+  c32* phic;              // Phi column-band-major: row pass writes, column pass reads
+  c32* phib;              // DM modified estimate, tiled layout
+  c32* phibc;             // DM modified estimate, column-band-major
   const float* mod;       // column-band-major modulus, frame stride mstride (0 = shared)
   const float* sup;       // tiled support, frame stride sstride (0 = shared)
   long long mstride;
@@ -107,5 +109,16 @@ hipError_t launch_stack_mean(const c32* data, long long per_frame, int frames,
                              c32* out, hipStream_t st);
 hipError_t launch_copy(void* dst, const void* src, size_t bytes,
                        hipStream_t st);
+// post-processing (module functions of speckle.phasing)
+hipError_t launch_conv_axis(const float* in, float* out, int rows, int cols, int axis,
+                            const int* off, const float* w, int taps, hipStream_t st);
+hipError_t launch_abs_c64(const c32* in, float* out, long long n, hipStream_t st);
+hipError_t launch_embed(const c32* in, c32* out, int r, int c, int n, int frames,
+                        hipStream_t st);
+hipError_t launch_prtf(const c32* F, int frames, int n, float* out, hipStream_t st);
+hipError_t launch_rftf_stack(const c32* F, const float* goal, int frames, int n, float* out,
+                             hipStream_t st);
+hipError_t launch_rftf(const c32* F, const float* goal, int n, double* partial, int nblk,
+                       int scale, float* out, hipStream_t st);
 
 }  // namespace spk

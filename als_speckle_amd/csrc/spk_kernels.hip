@@ -165,13 +165,18 @@ __global__ void __launch_bounds__((N / E) * TT) k_colpass(PassArgs a) {
   if (!(c.flags & kActive)) return;
   if (z && !(c.flags & kUseT)) return;
   const size_t fo = (size_t)f * N * N;
-  c32* __restrict__ data = (z ? a.phib : a.phi) + fo;
+  // dual layout: read Phi column-band-major (contiguous), write it tiled for
+  // the row pass; single layout: tiles in place
+  constexpr bool DUAL = dual_layout<N>();
+  const c32* src = (z ? (DUAL ? a.phibc : a.phib) : (DUAL ? a.phic : a.phi)) + fo;
+  c32* dst = (z ? a.phib : a.phi) + fo;
   const int tid = threadIdx.x, b = tid % TT, t = tid / TT;
 
   // issue every global load up front: data, modulus, twiddle table
   c32 x[E];
 #pragma unroll
-  for (int i = 0; i < E; ++i) x[i] = data[col_pos<N, TT, E>(band, b, t, i)];
+  for (int i = 0; i < E; ++i)
+    x[i] = src[DUAL ? mcol_pos<N, TT, E>(band, b, t, i) : col_pos<N, TT, E>(band, b, t, i)];
   // the modulus is loaded with the data where registers allow (16 values per
   // thread), otherwise after the forward transform
   constexpr bool PREFETCH = E <= 16 && (N / E) * TT <= 512;
@@ -210,7 +215,7 @@ __global__ void __launch_bounds__((N / E) * TT) k_colpass(PassArgs a) {
     fft_regs<N, E, TT, true, false>(x, t, b, lds, tw);
   }
 #pragma unroll
-  for (int i = 0; i < E; ++i) data[col_pos<N, TT, E>(band, b, t, i)] = x[i];
+  for (int i = 0; i < E; ++i) dst[col_pos<N, TT, E>(band, b, t, i)] = x[i];
   if constexpr (MODE == 0) {
     if (z == 0 && a.errp) {
       const double s = block_sum(err, reinterpret_cast<double*>(lds));
@@ -290,19 +295,30 @@ __global__ void __launch_bounds__((N / E) * TT) k_rowpass(PassArgs a) {
     x[i] = nv;
     if constexpr (EMIT_B) y[i] = scale(nv, s - c.g_next * u);
   }
+  // the new row transforms: column-band-major for the column pass (dual
+  // layout) or back in place (single layout)
+  constexpr bool DUAL = dual_layout<N>();
   if constexpr (EMIT_B) {
     fft_regs<N, E, TT, false, false>(y, t, b, lds, tw);
 #pragma unroll
-    for (int i = 0; i < E; ++i) a.phib[fo + SPK_RPOS(i)] = y[i];
+    for (int i = 0; i < E; ++i) {
+      if constexpr (DUAL) a.phibc[fo + cbrow_pos<N, TT, E>(band, b, t, i)] = y[i];
+      else a.phib[fo + SPK_RPOS(i)] = y[i];
+    }
   }
   fft_regs<N, E, TT, false, false>(x, t, b, lds, tw);
 #pragma unroll
-  for (int i = 0; i < E; ++i) a.phi[fo + SPK_RPOS(i)] = x[i];
+  for (int i = 0; i < E; ++i) {
+    if constexpr (DUAL) a.phic[fo + cbrow_pos<N, TT, E>(band, b, t, i)] = x[i];
+    else a.phi[fo + SPK_RPOS(i)] = x[i];
+  }
 #undef SPK_RPOS
 }
 
 // ---------------------------------------------------------------------------
-// Row transform of whole frames (tiled in -> tiled out, may alias).
+// Row transform of whole frames: tiled in -> the column pass's input layout
+// (tiled, or column-band-major in the dual layout); in and out may alias
+// only in the single layout.
 // MODE 0 forward, 1 inverse (unscaled), 2 forward of (s - g(1-s)) * in
 // ---------------------------------------------------------------------------
 template <int N, int TT, int E, int MODE>
@@ -328,7 +344,10 @@ __global__ void __launch_bounds__((N / E) * TT)
   load_twiddles<N, TT>(twl, tw);
   fft_regs<N, E, TT, MODE == 1, true>(x, t, b, lds, twiddle_view<N, TT>(twl));
 #pragma unroll
-  for (int i = 0; i < E; ++i) out[fo + row_pos<N, TT, E>(band, b, t, i)] = x[i];
+  for (int i = 0; i < E; ++i) {
+    if constexpr (dual_layout<N>()) out[fo + cbrow_pos<N, TT, E>(band, b, t, i)] = x[i];
+    else out[fo + row_pos<N, TT, E>(band, b, t, i)] = x[i];
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -617,6 +636,163 @@ static __global__ void __launch_bounds__(256) k_align_apply(c32* data, long long
   for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < per_frame;
        i += (long long)gridDim.x * 256) {
     d[i] = mul(d[i], c32{fr, fi});
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Generic periodic 1-D convolution along one axis of a row-major
+// (rows, cols) float image: out[x] = sum_j w[j] * in[(x - off[j]) mod n]
+// along the axis (refine_support's FFT blur, phasing.py:1787-1790, written as
+// its equivalent circular convolution with the taps that matter).
+// axis 0: along rows (index r), axis 1: along columns (index c).
+// ---------------------------------------------------------------------------
+static __global__ void __launch_bounds__(256) k_conv_axis(const float* in, float* out, int rows,
+                                                          int cols, int axis, const int* off,
+                                                          const float* w, int taps) {
+  const long long total = (long long)rows * cols;
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < total;
+       i += (long long)gridDim.x * 256) {
+    const int r = (int)(i / cols), c = (int)(i % cols);
+    double acc = 0.0;
+    if (axis == 0) {
+      for (int j = 0; j < taps; ++j) {
+        int rr = r - off[j];
+        rr %= rows;
+        if (rr < 0) rr += rows;
+        acc += (double)w[j] * in[(size_t)rr * cols + c];
+      }
+    } else {
+      for (int j = 0; j < taps; ++j) {
+        int cc = c - off[j];
+        cc %= cols;
+        if (cc < 0) cc += cols;
+        acc += (double)w[j] * in[(size_t)r * cols + cc];
+      }
+    }
+    out[i] = (float)acc;
+  }
+}
+
+// |x| of a complex image into a float image
+static __global__ void __launch_bounds__(256) k_abs_c64(const c32* in, float* out, long long n) {
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n;
+       i += (long long)gridDim.x * 256)
+    out[i] = sqrtf(in[i].x * in[i].x + in[i].y * in[i].y);
+}
+
+// zero-pad (frames, r, c) into (frames, n, n), top-left (prtf / rftf embedding)
+static __global__ void __launch_bounds__(256) k_embed(const c32* in, c32* out, int r, int c,
+                                                      int n, int frames) {
+  const long long total = (long long)frames * n * n;
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < total;
+       i += (long long)gridDim.x * 256) {
+    const int f = (int)(i / ((long long)n * n));
+    const int rem = (int)(i % ((long long)n * n));
+    const int y = rem / n, x = rem % n;
+    out[i] = (y < r && x < c) ? in[((size_t)f * r + y) * c + x] : c32{0.0f, 0.0f};
+  }
+}
+
+// PRTF (phasing.py:1443-1450, phasing_prtf_accumulate2.cl:16-25):
+// | (1/frames) sum_f F_f[k] / |F_f[k]| | with FP64 accumulation, written
+// centre-shifted (fftshift, even n) as the reference returns it; like the
+// reference there is no guard where |F| = 0
+static __global__ void __launch_bounds__(256) k_prtf(const c32* F, int frames, int n, float* out) {
+  const long long nn = (long long)n * n;
+  for (long long k = (long long)blockIdx.x * 256 + threadIdx.x; k < nn;
+       k += (long long)gridDim.x * 256) {
+    double re = 0.0, im = 0.0;
+    for (int f = 0; f < frames; ++f) {
+      const c32 v = F[(size_t)f * nn + k];
+      const double h = sqrt((double)v.x * v.x + (double)v.y * v.y);
+      re += v.x / h;
+      im += v.y / h;
+    }
+    re /= frames;
+    im /= frames;
+    const int y = (int)(k / n), x = (int)(k % n);
+    out[(size_t)((y + n / 2) % n) * n + (x + n / 2) % n] = (float)sqrt(re * re + im * im);
+  }
+}
+
+// RFTF alignment sums (phasing.py:1564-1565): d1 = sum |F - g|,
+// d2 = sum |shift(F) - g| with shift = fftshift (even n); block partials
+static __global__ void __launch_bounds__(256) k_rftf_diff(const c32* F, const float* goal, int n,
+                                                          double* partial) {
+  __shared__ double sh[256];
+  double d1 = 0.0, d2 = 0.0;
+  const long long nn = (long long)n * n;
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < nn;
+       i += (long long)gridDim.x * 256) {
+    const int y = (int)(i / n), x = (int)(i % n);
+    const c32 a = F[i];
+    const c32 b = F[(size_t)((y + n / 2) % n) * n + (x + n / 2) % n];
+    const double g = goal[i];
+    d1 += fabs(sqrt((double)a.x * a.x + (double)a.y * a.y) - g);
+    d2 += fabs(sqrt((double)b.x * b.x + (double)b.y * b.y) - g);
+  }
+  const double s1 = block_sum(d1, sh);
+  if (threadIdx.x == 0) partial[2 * blockIdx.x] = s1;
+  const double s2 = block_sum(d2, sh);
+  if (threadIdx.x == 0) partial[2 * blockIdx.x + 1] = s2;
+}
+
+// RFTF value (phasing.py:1567-1581): err = (F' - g)^2 / g^2 with F' = |F| or
+// shift(|F|) (whichever alignment is closer, decided from the partial sums),
+// output sqrt(1 / (1 + err)) in the reference's centre-shifted frame, with
+// the optional [1/sqrt(2), 1] -> [0, 1] rescale
+static __global__ void __launch_bounds__(256) k_rftf_out(const c32* F, const float* goal, int n,
+                                                         const double* partial, int nblk,
+                                                         int scale, float* out) {
+  double d1 = 0.0, d2 = 0.0;
+  for (int b = 0; b < nblk; ++b) {
+    d1 += partial[2 * b];
+    d2 += partial[2 * b + 1];
+  }
+  const bool centre = d1 > d2;  // goal modulus centred at n/2
+  const long long nn = (long long)n * n;
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < nn;
+       i += (long long)gridDim.x * 256) {
+    const int y = (int)(i / n), x = (int)(i % n);
+    // output pixel i; both branches end centre-shifted (phasing.py:1567-1573)
+    const long long j = (long long)((y + n / 2) % n) * n + (x + n / 2) % n;
+    double fa, g;
+    if (centre) {  // error = (shift(F) - g)^2 / g^2 at i
+      const c32 a = F[j];
+      fa = sqrt((double)a.x * a.x + (double)a.y * a.y);
+      g = goal[i];
+    } else {       // error = shift((F - g)^2 / g^2): value of pixel j
+      const c32 a = F[j];
+      fa = sqrt((double)a.x * a.x + (double)a.y * a.y);
+      g = goal[j];
+    }
+    const double e = (fa - g) * (fa - g) / (g * g);
+    double v = sqrt(1.0 / (1.0 + e));
+    if (scale) {
+      const double q = 1.0 / sqrt(2.0);
+      v = (v - q) / (1.0 - q);
+    }
+    out[i] = (float)v;
+  }
+}
+
+// per-frame RFTF averaged over a stack (gpu_rftf, phasing.py:1611-1708 with
+// phasing_rftf_accumulate.cl:15-24): (1/frames) sum_f 1/sqrt(1 + (g - |F_f|)^2/g^2)
+// with the goal at the corner, FP64 sums, written centre-shifted
+static __global__ void __launch_bounds__(256) k_rftf_stack(const c32* F, const float* goal,
+                                                           int frames, int n, float* out) {
+  const long long nn = (long long)n * n;
+  for (long long k = (long long)blockIdx.x * 256 + threadIdx.x; k < nn;
+       k += (long long)gridDim.x * 256) {
+    const double g = goal[k];
+    double acc = 0.0;
+    for (int f = 0; f < frames; ++f) {
+      const c32 v = F[(size_t)f * nn + k];
+      const double d1 = g - sqrt((double)v.x * v.x + (double)v.y * v.y);
+      acc += 1.0 / sqrt(1.0 + d1 * d1 / (g * g));
+    }
+    const int y = (int)(k / n), x = (int)(k % n);
+    out[(size_t)((y + n / 2) % n) * n + (x + n / 2) % n] = (float)(acc / frames);
   }
 }
 
@@ -998,6 +1174,50 @@ hipError_t launch_stack_mean(const c32* data, long long per_frame, int frames, c
   const long long blocks = std::min<long long>(4096, (per_frame + 255) / 256);
   hipLaunchKernelGGL(k_stack_mean, dim3((unsigned)blocks), dim3(256), 0, st, data, per_frame,
                      frames, out);
+  return hipGetLastError();
+}
+
+static unsigned grid_for(long long n) {
+  return (unsigned)std::min<long long>(8192, std::max<long long>(1, (n + 255) / 256));
+}
+
+hipError_t launch_conv_axis(const float* in, float* out, int rows, int cols, int axis,
+                            const int* off, const float* w, int taps, hipStream_t st) {
+  hipLaunchKernelGGL(k_conv_axis, dim3(grid_for((long long)rows * cols)), dim3(256), 0, st, in,
+                     out, rows, cols, axis, off, w, taps);
+  return hipGetLastError();
+}
+
+hipError_t launch_abs_c64(const c32* in, float* out, long long n, hipStream_t st) {
+  hipLaunchKernelGGL(k_abs_c64, dim3(grid_for(n)), dim3(256), 0, st, in, out, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_embed(const c32* in, c32* out, int r, int c, int n, int frames,
+                        hipStream_t st) {
+  hipLaunchKernelGGL(k_embed, dim3(grid_for((long long)frames * n * n)), dim3(256), 0, st, in,
+                     out, r, c, n, frames);
+  return hipGetLastError();
+}
+
+hipError_t launch_prtf(const c32* F, int frames, int n, float* out, hipStream_t st) {
+  hipLaunchKernelGGL(k_prtf, dim3(grid_for((long long)n * n)), dim3(256), 0, st, F, frames, n,
+                     out);
+  return hipGetLastError();
+}
+
+hipError_t launch_rftf(const c32* F, const float* goal, int n, double* partial, int nblk,
+                       int scale, float* out, hipStream_t st) {
+  hipLaunchKernelGGL(k_rftf_diff, dim3(nblk), dim3(256), 0, st, F, goal, n, partial);
+  hipLaunchKernelGGL(k_rftf_out, dim3(grid_for((long long)n * n)), dim3(256), 0, st, F, goal, n,
+                     partial, nblk, scale, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_rftf_stack(const c32* F, const float* goal, int frames, int n, float* out,
+                             hipStream_t st) {
+  hipLaunchKernelGGL(k_rftf_stack, dim3(grid_for((long long)n * n)), dim3(256), 0, st, F, goal,
+                     frames, n, out);
   return hipGetLastError();
 }
 

@@ -89,6 +89,14 @@ class Context:
         check(self.lib.spk_stack_mean(self.handle, _ptr(data), f, r, c, _ptr(out)))
         return out
 
+    def blur2d(self, img: torch.Tensor, sigma: float) -> torch.Tensor:
+        """refine_support's periodic Gaussian blur of a (rows, cols) float32 image."""
+        r, c = img.shape
+        _expect(img, torch.float32, (r, c), self.device, "image")
+        out = torch.empty_like(img)
+        check(self.lib.spk_blur2d(self.handle, _ptr(img), _ptr(out), r, c, float(sigma)))
+        return out
+
     def copy_bytes(self, dst: torch.Tensor, src: torch.Tensor) -> None:
         nb = src.numel() * src.element_size()
         check(self.lib.spk_copy_bytes(self.handle, _ptr(dst), _ptr(src), nb))
@@ -210,6 +218,31 @@ class Plan:
             rc = self.lib.spk_graph_end(self.handle, byref(h))
         check(rc)
         return Graph(self.lib, h, self)
+
+    def prtf(self, stack: torch.Tensor) -> torch.Tensor:
+        """|mean_f F_f / |F_f|| of a (frames, rows, cols) complex64 stack, padded to n x n."""
+        f, r, c = stack.shape
+        _expect(stack, torch.complex64, (f, r, c), self.ctx.device, "estimates")
+        out = torch.empty((self.n, self.n), dtype=torch.float32, device=self.ctx.device)
+        check(self.lib.spk_prtf(self.handle, _ptr(stack), f, r, c, _ptr(out)))
+        return out
+
+    def rftf(self, estimate: torch.Tensor, goal: torch.Tensor, scale: bool = False):
+        r, c = estimate.shape
+        _expect(estimate, torch.complex64, (r, c), self.ctx.device, "estimate")
+        _expect(goal, torch.float32, (self.n, self.n), self.ctx.device, "goal_modulus")
+        out = torch.empty((self.n, self.n), dtype=torch.float32, device=self.ctx.device)
+        check(self.lib.spk_rftf(self.handle, _ptr(estimate), r, c, _ptr(goal), int(scale),
+                                _ptr(out)))
+        return out
+
+    def rftf_stack(self, stack: torch.Tensor, goal: torch.Tensor) -> torch.Tensor:
+        f, r, c = stack.shape
+        _expect(stack, torch.complex64, (f, r, c), self.ctx.device, "estimates")
+        _expect(goal, torch.float32, (self.n, self.n), self.ctx.device, "goal_modulus")
+        out = torch.empty((self.n, self.n), dtype=torch.float32, device=self.ctx.device)
+        check(self.lib.spk_rftf_stack(self.handle, _ptr(stack), f, r, c, _ptr(goal), _ptr(out)))
+        return out
 
     def profile(self, enable: bool) -> None:
         check(self.lib.spk_plan_profile(self.handle, int(enable)))

@@ -22,9 +22,10 @@ import torch
 
 from . import _capi
 from .device import Context, Plan, require_gpu, to_device
-from .masking import bounding_box
+from .masking import bounding_box, find_all_objects
 
-__all__ = ["phasing", "expand_order", "shrinkwrap_schedule", "DeviceArray"]
+__all__ = ["phasing", "expand_order", "shrinkwrap_schedule", "DeviceArray",
+           "align_global_phase", "refine_support", "prtf", "rftf", "gpu_prtf", "gpu_rftf"]
 
 
 def expand_order(order, iterations):
@@ -425,3 +426,135 @@ def align_global_phase(data_in):
     t3 = t.unsqueeze(0) if t.dim() == 2 else t
     ctx.align_global_phase(t3)
     return t.cpu().numpy().astype(data.dtype)
+
+
+_post_plans = {}
+
+
+def _post_plan(n):
+    """A one-frame plan of size n, kept for the transforms of prtf / rftf."""
+    if n < 16 or n > 4096 or n & (n - 1):
+        raise ValueError("the transform size must be a power of 2 in [16, 4096] (is %d)" % n)
+    p = _post_plans.get(n)
+    if p is None:
+        p = _post_plans[n] = Plan(_context(), n, batch=1, shared=True)
+    return p
+
+
+def refine_support(support, average_mag, blur=3, local_threshold=.2,
+                   global_threshold=0, kill_weakest=False):
+    """Shrinkwrap between rounds, ``phasing.py:1747-1815`` (Marchesini 2003).
+
+    The blur of |average_mag| by the normalised, fftshifted Gaussian of
+    ``shape.gaussian(support.shape, (blur, blur))`` runs on the GPU as the
+    equivalent periodic separable convolution (FP64 sums of FP32 taps); it
+    keeps the reference's odd-size kernel offset.  The connected parts of the
+    support (values in [0.95, 1.05], ``masking.find_all_objects``) and the
+    thresholds are host work, as in the reference:
+    refined = sum over parts of [B*part > local_threshold * max(B*part)],
+    times [B > global_threshold * max B], optionally without the weakest part.
+    Returns (refined, blurred); refined has the support's dtype, blurred is
+    float64 (computed in FP32 storage).  Unlike the reference, the caller's
+    support array is not zeroed by the part search (``masking.py:428``).
+    """
+    assert isinstance(support, np.ndarray), "support must be array"
+    assert support.ndim == 2, "support must be 2d"
+    assert isinstance(average_mag, np.ndarray), "average_mag must be array"
+    assert average_mag.ndim == 2, "average_mag must be 2d"
+    assert isinstance(blur, (float, int)), "blur must be a number (is %s)" % type(blur)
+    assert isinstance(global_threshold, (float, int)), \
+        "global_threshold must be a number (is %s)" % type(global_threshold)
+    assert isinstance(local_threshold, (float, int)), \
+        "local_threshold must be a number (is %s)" % type(local_threshold)
+
+    ctx = _context()
+    mag = to_device(np.abs(average_mag).astype(np.float32), ctx.device, torch.float32)
+    blurred = ctx.blur2d(mag, float(blur)).cpu().numpy().astype(np.float64)
+    refined = np.zeros_like(support)
+    global_passed = np.where(blurred > blurred.max() * global_threshold, 1, 0)
+    parts = find_all_objects(support)
+    part_sums = np.zeros(len(parts), float)
+    for n, part in enumerate(parts):
+        current = blurred * part
+        refined += np.where(current > current.max() * local_threshold, 1, 0).astype(
+            refined.dtype)
+        if kill_weakest:
+            part_sums[n] = np.sum(np.abs(average_mag) * part)
+    refined *= global_passed.astype(refined.dtype)
+    if kill_weakest:
+        refined *= (1 - parts[part_sums.argmin()]).astype(refined.dtype)
+    return refined, blurred
+
+
+def prtf(estimates, N=None, silent=True, prtfq=False):
+    """Phase-retrieval transfer function, ``phasing.py:1406-1461``.
+
+    fftshift(|mean_f F_f / |F_f||) over the aligned estimates (frames, r, c),
+    each zero-padded top-left into N x N (default max(r, c)) and transformed
+    on the GPU; N must be a power of 2 in [16, 4096] (the reference's GPU
+    variant has the same restriction).  The reference's CPU version adds the
+    r x c transforms into an N x N sum, so it only runs when r == c == N; the
+    padding is the same thing there.  The q-average (``prtfq``) needs
+    ``wrapping.unwrap`` and is not provided.
+    """
+    assert isinstance(estimates, (np.ndarray, list, tuple)), "must be iterable"
+    est = np.asarray(estimates)
+    assert est.ndim == 3, "must be 3d"
+    if prtfq:
+        raise NotImplementedError("prtfq needs wrapping.unwrap, which is not provided")
+    f, r, c = est.shape
+    if N is None:
+        N = max(r, c)
+    plan = _post_plan(int(N))
+    t = to_device(est.astype(np.complex64), plan.ctx.device, torch.complex64)
+    return plan.prtf(t).cpu().numpy()
+
+
+def rftf(estimate, goal_modulus, hot_pixels=False, ipsf=None, rftfq=False, scale=False):
+    """Fourier-error transfer function, ``phasing.py:1521-1609``.
+
+    |F| of the estimate zero-padded into the goal's shape is compared with the
+    goal modulus at the corner and the centre alignment (the closer one is
+    used, ``phasing.py:1564-1573``); returns sqrt(1/(1+err)) with err =
+    (|F| - g)^2 / g^2, centre-shifted, optionally rescaled from
+    [1/sqrt(2), 1] to [0, 1].  The goal must be square with a power-of-2 side
+    in [16, 4096].  ``hot_pixels``, ``ipsf`` and ``rftfq`` are not provided.
+    """
+    assert isinstance(estimate, np.ndarray), "estimate must be ndarray"
+    assert isinstance(goal_modulus, np.ndarray), "goal_modulus must be ndarray"
+    assert estimate.shape <= goal_modulus.shape, "estimate must be smaller than goal_modulus"
+    if hot_pixels or ipsf is not None or rftfq:
+        raise NotImplementedError("hot_pixels, ipsf and rftfq are not provided")
+    n = goal_modulus.shape[0]
+    assert goal_modulus.shape == (n, n), "goal_modulus must be square"
+    plan = _post_plan(n)
+    dev = plan.ctx.device
+    e = to_device(estimate.astype(np.complex64), dev, torch.complex64)
+    g = to_device(goal_modulus.astype(np.float32), dev, torch.float32)
+    return plan.rftf(e, g, scale=bool(scale)).cpu().numpy()
+
+
+def gpu_prtf(gpuinfo, estimates, N=None, prtfq=False):
+    """``phasing.py:1463-1519``: the same as ``prtf`` (which already runs on the GPU)."""
+    return prtf(estimates, N=N, prtfq=prtfq)
+
+
+def gpu_rftf(gpuinfo, estimates, goal_modulus, ipsf=None, rftfq=False):
+    """Stack RFTF, ``phasing.py:1611-1708`` with ``phasing_rftf_accumulate.cl:15-24``.
+
+    Unlike ``rftf`` this averages the per-frame value
+    1/sqrt(1 + (g - |F_f|)^2 / g^2) over the estimates (frames, r, c), with
+    the goal taken at the corner (no alignment search), then centre-shifts.
+    ``gpuinfo`` is accepted for the signature and ignored.  ``ipsf`` and
+    ``rftfq`` are not provided.
+    """
+    assert isinstance(estimates, np.ndarray) and estimates.ndim == 3
+    assert isinstance(goal_modulus, np.ndarray) and goal_modulus.ndim == 2
+    assert goal_modulus.shape[0] == goal_modulus.shape[1]
+    if ipsf is not None or rftfq:
+        raise NotImplementedError("ipsf and rftfq are not provided")
+    plan = _post_plan(goal_modulus.shape[0])
+    dev = plan.ctx.device
+    e = to_device(estimates.astype(np.complex64), dev, torch.complex64)
+    g = to_device(goal_modulus.astype(np.float32), dev, torch.float32)
+    return plan.rftf_stack(e, g).cpu().numpy()

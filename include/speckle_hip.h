@@ -179,6 +179,34 @@ int spk_fft2(spk_plan* plan, const float* in_c64, float* out_c64, int count,
 int spk_align_global_phase(spk_ctx* ctx, float* data_c64, int frames,
                            int rows, int cols);
 
+/* refine_support's blur (phasing.py:1787-1790): out = Re ifft2(fft2(K) *
+ * fft2(in)) for K = fftshift(shape.gaussian((rows, cols), (sigma, sigma)))
+ * normalised to sum 1 (shape.py:410-530), computed as the equivalent
+ * periodic separable convolution.  in/out: device float (rows, cols). */
+int spk_blur2d(spk_ctx* ctx, const float* in, float* out, int rows, int cols,
+               float sigma);
+
+/* PRTF (phasing.py:1406-1461 / gpu_prtf 1463-1519): stack (frames, rows,
+ * cols) c64 zero-padded top-left into the plan's n x n, transformed, and
+ * out = fftshift(|mean_f F_f / |F_f||) as the reference returns it (no guard
+ * where |F| = 0, as in the reference).  out: n x n float. */
+int spk_prtf(spk_plan* plan, const float* stack_c64, int frames, int rows,
+             int cols, float* out);
+
+/* RFTF (phasing.py:1521-1609 without hot-pixel removal, ipsf or q-average):
+ * estimate (rows, cols) c64 zero-padded into n x n, |F| compared with the
+ * goal modulus at both alignments, out = sqrt(1/(1+err)) centre-shifted as
+ * the reference returns it; scale != 0 maps [1/sqrt(2), 1] to [0, 1]. */
+int spk_rftf(spk_plan* plan, const float* estimate_c64, int rows, int cols,
+             const float* goal, int scale, float* out);
+
+/* gpu_rftf (phasing.py:1611-1708, phasing_rftf_accumulate.cl:15-24): the
+ * per-frame value 1/sqrt(1 + (g - |F_f|)^2 / g^2) with the goal at the corner,
+ * averaged over the stack (frames, rows, cols) c64 zero-padded into n x n;
+ * out n x n float, centre-shifted as the reference returns it. */
+int spk_rftf_stack(spk_plan* plan, const float* stack_c64, int frames, int rows,
+                   int cols, const float* goal, float* out);
+
 /* trial average: out (rows, cols) c64 = mean over frames of data; the
  * reference averages aligned trials with np.mean / np.sum on the host
  * (phasing.py:960-961, basic_phasing_example.py:43); FP64 sums */

@@ -543,3 +543,20 @@ def rftf(estimate, goal_modulus, scale=False):
         x = 1.0 / np.sqrt(2)
         out = (out - x) / (1 - x)
     return out
+
+
+def rftf_stack(estimates, goal_modulus):
+    """Stack RFTF of ``gpu_rftf``, ``phasing.py:1611-1708`` with
+    ``phasing_rftf_accumulate.cl:15-24``: per pixel, the mean over frames of
+    1/sqrt(1 + (g - |F_f|)^2 / g^2), goal at the corner, then fftshift.
+    """
+    est = np.asarray(estimates)
+    g = np.asarray(goal_modulus, np.float64)
+    n = g.shape[0]
+    acc = np.zeros(g.shape, np.float64)
+    for e in est:
+        new = np.zeros(g.shape, np.complex128)
+        new[:e.shape[0], :e.shape[1]] = e
+        d = g - np.abs(np.fft.fft2(new))
+        acc += 1.0 / np.sqrt(1.0 + d * d / (g * g))
+    return np.fft.fftshift(acc / len(est))

@@ -53,19 +53,30 @@ int fft2_tiled_tw(int inverse, const c32* in, c32* out, const TW& tw) {
     tiled[p] = in[(size_t)r * N + c];
   }
   std::vector<std::array<c32, E>> x(T * TT);
-  // row transforms, one band of TT rows per workgroup
+  std::vector<c32> colband((size_t)N * N);
+  // row transforms, one band of TT rows per workgroup: read the tiled layout,
+  // write tiled (single layout) or column-band-major (dual layout), as
+  // k_rowpass / k_rowfft
   for (int band = 0; band < N / TT; ++band) {
     for (int tid = 0; tid < T * TT; ++tid)
       for (int i = 0; i < E; ++i) x[tid][i] = tiled[row_pos<N, TT, E>(band, tid % TT, tid / TT, i)];
     if (inverse) emu_stages<N, TT, E, true, 0>(x, lds.data(), tw);
     else emu_stages<N, TT, E, false, 0>(x, lds.data(), tw);
     for (int tid = 0; tid < T * TT; ++tid)
-      for (int i = 0; i < E; ++i) tiled[row_pos<N, TT, E>(band, tid % TT, tid / TT, i)] = x[tid][i];
+      for (int i = 0; i < E; ++i) {
+        if constexpr (dual_layout<N>())
+          colband[cbrow_pos<N, TT, E>(band, tid % TT, tid / TT, i)] = x[tid][i];
+        else
+          tiled[row_pos<N, TT, E>(band, tid % TT, tid / TT, i)] = x[tid][i];
+      }
   }
-  // column transforms, one band of TT columns per workgroup
+  // column transforms, one band of TT columns per workgroup: read tiled or
+  // column-band-major, write the tiled layout (as k_colpass)
   for (int band = 0; band < N / TT; ++band) {
     for (int tid = 0; tid < T * TT; ++tid)
-      for (int i = 0; i < E; ++i) x[tid][i] = tiled[col_pos<N, TT, E>(band, tid % TT, tid / TT, i)];
+      for (int i = 0; i < E; ++i)
+        x[tid][i] = dual_layout<N>() ? colband[mcol_pos<N, TT, E>(band, tid % TT, tid / TT, i)]
+                                     : tiled[col_pos<N, TT, E>(band, tid % TT, tid / TT, i)];
     if (inverse) emu_stages<N, TT, E, true, 0>(x, lds.data(), tw);
     else emu_stages<N, TT, E, false, 0>(x, lds.data(), tw);
     for (int tid = 0; tid < T * TT; ++tid)

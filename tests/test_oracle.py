@@ -286,6 +286,34 @@ def test_prtf_rftf_closed_forms():
     assert np.allclose(O.rftf(obj, m + 1e-30), 1.0)
 
 
+def test_rftf_alignment_and_scale():
+    obj, s, m = known_object(32)
+    g = m + 1e-3
+    fa = np.abs(np.fft.fft2(obj))
+    # goal twice the modulus: err = 1/4 everywhere -> sqrt(1/1.25)
+    assert np.allclose(O.rftf(obj, 2 * fa + 1e-30), math.sqrt(1 / 1.25))
+    # a centre-shifted goal is detected and gives the same map
+    assert np.allclose(O.rftf(obj, np.fft.fftshift(g)), O.rftf(obj, g))
+    # scale maps 1/sqrt(2) (err = 1) to 0 and 1 to 1
+    r = O.rftf(obj, fa + 1e-30, scale=True)
+    assert np.allclose(r, 1.0)
+    assert np.allclose(O.rftf(obj, 0.5 * fa + 1e-30, scale=True), 0.0, atol=1e-12)
+    # one frame at the corner alignment: the stack form equals rftf
+    assert np.allclose(O.rftf_stack(obj[None], g), O.rftf(obj, g))
+    # a stack averages the per-frame values
+    rng = np.random.default_rng(2)
+    a, b = obj + 0.1 * crand(obj.shape, rng), obj + 0.1 * crand(obj.shape, rng)
+    want = 0.5 * (O.rftf_stack(a[None], g) + O.rftf_stack(b[None], g))
+    assert np.allclose(O.rftf_stack(np.stack([a, b]), g), want)
+
+
+def test_blur_kernel_odd_size_peak():
+    # centre at n//2 then fftshift: the peak lands at n - 1 for odd n
+    k = O.gaussian_kernel((9, 7), 1.5)
+    assert np.unravel_index(k.argmax(), k.shape) == (8, 6)
+    assert math.isclose(k.sum(), 1.0, rel_tol=1e-14)
+
+
 # ------------------------------------------------ restated reference path
 def test_ref_numpy_one_iteration_matches_oracle():
     obj, s, m = known_object(32)
