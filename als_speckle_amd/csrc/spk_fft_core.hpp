@@ -318,7 +318,21 @@ SPK_HD void stage_butterflies(c32 (&x)[E], int t, int b, c32* lds, const TW& tw)
       for (int r = 0; r < R; ++r) x[q + r * Q] = v[r];
     } else {
       const int base = (j / NS) * NS * R + (j & (NS - 1));
-      for (int r = 0; r < R; ++r) lds[lds_idx<TT>(base + r * NS, b)] = v[r];
+      // slots of positions base + r*NS.  The pad term of lds_idx is linear in
+      // r when the steps are whole 16-slot groups, so the R slots are one
+      // base plus compile-time offsets (immediate LDS offsets, no per-value
+      // index arithmetic).
+      if constexpr ((NS * TT) % 16 == 0) {
+        const int i0 = lds_idx<TT>(base, b);
+        for (int r = 0; r < R; ++r) lds[i0 + r * (NS * TT + NS * TT / 16)] = v[r];
+      } else if constexpr (NS == 1 && (R * TT) % 16 == 0) {
+        // base = j*R starts a 16-slot group; offsets r*TT stay inside the
+        // groups they start in (TT divides 16, b < TT)
+        const int i0 = j * (R * TT + R * TT / 16) + b;
+        for (int r = 0; r < R; ++r) lds[i0 + r * TT + ((r * TT) >> 4)] = v[r];
+      } else {
+        for (int r = 0; r < R; ++r) lds[lds_idx<TT>(base + r * NS, b)] = v[r];
+      }
     }
   }
 }
@@ -326,7 +340,12 @@ SPK_HD void stage_butterflies(c32 (&x)[E], int t, int b, c32* lds, const TW& tw)
 template <int N, int E, int TT>
 SPK_HD void stage_gather(c32 (&x)[E], int t, int b, const c32* lds) {
   constexpr int T = N / E;
-  for (int i = 0; i < E; ++i) x[i] = lds[lds_idx<TT>(t + i * T, b)];
+  if constexpr ((T * TT) % 16 == 0) {  // linear slots, as in stage_butterflies
+    const int i0 = lds_idx<TT>(t, b);
+    for (int i = 0; i < E; ++i) x[i] = lds[i0 + i * (T * TT + T * TT / 16)];
+  } else {
+    for (int i = 0; i < E; ++i) x[i] = lds[lds_idx<TT>(t + i * T, b)];
+  }
 }
 
 }  // namespace spk

@@ -1,7 +1,7 @@
 // gfx950 kernels of the phase-retrieval loop.
 //
-// Internal layout: every n x n frame is stored as TT x TT tiles (TT = 4 or
-// 8), tiles in row-major order, elements row-major inside a tile:
+// Internal layout: every n x n frame is stored as TT x TT tiles (TT = 2, 4
+// or 8), tiles in row-major order, elements row-major inside a tile:
 //   pos(r, c) = ((r/TT)*(n/TT) + c/TT)*TT*TT + (r%TT)*TT + c%TT
 // A band of TT rows is then one contiguous block, and a band of TT columns is
 // n/TT contiguous TT*TT blocks, so both the row pass and the column pass move
@@ -47,18 +47,46 @@ struct LdsPlan {
   static_assert(FULL_TW || N >= 64, "two-level twiddles need N >= 64");
 };
 
-// load the kernel's twiddle table into LDS (all threads; no barrier here:
-// stage 0 uses no twiddles and ends with a barrier)
-template <int N, int TT>
-__device__ __forceinline__ void load_twiddles(c32* twl, const c32* __restrict__ tw) {
+// The kernel's twiddle table, staged through registers: every thread fetches
+// its share before the data loads are issued (fetch), and writes it to LDS
+// once the data loads are in flight (put), so the table's latency overlaps
+// the data's.  No barrier here: stage 0 uses no twiddles and ends with one.
+template <int N, int TT, int NT>
+struct TwStage {
   using L = LdsPlan<N, TT>;
-  const int nt = blockDim.x;
-  if constexpr (L::FULL_TW) {
-    for (int k = threadIdx.x; k < N; k += nt) twl[k] = tw[k];
-  } else {
-    for (int k = threadIdx.x; k < (N >> 6); k += nt) twl[k] = tw[k * 64];
-    for (int k = threadIdx.x; k < 64; k += nt) twl[(N >> 6) + k] = tw[k];
+  static constexpr int CNT = L::TW_ELEMS;
+  static constexpr int PER = (CNT + NT - 1) / NT;
+  c32 v[PER];
+  __device__ __forceinline__ void fetch(const c32* __restrict__ tw) {
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+      const int i = (int)threadIdx.x + k * NT;
+      if (CNT % NT == 0 || i < CNT) {
+        if constexpr (L::FULL_TW) v[k] = tw[i];
+        else v[k] = i < (N >> 6) ? tw[i * 64] : tw[i - (N >> 6)];
+      }
+    }
   }
+  __device__ __forceinline__ void put(c32* twl) const {
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+      const int i = (int)threadIdx.x + k * NT;
+      if (CNT % NT == 0 || i < CNT) twl[i] = v[k];
+    }
+  }
+};
+
+// Global access at a 32-bit element index from a workgroup-uniform base: the
+// byte offset stays 32-bit, so the access uses the scalar-base plus
+// vector-offset form instead of per-value 64-bit address arithmetic.
+// Frames are at most 4096^2 c64 = 128 MB, so offsets within one fit.
+template <class T>
+__device__ __forceinline__ T gld(const T* base, unsigned i) {
+  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + i * (unsigned)sizeof(T));
+}
+template <class T>
+__device__ __forceinline__ void gst(T* base, unsigned i, const T& v) {
+  *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + i * (unsigned)sizeof(T)) = v;
 }
 
 template <int N, int TT>
@@ -149,6 +177,17 @@ __device__ __forceinline__ float block_max(float v, float* sh) {
   return sh[0];
 }
 
+// Column band of a workgroup.  Consecutive workgroup ids are dispatched to
+// the 8 XCDs in turn, each XCD with its own L2; this gives every XCD a
+// contiguous run of bands instead, so that tiles narrower than a 128-byte
+// line (tile 2: 32 B) share their lines within one L2.  Needs NB % 8 == 0
+// (then the frame index in grid.y does not change the XCD of a band).
+template <int NB>
+__device__ __forceinline__ int xcd_band(int bid) {
+  if constexpr (NB % 8 == 0) return (bid % 8) * (NB / 8) + bid / 8;
+  else return bid;
+}
+
 // ---------------------------------------------------------------------------
 // Column pass.  MODE 0: FFT -> projection (+ error) -> IFFT
 //               MODE 2: forward FFT only; MODE 3: inverse FFT only
@@ -159,7 +198,7 @@ __global__ void __launch_bounds__((N / E) * TT) k_colpass(PassArgs a) {
   constexpr int NB = N / TT;
   using L = LdsPlan<N, TT>;
   __shared__ c32 lds[L::TOTAL];
-  const int band = blockIdx.x, z = blockIdx.z;
+  const int band = xcd_band<NB>(blockIdx.x), z = blockIdx.z;
   const int f = a.fmap ? a.fmap[blockIdx.y] : blockIdx.y;
   const Coef c = a.tab ? a.tab[f] : a.c;
   if (!(c.flags & kActive)) return;
@@ -171,23 +210,26 @@ __global__ void __launch_bounds__((N / E) * TT) k_colpass(PassArgs a) {
   const c32* src = (z ? (DUAL ? a.phibc : a.phib) : (DUAL ? a.phic : a.phi)) + fo;
   c32* dst = (z ? a.phib : a.phi) + fo;
   const int tid = threadIdx.x, b = tid % TT, t = tid / TT;
+  constexpr int NT = (N / E) * TT;
 
-  // issue every global load up front: data, modulus, twiddle table
+  // every global load up front: twiddle table, data, modulus
+  TwStage<N, TT, NT> tws;
+  tws.fetch(a.tw);
   c32 x[E];
 #pragma unroll
   for (int i = 0; i < E; ++i)
-    x[i] = src[DUAL ? mcol_pos<N, TT, E>(band, b, t, i) : col_pos<N, TT, E>(band, b, t, i)];
-  // the modulus is loaded with the data where registers allow (16 values per
-  // thread), otherwise after the forward transform
-  constexpr bool PREFETCH = E <= 16 && (N / E) * TT <= 512;
+    x[i] = gld(src, DUAL ? mcol_pos<N, TT, E>(band, b, t, i) : col_pos<N, TT, E>(band, b, t, i));
+  // the modulus is loaded with the data where registers allow, otherwise
+  // after the forward transform
+  constexpr bool PREFETCH = E <= 16 && NT <= 512;
   const float* __restrict__ mod = a.mod + (size_t)f * a.mstride;
   float m[MODE == 0 ? E : 1];
   if constexpr (MODE == 0 && PREFETCH) {
 #pragma unroll
-    for (int i = 0; i < E; ++i) m[i] = mod[mcol_pos<N, TT, E>(band, b, t, i)];
+    for (int i = 0; i < E; ++i) m[i] = gld(mod, mcol_pos<N, TT, E>(band, b, t, i));
   }
   c32* twl = lds + L::NBUF * L::BUF;
-  load_twiddles<N, TT>(twl, a.tw);
+  tws.put(twl);
   const auto tw = twiddle_view<N, TT>(twl);
 
   double err = 0.0;
@@ -199,23 +241,26 @@ __global__ void __launch_bounds__((N / E) * TT) k_colpass(PassArgs a) {
   if constexpr (MODE == 0) {
     if constexpr (!PREFETCH) {
 #pragma unroll
-      for (int i = 0; i < E; ++i) m[i] = mod[mcol_pos<N, TT, E>(band, b, t, i)];
+      for (int i = 0; i < E; ++i) m[i] = gld(mod, mcol_pos<N, TT, E>(band, b, t, i));
     }
 #pragma unroll
     for (int i = 0; i < E; ++i) {
-      const float am = sqrtf(x[i].x * x[i].x + x[i].y * x[i].y);
       // modulus replacement m * Psi / |Psi| (phasing_fourier.cl:12-15,
-      // phasing.py:1217); 0 where |Psi| = 0 (the reference gives NaN there)
-      const float r = am > 0.0f ? m[i] / am : 0.0f;
+      // phasing.py:1217) through one reciprocal square root of |Psi|^2
+      // (v_rsq_f32, about 1 ulp); 0 where |Psi|^2 is below the smallest
+      // normal float, i.e. |Psi| < 1.1e-19 (the reference gives NaN at 0)
+      const float p2 = x[i].x * x[i].x + x[i].y * x[i].y;
+      const float rs = p2 >= 1.17549435e-38f ? __builtin_amdgcn_rsqf(p2) : 0.0f;
+      const float am = p2 * rs;  // |Psi|
       // error term: exact difference and square in FP64 of the FP32 |Psi|
       const double d = (double)am - (double)m[i];
       err += d * d;
-      x[i] = scale(x[i], r);
+      x[i] = scale(x[i], m[i] * rs);
     }
     fft_regs<N, E, TT, true, false>(x, t, b, lds, tw);
   }
 #pragma unroll
-  for (int i = 0; i < E; ++i) dst[col_pos<N, TT, E>(band, b, t, i)] = x[i];
+  for (int i = 0; i < E; ++i) gst(dst, col_pos<N, TT, E>(band, b, t, i), x[i]);
   if constexpr (MODE == 0) {
     if (z == 0 && a.errp) {
       const double s = block_sum(err, reinterpret_cast<double*>(lds));
@@ -239,32 +284,37 @@ __global__ void __launch_bounds__((N / E) * TT) k_rowpass(PassArgs a) {
   if (!(c.flags & kActive)) return;
   const size_t fo = (size_t)f * N * N;
   const int tid = threadIdx.x, b = tid % TT, t = tid / TT;
+  constexpr int NT = (N / E) * TT;
 #define SPK_RPOS(i) row_pos<N, TT, E>(band, b, t, (i))
+  c32* phi = a.phi + fo;
+  c32* psi = a.psi + fo;
 
-  // every global load up front: phi (and phib), psi_in, support, twiddles
+  // every global load up front: twiddles, phi (and phib), psi_in, support
+  TwStage<N, TT, NT> tws;
+  tws.fetch(a.tw);
   c32 x[E];
 #pragma unroll
-  for (int i = 0; i < E; ++i) x[i] = a.phi[fo + SPK_RPOS(i)];
+  for (int i = 0; i < E; ++i) x[i] = gld(phi, SPK_RPOS(i));
   c32 tv[USE_T ? E : 1];
   if constexpr (USE_T) {
 #pragma unroll
-    for (int i = 0; i < E; ++i) tv[i] = a.phib[fo + SPK_RPOS(i)];
+    for (int i = 0; i < E; ++i) tv[i] = gld(a.phib + fo, SPK_RPOS(i));
   }
-  // psi_in and the support are loaded with phi where registers allow (16
-  // values per thread, no DM), otherwise after the transforms
-  constexpr bool PREFETCH = E <= 16 && (N / E) * TT <= 512 && !USE_T;
+  // psi_in and the support are loaded with phi where registers allow (no
+  // DM), otherwise after the transforms
+  constexpr bool PREFETCH = E <= 16 && NT <= 512 && !USE_T;
   const float* __restrict__ sup = a.sup + (size_t)f * a.sstride;
   c32 pin[E];
   float sv[E];
   if constexpr (PREFETCH) {
 #pragma unroll
     for (int i = 0; i < E; ++i) {
-      pin[i] = a.psi[fo + SPK_RPOS(i)];
-      sv[i] = sup[SPK_RPOS(i)];
+      pin[i] = gld(psi, SPK_RPOS(i));
+      sv[i] = gld(sup, SPK_RPOS(i));
     }
   }
   c32* twl = lds + L::NBUF * L::BUF;
-  load_twiddles<N, TT>(twl, a.tw);
+  tws.put(twl);
   const auto tw = twiddle_view<N, TT>(twl);
 
   fft_regs<N, E, TT, true, true>(x, t, b, lds, tw);
@@ -272,8 +322,8 @@ __global__ void __launch_bounds__((N / E) * TT) k_rowpass(PassArgs a) {
   if constexpr (!PREFETCH) {
 #pragma unroll
     for (int i = 0; i < E; ++i) {
-      pin[i] = a.psi[fo + SPK_RPOS(i)];
-      sv[i] = sup[SPK_RPOS(i)];
+      pin[i] = gld(psi, SPK_RPOS(i));
+      sv[i] = gld(sup, SPK_RPOS(i));
     }
   }
 
@@ -291,7 +341,7 @@ __global__ void __launch_bounds__((N / E) * TT) k_rowpass(PassArgs a) {
       out_s = add(out_s, scale(tt, c.bt));
     }
     const c32 nv = add(scale(in_s, s), scale(out_s, u));
-    a.psi[fo + SPK_RPOS(i)] = nv;
+    gst(psi, SPK_RPOS(i), nv);
     x[i] = nv;
     if constexpr (EMIT_B) y[i] = scale(nv, s - c.g_next * u);
   }
@@ -302,15 +352,15 @@ __global__ void __launch_bounds__((N / E) * TT) k_rowpass(PassArgs a) {
     fft_regs<N, E, TT, false, false>(y, t, b, lds, tw);
 #pragma unroll
     for (int i = 0; i < E; ++i) {
-      if constexpr (DUAL) a.phibc[fo + cbrow_pos<N, TT, E>(band, b, t, i)] = y[i];
-      else a.phib[fo + SPK_RPOS(i)] = y[i];
+      if constexpr (DUAL) gst(a.phibc + fo, cbrow_pos<N, TT, E>(band, b, t, i), y[i]);
+      else gst(a.phib + fo, SPK_RPOS(i), y[i]);
     }
   }
   fft_regs<N, E, TT, false, false>(x, t, b, lds, tw);
 #pragma unroll
   for (int i = 0; i < E; ++i) {
-    if constexpr (DUAL) a.phic[fo + cbrow_pos<N, TT, E>(band, b, t, i)] = x[i];
-    else a.phi[fo + SPK_RPOS(i)] = x[i];
+    if constexpr (DUAL) gst(a.phic + fo, cbrow_pos<N, TT, E>(band, b, t, i), x[i]);
+    else gst(phi, SPK_RPOS(i), x[i]);
   }
 #undef SPK_RPOS
 }
@@ -330,23 +380,29 @@ __global__ void __launch_bounds__((N / E) * TT)
   const int band = blockIdx.x, f = fmap ? fmap[blockIdx.y] : blockIdx.y;
   const size_t fo = (size_t)f * N * N;
   const int tid = threadIdx.x, b = tid % TT, t = tid / TT;
+  constexpr int NT = (N / E) * TT;
+  TwStage<N, TT, NT> tws;
+  tws.fetch(tw);
+  const c32* src = in + fo;
+  const float* sp = sup + (size_t)f * sstride;
   c32 x[E];
 #pragma unroll
   for (int i = 0; i < E; ++i) {
     const int p = row_pos<N, TT, E>(band, b, t, i);
-    x[i] = in[fo + p];
+    x[i] = gld(src, p);
     if constexpr (MODE == 2) {
-      const float s = sup[(size_t)f * sstride + p];
+      const float s = gld(sp, p);
       x[i] = scale(x[i], s - g * (1.0f - s));
     }
   }
   c32* twl = lds + L::NBUF * L::BUF;
-  load_twiddles<N, TT>(twl, tw);
+  tws.put(twl);
   fft_regs<N, E, TT, MODE == 1, true>(x, t, b, lds, twiddle_view<N, TT>(twl));
+  c32* dst = out + fo;
 #pragma unroll
   for (int i = 0; i < E; ++i) {
-    if constexpr (dual_layout<N>()) out[fo + cbrow_pos<N, TT, E>(band, b, t, i)] = x[i];
-    else out[fo + row_pos<N, TT, E>(band, b, t, i)] = x[i];
+    if constexpr (dual_layout<N>()) gst(dst, cbrow_pos<N, TT, E>(band, b, t, i), x[i]);
+    else gst(dst, row_pos<N, TT, E>(band, b, t, i), x[i]);
   }
 }
 
@@ -533,7 +589,7 @@ __global__ void __launch_bounds__(256) k_blur_cols(const float* tmp, float* blur
   __shared__ float sh[TT * LS];
   __shared__ float wsh[RMAX + 1];
   __shared__ float red[256];
-  const int band = blockIdx.x, f = fmap ? fmap[blockIdx.y] : blockIdx.y;
+  const int band = xcd_band<NB>(blockIdx.x), f = fmap ? fmap[blockIdx.y] : blockIdx.y;
   const size_t fo = (size_t)f * N * N;
   for (int k = threadIdx.x; k < N * TT; k += 256) {
     const int tr = k / (TT * TT), within = k % (TT * TT);
@@ -917,7 +973,7 @@ static hipError_t shrinkwrap_nt(const c32* psi, float* sup, long long sstride, f
   return hipGetLastError();
 }
 
-// Shapes built per frame size: tile 4 and 8 (not 8 at 4096, whose LDS
+// Shapes built per frame size: tile 2, 4 and 8 (not 8 at 4096, whose LDS
 // exchange would not fit), 16 values per thread, and 8 values per thread
 // where the workgroup stays within 1024 threads.
 template <int N, int TT, int E>
@@ -928,6 +984,8 @@ constexpr bool shape_ok() {
 #define SPK_SHAPE_SWITCH(n, CALLT)                                       \
   if (tile == 4 && ept == 16) return CALLT(4, 16);                       \
   if (tile == 4 && ept == 8) return CALLT(4, 8);                         \
+  if (tile == 2 && ept == 16) return CALLT(2, 16);                       \
+  if (tile == 2 && ept == 8) return CALLT(2, 8);                         \
   if (tile == 8 && ept == 16) return CALLT(8, 16);                       \
   if (tile == 8 && ept == 8) return CALLT(8, 8);                         \
   return hipErrorInvalidValue;
@@ -985,6 +1043,8 @@ constexpr bool shape_ok() {
                             hipStream_t st) {                                               \
     if (tile == 4) return shrinkwrap_nt<n, 4>(psi, sup, sstride, tmp, blur, maxp, tab, p0, \
                                               fmap, frames, rmax, st);                      \
+    if (tile == 2) return shrinkwrap_nt<n, 2>(psi, sup, sstride, tmp, blur, maxp, tab, p0, \
+                                              fmap, frames, rmax, st);                      \
     if (tile == 8 && n != 4096)                                                             \
       return shrinkwrap_nt<n, (n == 4096 ? 4 : 8)>(psi, sup, sstride, tmp, blur, maxp, tab, \
                                                    p0, fmap, frames, rmax, st);             \
@@ -1026,7 +1086,7 @@ SPK_SIZED_DEF(4096)
 thread_local hipEvent_t g_launch_events[2] = {nullptr, nullptr};
 
 bool size_supported(int n, int tile, int ept) {
-  if (tile != 4 && tile != 8) return false;
+  if (tile != 2 && tile != 4 && tile != 8) return false;
   if (ept != 8 && ept != 16) return false;
   if (n < 16 || n > 4096 || (n & (n - 1))) return false;
   if (n == 4096 && tile == 8) return false;
@@ -1085,6 +1145,8 @@ hipError_t launch_pack_c64(int n, int tile, c32* dst, const c32* src, const floa
   dim3 grid((unsigned)((size_t)n * n / 256), frames);
   if (tile == 8)
     hipLaunchKernelGGL(k_pack_c64<8>, grid, dim3(256), 0, st, dst, src, n, sup, sstride);
+  else if (tile == 2)
+    hipLaunchKernelGGL(k_pack_c64<2>, grid, dim3(256), 0, st, dst, src, n, sup, sstride);
   else
     hipLaunchKernelGGL(k_pack_c64<4>, grid, dim3(256), 0, st, dst, src, n, sup, sstride);
   return hipGetLastError();
@@ -1095,6 +1157,8 @@ hipError_t launch_unpack_c64(int n, int tile, c32* dst, const c32* src, float sc
   dim3 grid((unsigned)((size_t)n * n / 256), frames);
   if (tile == 8)
     hipLaunchKernelGGL(k_unpack_c64<8>, grid, dim3(256), 0, st, dst, src, n, scale);
+  else if (tile == 2)
+    hipLaunchKernelGGL(k_unpack_c64<2>, grid, dim3(256), 0, st, dst, src, n, scale);
   else
     hipLaunchKernelGGL(k_unpack_c64<4>, grid, dim3(256), 0, st, dst, src, n, scale);
   return hipGetLastError();
@@ -1105,6 +1169,8 @@ hipError_t launch_pack_f32(int n, int tile, float* dst, const float* src, int fr
   dim3 grid((unsigned)((size_t)n * n / 256), frames);
   if (tile == 8)
     hipLaunchKernelGGL(k_pack_f32<8>, grid, dim3(256), 0, st, dst, src, n);
+  else if (tile == 2)
+    hipLaunchKernelGGL(k_pack_f32<2>, grid, dim3(256), 0, st, dst, src, n);
   else
     hipLaunchKernelGGL(k_pack_f32<4>, grid, dim3(256), 0, st, dst, src, n);
   return hipGetLastError();
@@ -1115,6 +1181,8 @@ hipError_t launch_pack_mod(int n, int tile, float* dst, const float* src, int fr
   dim3 grid((unsigned)((size_t)n * n / 256), frames);
   if (tile == 8)
     hipLaunchKernelGGL(k_pack_mod<8>, grid, dim3(256), 0, st, dst, src, n);
+  else if (tile == 2)
+    hipLaunchKernelGGL(k_pack_mod<2>, grid, dim3(256), 0, st, dst, src, n);
   else
     hipLaunchKernelGGL(k_pack_mod<4>, grid, dim3(256), 0, st, dst, src, n);
   return hipGetLastError();
@@ -1125,6 +1193,8 @@ hipError_t launch_unpack_f32(int n, int tile, float* dst, const float* src, int 
   dim3 grid((unsigned)((size_t)n * n / 256), frames);
   if (tile == 8)
     hipLaunchKernelGGL(k_unpack_f32<8>, grid, dim3(256), 0, st, dst, src, n);
+  else if (tile == 2)
+    hipLaunchKernelGGL(k_unpack_f32<2>, grid, dim3(256), 0, st, dst, src, n);
   else
     hipLaunchKernelGGL(k_unpack_f32<4>, grid, dim3(256), 0, st, dst, src, n);
   return hipGetLastError();
@@ -1155,6 +1225,8 @@ hipError_t launch_copy_roi(int n, int tile, const c32* psi, c32* dst, int r0, in
   dim3 grid((total + 255) / 256);
   if (tile == 8)
     hipLaunchKernelGGL(k_copy_roi<8>, grid, dim3(256), 0, st, psi, dst, n, r0, c0, rows, cols);
+  else if (tile == 2)
+    hipLaunchKernelGGL(k_copy_roi<2>, grid, dim3(256), 0, st, psi, dst, n, r0, c0, rows, cols);
   else
     hipLaunchKernelGGL(k_copy_roi<4>, grid, dim3(256), 0, st, psi, dst, n, r0, c0, rows, cols);
   return hipGetLastError();

@@ -62,7 +62,7 @@ typedef struct spk_step {
 
 /* Kernel shape choices for a plan (autotuner output).  Zero = library default. */
 typedef struct spk_tune {
-  int32_t tile;       /* tile edge of the internal layout: 4 or 8            */
+  int32_t tile;       /* tile edge of the internal layout: 2, 4 or 8         */
   int32_t ept;        /* values per thread of one transform: 8 or 16         */
   int32_t reserved[6];
 } spk_tune;

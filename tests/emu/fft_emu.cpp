@@ -117,6 +117,8 @@ int emu_fft2(int n, int tile, int inverse, const float* in, float* out, int two_
     if (ept == 16 && tile == 4) return fft2_tiled<NN, 4, 16>(inverse, i, o, two_level); \
     if (ept == 8 && tile == 8) return fft2_tiled<NN, 8, 8>(inverse, i, o, two_level);   \
     if (ept == 8 && tile == 4) return fft2_tiled<NN, 4, 8>(inverse, i, o, two_level);   \
+    if (ept == 16 && tile == 2) return fft2_tiled<NN, 2, 16>(inverse, i, o, two_level); \
+    if (ept == 8 && tile == 2) return fft2_tiled<NN, 2, 8>(inverse, i, o, two_level);   \
     return -1;                                                  \
   }
   EMU_CASE(16) EMU_CASE(32) EMU_CASE(64) EMU_CASE(128) EMU_CASE(256) EMU_CASE(512)

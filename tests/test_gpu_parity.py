@@ -60,7 +60,7 @@ def step(alg, beta=0.9, sem="numpy", sigma=0.0, tau=0.0):
 
 # ------------------------------------------------------------------ FFT
 @pytest.mark.parametrize("n", [16, 32, 64, 128, 256, 512, 1024, 2048, 4096])
-@pytest.mark.parametrize("tile", [4, 8])
+@pytest.mark.parametrize("tile", [2, 4, 8])
 @pytest.mark.parametrize("ept", [16, 8])
 def test_fft2_matches_numpy(gpu_ctx, n, tile, ept):
     if n == 4096 and tile == 8:
@@ -84,7 +84,7 @@ def test_fft2_matches_numpy(gpu_ctx, n, tile, ept):
 @pytest.mark.parametrize("alg,sem", [("hio", "numpy"), ("er", "numpy"), ("er", "opencl"),
                                      ("raar", "numpy"), ("sf", "numpy"), ("sf", "opencl"),
                                      ("dm", "numpy")])
-@pytest.mark.parametrize("tile,ept", [(4, 16), (8, 16), (4, 8)])
+@pytest.mark.parametrize("tile,ept", [(4, 16), (8, 16), (4, 8), (2, 8), (2, 16)])
 def test_one_step_matches_oracle(gpu_ctx, n, alg, sem, tile, ept):
     print(f"[one-step n={n} {alg}/{sem} tile={tile} ept={ept}]", flush=True)
     f = frame(n)
@@ -163,7 +163,7 @@ def test_dm_doubles_solution(gpu_ctx):
     assert rel(p.get_psi().cpu().numpy(), 2 * f.obj) < 2e-6
 
 
-@pytest.mark.parametrize("tile,ept", [(4, 16), (8, 8)])
+@pytest.mark.parametrize("tile,ept", [(4, 16), (8, 8), (2, 8)])
 def test_dm_multi_step_lockstep(gpu_ctx, tile, ept):
     """DM over several iterations uses the row pass's phib emission; the
     result equals step-by-step calls (phib rebuilt each call) and the oracle
