@@ -26,7 +26,8 @@ class SpkStep(ctypes.Structure):
 
 
 class SpkTune(ctypes.Structure):
-    _fields_ = [("tile", c_int32), ("ept", c_int32), ("reserved", c_int32 * 6)]
+    _fields_ = [("tile", c_int32), ("ept", c_int32), ("persist", c_int32),
+                ("reserved", c_int32 * 5)]
 
 
 class SpkError(RuntimeError):
@@ -48,6 +49,7 @@ SIGNATURES = {
     "spk_supported_sizes": (c_uint32, []),
     "spk_plan_create": (c_int, [c_void_p, c_int, c_int, c_int, POINTER(SpkTune), POINTER(c_void_p)]),
     "spk_plan_destroy": (c_int, [c_void_p]),
+    "spk_plan_check": (c_int, [c_void_p]),
     "spk_plan_bytes": (c_int, [c_int, c_int, c_int, c_int, POINTER(c_size_t)]),
     "spk_set_modulus": (c_int, [c_void_p, c_int, c_void_p]),
     "spk_set_support": (c_int, [c_void_p, c_int, c_void_p]),

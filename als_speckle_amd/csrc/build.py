@@ -29,6 +29,9 @@ COMMON = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",
     # no packed-FP32 instructions in device code
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # fused multiply-adds only where the source writes fmaf: results do not
+    # depend on the inlining context of the shared arithmetic
+    "-ffp-contract=off",
     "-I", os.path.join(ROOT, "include"), "-I", HERE,
     "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
 ]

@@ -37,7 +37,9 @@ int fail(int code, const std::string& msg) {
 
 bool is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
 
-int default_tile(int n) { (void)n; return 4; }
+// 4096: tile 2 with 8 values per thread keeps the 1024-thread passes free of
+// register spills (tile 4 needs 16 values per thread there, which spill)
+int default_tile(int n) { return n >= 4096 ? 2 : 4; }
 // 8 values per thread (more threads, shorter chains) wherever it fits
 int default_ept(int n, int tile) { return (n / 8) * tile <= 1024 ? 8 : 16; }
 
@@ -114,6 +116,11 @@ struct spk_plan {
   std::vector<hipEvent_t> prof_ev[2];  // start/stop pairs per pass
   bool mod_set = false, sup_set = false, psi_set = false;
   bool phib_fresh = false;    // phib == rowFFT(dm1(psi)) for the next DM step
+  // persistent iteration blocks (k_persist): on/off, co-resident grid size
+  // (0 = not queried yet), barrier counter [0] and give-up flag [1]
+  int persist = 0;
+  int pgrid = 0;
+  unsigned* pbar = nullptr;
   float phib_g = 0.0f;
   long long nn() const { return (long long)n * n; }
   // Phi kept in two layouts (see spk_fft_core.hpp dual_layout): n >= 4096
@@ -401,13 +408,14 @@ int spk_plan_create(spk_ctx* ctx, int n, int batch, int shared, const spk_tune* 
   p->tile = tile;
   p->ept = ept;
   p->bands = n / tile;
+  p->persist = (tune && tune->persist > 0) ? 1 : 0;
   const size_t nn = (size_t)n * n;
   const size_t fms = shared ? 1 : (size_t)batch;
   if ((rc = dalloc(&p->psi, nn * batch)) || (rc = dalloc(&p->phi, nn * batch)) ||
       (p->dual() && (rc = dalloc(&p->phic, nn * batch))) ||
       (rc = dalloc(&p->mod, nn * fms)) || (rc = dalloc(&p->sup, nn * fms)) ||
       (rc = dalloc(&p->tw, (size_t)n)) || (rc = dalloc(&p->summ2, (size_t)batch)) ||
-      (rc = dalloc(&p->sumpart, (size_t)64 * batch))) {
+      (rc = dalloc(&p->sumpart, (size_t)64 * batch)) || (rc = dalloc(&p->pbar, (size_t)2))) {
     spk_plan_destroy(p);
     return rc;
   }
@@ -418,12 +426,29 @@ int spk_plan_create(spk_ctx* ctx, int n, int batch, int shared, const spk_tune* 
     tw[k] = c32{(float)std::cos(ang), (float)std::sin(ang)};
   }
   hipError_t e = hipMemcpy(p->tw, tw.data(), n * sizeof(c32), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemset(p->pbar, 0, 2 * sizeof(unsigned));
   if (e == hipSuccess) e = hipEventCreateWithFlags(&p->upload_done, hipEventDisableTiming);
   if (e != hipSuccess) {
     spk_plan_destroy(p);
     return fail(SPK_EHIP, std::string("plan setup: ") + hipGetErrorString(e));
   }
   *out = p;
+  return SPK_OK;
+}
+
+int spk_plan_check(spk_plan* p) {
+  SPK_REQUIRE(p, SPK_EINVAL, "null plan");
+  int rc = set_device(p->ctx);
+  if (rc) return rc;
+  if (p->ctx->stream) SPK_HIP(hipStreamSynchronize(p->ctx->stream));
+  else SPK_HIP(hipDeviceSynchronize());
+  unsigned h[2] = {0, 0};
+  SPK_HIP(hipMemcpy(h, p->pbar, sizeof(h), hipMemcpyDeviceToHost));
+  if (h[1]) {
+    SPK_HIP(hipMemset(p->pbar + 1, 0, sizeof(unsigned)));
+    return fail(SPK_EHIP, "a grid barrier of a persistent iteration block gave up waiting; "
+                          "results of that block are invalid");
+  }
   return SPK_OK;
 }
 
@@ -436,7 +461,7 @@ int spk_plan_destroy(spk_plan* p) {
   }
   dfree(p->psi); dfree(p->phi); dfree(p->phic); dfree(p->phib); dfree(p->phibc);
   dfree(p->mod); dfree(p->sup);
-  dfree(p->tw); dfree(p->summ2); dfree(p->sumpart); dfree(p->errp);
+  dfree(p->tw); dfree(p->summ2); dfree(p->sumpart); dfree(p->errp); dfree(p->pbar);
   dfree(p->sw_tmp); dfree(p->sw_blur); dfree(p->sw_maxp); dfree(p->fftbuf);
   dfree(p->stage_d);
   prof_clear(p, 0);
@@ -637,7 +662,6 @@ int spk_iterate(spk_plan* p, const spk_step* steps, int nsteps, int per_frame,
         swp_off[k] = stg.add(prm.data(), prm.size());
       }
     }
-    if ((rc = upload_stage(p, stg))) return rc;
   } else {
     for (int k = 0; k < nsteps; ++k) {
       if (any_dm[k]) dm_g[k] = 1.0f / steps[k].beta;
@@ -648,6 +672,40 @@ int spk_iterate(spk_plan* p, const spk_step* steps, int nsteps, int per_frame,
       }
     }
   }
+  // Persistent blocks: runs of at least two steps with no DM step and no DM
+  // successor, each run ending at a step with a shrinkwrap (which follows the
+  // block).  block_end[k0] = k1 for a block of steps [k0, k1).
+  std::vector<char> emits(nsteps, 0);
+  for (int k = 0; k < nsteps; ++k)
+    for (int f = 0; f < W; ++f) emits[k] |= (coefs[(size_t)k * W + f].flags & kEmitB) != 0;
+  std::vector<int> block_end(nsteps, 0);
+  std::vector<size_t> block_off(nsteps, 0);
+  if (p->persist) {
+    auto fits = [&](int j) { return !any_dm[j] && !emits[j]; };
+    for (int k = 0; k < nsteps;) {
+      if (!fits(k)) { ++k; continue; }
+      int k1 = k;
+      while (k1 < nsteps && fits(k1)) {
+        ++k1;
+        if (any_sw[k1 - 1]) break;
+      }
+      if (k1 - k >= 2) {
+        std::vector<PStep> ps(k1 - k);
+        for (int j = k; j < k1; ++j) {
+          PStep& q = ps[j - k];
+          q.c = coefs[(size_t)j * W];
+          q.tab_off = per_frame ? (long long)(coef_off + (size_t)j * W * sizeof(Coef)) : -1;
+          q.fmap_off = per_frame ? (long long)map_off[j] : -1;
+          q.nact = map_n[j];
+          q.pad_ = 0;
+        }
+        block_end[k] = k1;
+        block_off[k] = stg.add(ps.data(), ps.size());
+      }
+      k = k1;
+    }
+  }
+  if ((rc = upload_stage(p, stg))) return rc;
   unsigned char* dbase = p->stage_d;
   auto dptr = [&](size_t off) { return dbase + off; };
 
@@ -664,10 +722,38 @@ int spk_iterate(spk_plan* p, const spk_step* steps, int nsteps, int per_frame,
   }
 
   PassArgs a = base_args(p);
-  for (int k = 0; k < nsteps; ++k) {
+  auto shrinkwrap_after = [&](int k) -> int {
+    if (per_frame) {
+      SPK_HIP(launch_shrinkwrap(p->n, p->tile, p->psi, p->sup, p->mstride(), p->sw_tmp,
+                                p->sw_blur, p->sw_maxp, (const SwParam*)dptr(swp_off[k]),
+                                SwParam{}, (const int*)dptr(sw_off[k]), 0, sw_n[k], sw_r[k],
+                                st));
+    } else {
+      SPK_HIP(launch_shrinkwrap(p->n, p->tile, p->psi, p->sup, p->mstride(), p->sw_tmp,
+                                p->sw_blur, p->sw_maxp, nullptr, sw_first[k], nullptr, 0, B,
+                                sw_r[k], st));
+    }
+    p->phib_fresh = false;
+    return SPK_OK;
+  };
+  for (int k = 0; k < nsteps;) {
+    if (block_end[k]) {
+      const int k1 = block_end[k];
+      if (!p->pgrid) SPK_HIP(persist_grid(p->n, p->tile, p->ept, &p->pgrid));
+      SPK_REQUIRE(p->pgrid > 0, SPK_EUNSUP, "persistent kernel does not fit on this device");
+      const int grid = (int)std::min<long long>(p->pgrid, (long long)B * p->bands);
+      PassArgs pa = base_args(p);
+      SPK_HIP(launch_persist(p->n, p->tile, p->ept, pa, dbase, (const PStep*)dptr(block_off[k]),
+                             k1 - k, err_trace ? p->errp + (size_t)k * B * p->bands : nullptr,
+                             (long long)B * p->bands, p->pbar, (int*)(p->pbar + 1), grid, st));
+      p->phib_fresh = false;
+      if (any_sw[k1 - 1] && (rc = shrinkwrap_after(k1 - 1))) return rc;
+      k = k1;
+      continue;
+    }
     const size_t row = (size_t)k * W;
     if (per_frame) {
-      if (map_n[k] == 0) continue;   // every frame idle this step
+      if (map_n[k] == 0) { ++k; continue; }   // every frame idle this step
       a.tab = (const Coef*)dptr(coef_off) + row;
       a.fmap = (const int*)dptr(map_off[k]);
     } else {
@@ -685,25 +771,12 @@ int spk_iterate(spk_plan* p, const spk_step* steps, int nsteps, int per_frame,
     a.errp = err_trace ? p->errp + (size_t)k * B * p->bands : nullptr;
     if ((rc = prof_arm(p, 0))) return rc;
     SPK_HIP(launch_colpass(p->n, p->tile, p->ept, 0, a, nact, any_dm[k] ? 2 : 1, st));
-    bool emit = false;
-    for (int f = 0; f < W; ++f) emit |= (coefs[row + f].flags & kEmitB) != 0;
     a.errp = nullptr;
     if ((rc = prof_arm(p, 1))) return rc;
-    SPK_HIP(launch_rowpass(p->n, p->tile, p->ept, any_dm[k] != 0, emit, a, nact, st));
-    p->phib_fresh = emit;
-    if (any_sw[k]) {
-      if (per_frame) {
-        SPK_HIP(launch_shrinkwrap(p->n, p->tile, p->psi, p->sup, p->mstride(), p->sw_tmp,
-                                  p->sw_blur, p->sw_maxp, (const SwParam*)dptr(swp_off[k]),
-                                  SwParam{}, (const int*)dptr(sw_off[k]), 0, sw_n[k], sw_r[k],
-                                  st));
-      } else {
-        SPK_HIP(launch_shrinkwrap(p->n, p->tile, p->psi, p->sup, p->mstride(), p->sw_tmp,
-                                  p->sw_blur, p->sw_maxp, nullptr, sw_first[k], nullptr, 0, B,
-                                  sw_r[k], st));
-      }
-      p->phib_fresh = false;
-    }
+    SPK_HIP(launch_rowpass(p->n, p->tile, p->ept, any_dm[k] != 0, emits[k] != 0, a, nact, st));
+    p->phib_fresh = emits[k] != 0;
+    if (any_sw[k] && (rc = shrinkwrap_after(k))) return rc;
+    ++k;
   }
   if (err_trace) {
     SPK_HIP(launch_err_finalize(p->errp, nsteps, B, p->bands, p->summ2, err_trace,

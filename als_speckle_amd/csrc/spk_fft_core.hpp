@@ -23,6 +23,8 @@
 // and last store are both coalesced along t.
 #pragma once
 
+#include <math.h>
+
 #if defined(__HIPCC__)
 #define SPK_HD __host__ __device__ __forceinline__
 #else
@@ -35,18 +37,28 @@ struct alignas(8) c32 {
   float x, y;
 };
 
+// Fused multiply-adds are written out (fmaf) and the library is compiled
+// without implicit contraction, so every value has one fixed sequence of
+// roundings wherever a function is inlined: kernels that share these
+// functions (the per-pass kernels and the persistent kernel) agree bitwise.
 SPK_HD c32 mk(float x, float y) { return c32{x, y}; }
 SPK_HD c32 add(c32 a, c32 b) { return c32{a.x + b.x, a.y + b.y}; }
 SPK_HD c32 sub(c32 a, c32 b) { return c32{a.x - b.x, a.y - b.y}; }
 SPK_HD c32 scale(c32 a, float s) { return c32{a.x * s, a.y * s}; }
 SPK_HD c32 mul(c32 a, c32 b) {
-  return c32{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+  return c32{fmaf(a.x, b.x, -(a.y * b.y)), fmaf(a.x, b.y, a.y * b.x)};
 }
+// a*s + b*t for complex a, b and real s, t
+SPK_HD c32 axpby(c32 a, float s, c32 b, float t) {
+  return c32{fmaf(a.x, s, b.x * t), fmaf(a.y, s, b.y * t)};
+}
+// acc + a*s
+SPK_HD c32 axpy(c32 acc, c32 a, float s) { return c32{fmaf(a.x, s, acc.x), fmaf(a.y, s, acc.y)}; }
 SPK_HD c32 conj(c32 a) { return c32{a.x, -a.y}; }
 // a * w for forward, a * conj(w) for inverse (table holds forward twiddles)
 template <bool INV>
 SPK_HD c32 twiddle(c32 a, c32 w) {
-  return INV ? c32{a.x * w.x + a.y * w.y, a.y * w.x - a.x * w.y} : mul(a, w);
+  return INV ? c32{fmaf(a.x, w.x, a.y * w.y), fmaf(a.y, w.x, -(a.x * w.y))} : mul(a, w);
 }
 // a * (-i) forward, a * (+i) inverse
 template <bool INV>

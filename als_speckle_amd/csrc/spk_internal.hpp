@@ -46,6 +46,17 @@ struct PassArgs {
   float inv_nn;           // 1/(n*n)
 };
 
+// One step of a persistent iteration block (k_persist).  Offsets are bytes
+// into the block's staged table upload (-1: none): tab = per-frame
+// coefficients (else c for every frame), fmap = the nact frames to process.
+struct PStep {
+  Coef c;
+  long long tab_off;
+  long long fmap_off;
+  int nact;
+  int pad_;
+};
+
 struct BlurW {
   float w[48];  // w[d] for |d| <= radius, normalised over the circle
   int radius;
@@ -63,13 +74,21 @@ struct SwParam {  // one frame's shrinkwrap: blur weights and threshold fraction
 extern thread_local hipEvent_t g_launch_events[2];
 
 // Kernel launchers (spk_kernels.hip).  All return hipError_t of the launch.
-// `tile` is the internal layout tile edge (4 or 8).
+// `tile` is the internal layout tile edge (2, 4 or 8).
 // `ept` is the number of values per thread of one transform (8 or 16).
 bool size_supported(int n, int tile, int ept);
 hipError_t launch_colpass(int n, int tile, int ept, int mode, const PassArgs& a,
                           int frames, int zdim, hipStream_t st);
 hipError_t launch_rowpass(int n, int tile, int ept, bool use_t, bool emit_b,
                           const PassArgs& a, int frames, hipStream_t st);
+// persistent iteration block: one cooperative launch of `grid` workgroups runs
+// nsteps (column pass, grid barrier, row pass) steps; bar = barrier counter
+// (zeroed by the launcher), flag set to 1 if a barrier wait gave up
+hipError_t launch_persist(int n, int tile, int ept, const PassArgs& a, const unsigned char* stage,
+                          const PStep* steps, int nsteps, double* errp0, long long errp_step,
+                          unsigned* bar, int* flag, int grid, hipStream_t st);
+// workgroups of the persistent kernel that fit on the current device at once
+hipError_t persist_grid(int n, int tile, int ept, int* out);
 // mode 0: out = rowFFT(in); 1: out = rowIFFT(in) (unscaled);
 // 2: out = rowFFT((s - g*(1-s)) * in)
 hipError_t launch_rowfft(int n, int tile, int ept, int mode, const c32* in, c32* out,

@@ -191,18 +191,16 @@ __device__ __forceinline__ int xcd_band(int bid) {
 // ---------------------------------------------------------------------------
 // Column pass.  MODE 0: FFT -> projection (+ error) -> IFFT
 //               MODE 2: forward FFT only; MODE 3: inverse FFT only
-// grid (n/TT bands, frames, z) ; z = 1 processes phib (DM) without error
+// One item = one band of TT columns of frame f; z = 1 processes phib (DM)
+// without error.  OWN_TW: the item stages the twiddle table into LDS itself
+// (a one-item kernel); otherwise the table is already there (persistent
+// kernel).  errp: partial error sums [frame][band] (MODE 0, z = 0).
 // ---------------------------------------------------------------------------
-template <int N, int TT, int E, int MODE>
-__global__ void __launch_bounds__((N / E) * TT) k_colpass(PassArgs a) {
+template <int N, int TT, int E, int MODE, bool OWN_TW>
+__device__ __forceinline__ void colpass_item(const PassArgs& a, int f, int band, int z,
+                                             double* errp, c32* lds) {
   constexpr int NB = N / TT;
   using L = LdsPlan<N, TT>;
-  __shared__ c32 lds[L::TOTAL];
-  const int band = xcd_band<NB>(blockIdx.x), z = blockIdx.z;
-  const int f = a.fmap ? a.fmap[blockIdx.y] : blockIdx.y;
-  const Coef c = a.tab ? a.tab[f] : a.c;
-  if (!(c.flags & kActive)) return;
-  if (z && !(c.flags & kUseT)) return;
   const size_t fo = (size_t)f * N * N;
   // dual layout: read Phi column-band-major (contiguous), write it tiled for
   // the row pass; single layout: tiles in place
@@ -214,7 +212,7 @@ __global__ void __launch_bounds__((N / E) * TT) k_colpass(PassArgs a) {
 
   // every global load up front: twiddle table, data, modulus
   TwStage<N, TT, NT> tws;
-  tws.fetch(a.tw);
+  if constexpr (OWN_TW) tws.fetch(a.tw);
   c32 x[E];
 #pragma unroll
   for (int i = 0; i < E; ++i)
@@ -229,7 +227,7 @@ __global__ void __launch_bounds__((N / E) * TT) k_colpass(PassArgs a) {
     for (int i = 0; i < E; ++i) m[i] = gld(mod, mcol_pos<N, TT, E>(band, b, t, i));
   }
   c32* twl = lds + L::NBUF * L::BUF;
-  tws.put(twl);
+  if constexpr (OWN_TW) tws.put(twl);
   const auto tw = twiddle_view<N, TT>(twl);
 
   double err = 0.0;
@@ -249,12 +247,12 @@ __global__ void __launch_bounds__((N / E) * TT) k_colpass(PassArgs a) {
       // phasing.py:1217) through one reciprocal square root of |Psi|^2
       // (v_rsq_f32, about 1 ulp); 0 where |Psi|^2 is below the smallest
       // normal float, i.e. |Psi| < 1.1e-19 (the reference gives NaN at 0)
-      const float p2 = x[i].x * x[i].x + x[i].y * x[i].y;
+      const float p2 = fmaf(x[i].x, x[i].x, x[i].y * x[i].y);
       const float rs = p2 >= 1.17549435e-38f ? __builtin_amdgcn_rsqf(p2) : 0.0f;
       const float am = p2 * rs;  // |Psi|
       // error term: exact difference and square in FP64 of the FP32 |Psi|
       const double d = (double)am - (double)m[i];
-      err += d * d;
+      err = fma(d, d, err);
       x[i] = scale(x[i], m[i] * rs);
     }
     fft_regs<N, E, TT, true, false>(x, t, b, lds, tw);
@@ -262,26 +260,35 @@ __global__ void __launch_bounds__((N / E) * TT) k_colpass(PassArgs a) {
 #pragma unroll
   for (int i = 0; i < E; ++i) gst(dst, col_pos<N, TT, E>(band, b, t, i), x[i]);
   if constexpr (MODE == 0) {
-    if (z == 0 && a.errp) {
+    if (z == 0 && errp) {
       const double s = block_sum(err, reinterpret_cast<double*>(lds));
-      if (tid == 0) a.errp[(size_t)f * NB + band] = s;
+      if (tid == 0) errp[(size_t)f * NB + band] = s;
     }
   }
+}
+
+// grid (n/TT bands, frames, z)
+template <int N, int TT, int E, int MODE>
+__global__ void __launch_bounds__((N / E) * TT) k_colpass(PassArgs a) {
+  constexpr int NB = N / TT;
+  __shared__ c32 lds[LdsPlan<N, TT>::TOTAL];
+  const int band = xcd_band<NB>(blockIdx.x), z = blockIdx.z;
+  const int f = a.fmap ? a.fmap[blockIdx.y] : blockIdx.y;
+  const Coef c = a.tab ? a.tab[f] : a.c;
+  if (!(c.flags & kActive)) return;
+  if (z && !(c.flags & kUseT)) return;
+  colpass_item<N, TT, E, MODE, true>(a, f, band, z, a.errp, lds);
 }
 
 // ---------------------------------------------------------------------------
 // Row pass: row IFFT of phi (and phib for DM) -> update -> psi; row FFT of the
 // new psi -> phi (and of dm1(psi) -> phib when the next step is DM).
-// grid (n/TT bands, frames)
+// One item = one band of TT rows of frame f with coefficients c.
 // ---------------------------------------------------------------------------
-template <int N, int TT, int E, bool USE_T, bool EMIT_B>
-__global__ void __launch_bounds__((N / E) * TT) k_rowpass(PassArgs a) {
+template <int N, int TT, int E, bool USE_T, bool EMIT_B, bool OWN_TW>
+__device__ __forceinline__ void rowpass_item(const PassArgs& a, int f, int band, const Coef& c,
+                                             c32* lds) {
   using L = LdsPlan<N, TT>;
-  __shared__ c32 lds[L::TOTAL];
-  const int band = blockIdx.x;
-  const int f = a.fmap ? a.fmap[blockIdx.y] : blockIdx.y;
-  const Coef c = a.tab ? a.tab[f] : a.c;
-  if (!(c.flags & kActive)) return;
   const size_t fo = (size_t)f * N * N;
   const int tid = threadIdx.x, b = tid % TT, t = tid / TT;
   constexpr int NT = (N / E) * TT;
@@ -291,7 +298,7 @@ __global__ void __launch_bounds__((N / E) * TT) k_rowpass(PassArgs a) {
 
   // every global load up front: twiddles, phi (and phib), psi_in, support
   TwStage<N, TT, NT> tws;
-  tws.fetch(a.tw);
+  if constexpr (OWN_TW) tws.fetch(a.tw);
   c32 x[E];
 #pragma unroll
   for (int i = 0; i < E; ++i) x[i] = gld(phi, SPK_RPOS(i));
@@ -314,7 +321,7 @@ __global__ void __launch_bounds__((N / E) * TT) k_rowpass(PassArgs a) {
     }
   }
   c32* twl = lds + L::NBUF * L::BUF;
-  tws.put(twl);
+  if constexpr (OWN_TW) tws.put(twl);
   const auto tw = twiddle_view<N, TT>(twl);
 
   fft_regs<N, E, TT, true, true>(x, t, b, lds, tw);
@@ -333,17 +340,17 @@ __global__ void __launch_bounds__((N / E) * TT) k_rowpass(PassArgs a) {
     const float s = sv[i];
     const float u = 1.0f - s;
     const c32 pout = scale(x[i], a.inv_nn);
-    c32 in_s = add(scale(pout, c.ao), scale(pin[i], c.ai));
-    c32 out_s = add(scale(pout, c.bo), scale(pin[i], c.bi));
+    c32 in_s = axpby(pout, c.ao, pin[i], c.ai);
+    c32 out_s = axpby(pout, c.bo, pin[i], c.bi);
     if constexpr (USE_T) {
       const c32 tt = scale(tv[i], a.inv_nn);
-      in_s = add(in_s, scale(tt, c.at));
-      out_s = add(out_s, scale(tt, c.bt));
+      in_s = axpy(in_s, tt, c.at);
+      out_s = axpy(out_s, tt, c.bt);
     }
-    const c32 nv = add(scale(in_s, s), scale(out_s, u));
+    const c32 nv = axpby(in_s, s, out_s, u);
     gst(psi, SPK_RPOS(i), nv);
     x[i] = nv;
-    if constexpr (EMIT_B) y[i] = scale(nv, s - c.g_next * u);
+    if constexpr (EMIT_B) y[i] = scale(nv, fmaf(-c.g_next, u, s));
   }
   // the new row transforms: column-band-major for the column pass (dual
   // layout) or back in place (single layout)
@@ -363,6 +370,105 @@ __global__ void __launch_bounds__((N / E) * TT) k_rowpass(PassArgs a) {
     else gst(phi, SPK_RPOS(i), x[i]);
   }
 #undef SPK_RPOS
+}
+
+// grid (n/TT bands, frames)
+template <int N, int TT, int E, bool USE_T, bool EMIT_B>
+__global__ void __launch_bounds__((N / E) * TT) k_rowpass(PassArgs a) {
+  __shared__ c32 lds[LdsPlan<N, TT>::TOTAL];
+  const int f = a.fmap ? a.fmap[blockIdx.y] : blockIdx.y;
+  const Coef c = a.tab ? a.tab[f] : a.c;
+  if (!(c.flags & kActive)) return;
+  rowpass_item<N, TT, E, USE_T, EMIT_B, true>(a, f, blockIdx.x, c, lds);
+}
+
+// ---------------------------------------------------------------------------
+// Persistent iteration block: one cooperative launch runs `nsteps` steps
+// (column pass, grid barrier, row pass, grid barrier), each workgroup taking
+// items (frame, band) in a strided loop.  Steps here use neither DM nor a
+// DM successor (no phib); shrinkwrap events end a block on the host side.
+// The twiddle table is staged into LDS once per launch.
+//
+// Grid barrier: a monotonically increasing counter in device memory, zeroed
+// before the launch; barrier k waits until it reaches k * gridDim.x.  After a
+// workgroup barrier, thread 0 fences at device scope before its arrival
+// (release: the workgroup's stores leave the XCD's L2) and after the wait
+// (acquire: stale lines of other XCDs' data are dropped), then the workgroup
+// barrier releases the other waves.  The wait is bounded: after 2^18
+// polls (a quarter second or more, far above any legitimate wait; the launch
+// is cooperative, so every workgroup is resident) the flag is set and the
+// workgroup goes on, later barriers of the launch pass at once, and every
+// wave ends; the host reports the flag as an error (spk_plan_check).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void grid_sync(unsigned* bar, unsigned target, int* flag) {
+  // every wave waits for its own stores to reach L2 (s_waitcnt vmcnt(0):
+  // gfx9 encoding, expcnt and lgkmcnt left at their maxima), then thread 0
+  // publishes the workgroup's stores device-wide: one L2 write-back and one
+  // invalidate per workgroup, not per wave (a workgroup runs on one CU)
+  __builtin_amdgcn_s_waitcnt(0x0F70);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __hip_atomic_fetch_add(bar, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    // after one give-up every later barrier of the launch passes at once
+    if (!__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+      unsigned polls = 0;
+      while (__hip_atomic_load(bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+        __builtin_amdgcn_s_sleep(1);
+        if (++polls > (1u << 18)) {
+          __hip_atomic_store(flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          break;
+        }
+      }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  }
+  __syncthreads();
+}
+
+template <int N, int TT, int E>
+__global__ void __launch_bounds__((N / E) * TT)
+    k_persist(PassArgs a, const unsigned char* stage, const PStep* steps, int nsteps,
+              double* errp0, long long errp_step, unsigned* bar, int* flag) {
+  constexpr int NB = N / TT;
+  constexpr int NT = (N / E) * TT;
+  using L = LdsPlan<N, TT>;
+  __shared__ c32 lds[L::TOTAL];
+  {
+    TwStage<N, TT, NT> tws;
+    tws.fetch(a.tw);
+    tws.put(lds + L::NBUF * L::BUF);
+  }
+  const unsigned G = gridDim.x;
+  unsigned epoch = 0;
+  for (int k = 0; k < nsteps; ++k) {
+    const PStep ps = steps[k];
+    const int items = ps.nact * NB;
+    const Coef* tab = ps.tab_off >= 0 ? reinterpret_cast<const Coef*>(stage + ps.tab_off) : nullptr;
+    const int* fmap = ps.fmap_off >= 0 ? reinterpret_cast<const int*>(stage + ps.fmap_off) : nullptr;
+    double* errp = errp0 ? errp0 + (size_t)k * errp_step : nullptr;
+    for (int it = blockIdx.x; it < items; it += G) {
+      const int fi = it / NB, band = it % NB;
+      const int f = fmap ? fmap[fi] : fi;
+      const Coef c = tab ? tab[f] : ps.c;
+      __syncthreads();  // twiddles staged / previous item done with LDS
+      if (c.flags & kActive) colpass_item<N, TT, E, 0, false>(a, f, band, 0, errp, lds);
+    }
+    grid_sync(bar, ++epoch * G, flag);
+    for (int it = blockIdx.x; it < items; it += G) {
+      const int fi = it / NB, band = it % NB;
+      const int f = fmap ? fmap[fi] : fi;
+      const Coef c = tab ? tab[f] : ps.c;
+      __syncthreads();
+      if (c.flags & kActive) rowpass_item<N, TT, E, false, false, false>(a, f, band, c, lds);
+    }
+    if (k + 1 < nsteps) grid_sync(bar, ++epoch * G, flag);
+  }
+}
+
+static __global__ void k_zero_u32(unsigned* p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0u;
 }
 
 // ---------------------------------------------------------------------------
@@ -534,7 +640,7 @@ __device__ __forceinline__ void blur_window(const float* line, int c0, const flo
 #pragma unroll
   for (int d = 1; d <= RMAX; ++d) {
 #pragma unroll
-    for (int k = 0; k < K; ++k) acc[k] += w[d] * (win[k + RMAX + d] + win[k + RMAX - d]);
+    for (int k = 0; k < K; ++k) acc[k] = fmaf(w[d], win[k + RMAX + d] + win[k + RMAX - d], acc[k]);
   }
 }
 
@@ -938,6 +1044,31 @@ static hipError_t rowpass_nt(bool use_t, bool emit_b, const PassArgs& a, int fra
 }
 
 template <int N, int TT, int E>
+static hipError_t persist_nt(const PassArgs& a, const unsigned char* stage, const PStep* steps,
+                             int nsteps, double* errp0, long long errp_step, unsigned* bar,
+                             int* flag, int grid, hipStream_t st) {
+  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, bar, 1);
+  void* args[] = {(void*)&a, (void*)&stage, (void*)&steps, (void*)&nsteps, (void*)&errp0,
+                  (void*)&errp_step, (void*)&bar, (void*)&flag};
+  return hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&k_persist<N, TT, E>),
+                                    dim3(grid), dim3((N / E) * TT), args, 0, st);
+}
+
+// co-resident workgroups of the persistent kernel on the current device
+template <int N, int TT, int E>
+static hipError_t persist_grid_nt(int* out) {
+  int occ = 0, dev = 0, cus = 0;
+  hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+      &occ, reinterpret_cast<const void*>(&k_persist<N, TT, E>), (N / E) * TT, 0);
+  if (e != hipSuccess) return e;
+  if ((e = hipGetDevice(&dev)) != hipSuccess) return e;
+  if ((e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev)) != hipSuccess)
+    return e;
+  *out = occ * cus;
+  return hipSuccess;
+}
+
+template <int N, int TT, int E>
 static hipError_t rowfft_nt(int mode, const c32* in, c32* out, const float* sup,
                             long long sstride, float g, const c32* tw, const int* fmap,
                             int frames, hipStream_t st) {
@@ -1001,7 +1132,11 @@ constexpr bool shape_ok() {
   hipError_t shrinkwrap_##n(int tile, const c32* psi, float* sup, long long sstride,       \
                             float* tmp, float* blur, float* maxp, const SwParam* tab,      \
                             const SwParam& p0, const int* fmap, int frames, int rmax,      \
-                            hipStream_t st);
+                            hipStream_t st);                                                \
+  hipError_t persist_##n(int tile, int ept, const PassArgs& a, const unsigned char* stage,  \
+                         const PStep* steps, int nsteps, double* errp0, long long errp_step,\
+                         unsigned* bar, int* flag, int grid, hipStream_t st);               \
+  hipError_t persist_grid_##n(int tile, int ept, int* out);
 
 // a shape that is not built compiles to an error return, not a kernel
 #define SPK_CALL_COL(TT_, E_)                                                   \
@@ -1009,6 +1144,17 @@ constexpr bool shape_ok() {
                                            (shape_ok<SPK_N, TT_, E_>() ? E_ : 16)>(     \
                                     mode, a, frames, zdim, st)                          \
                               : hipErrorInvalidValue)
+#define SPK_CALL_PERSIST(TT_, E_)                                                 \
+  (shape_ok<SPK_N, TT_, E_>() ? persist_nt<SPK_N, (shape_ok<SPK_N, TT_, E_>() ? TT_ : 4),  \
+                                           (shape_ok<SPK_N, TT_, E_>() ? E_ : 16)>(      \
+                                    a, stage, steps, nsteps, errp0, errp_step, bar, flag,  \
+                                    grid, st)                                              \
+                              : hipErrorInvalidValue)
+#define SPK_CALL_PGRID(TT_, E_)                                                   \
+  (shape_ok<SPK_N, TT_, E_>()                                                     \
+       ? persist_grid_nt<SPK_N, (shape_ok<SPK_N, TT_, E_>() ? TT_ : 4),           \
+                         (shape_ok<SPK_N, TT_, E_>() ? E_ : 16)>(out)             \
+       : hipErrorInvalidValue)
 #define SPK_CALL_ROW(TT_, E_)                                                   \
   (shape_ok<SPK_N, TT_, E_>() ? rowpass_nt<SPK_N, (shape_ok<SPK_N, TT_, E_>() ? TT_ : 4), \
                                            (shape_ok<SPK_N, TT_, E_>() ? E_ : 16)>(     \
@@ -1049,6 +1195,16 @@ constexpr bool shape_ok() {
       return shrinkwrap_nt<n, (n == 4096 ? 4 : 8)>(psi, sup, sstride, tmp, blur, maxp, tab, \
                                                    p0, fmap, frames, rmax, st);             \
     return hipErrorInvalidValue;                                                            \
+  }                                                                                         \
+  hipError_t persist_##n(int tile, int ept, const PassArgs& a, const unsigned char* stage,  \
+                         const PStep* steps, int nsteps, double* errp0, long long errp_step,\
+                         unsigned* bar, int* flag, int grid, hipStream_t st) {              \
+    constexpr int SPK_N = n;                                                                \
+    SPK_SHAPE_SWITCH(n, SPK_CALL_PERSIST)                                                   \
+  }                                                                                         \
+  hipError_t persist_grid_##n(int tile, int ept, int* out) {                                \
+    constexpr int SPK_N = n;                                                                \
+    SPK_SHAPE_SWITCH(n, SPK_CALL_PGRID)                                                     \
   }
 
 SPK_SIZES(SPK_SIZED_DECL)
@@ -1122,6 +1278,17 @@ hipError_t launch_rowfft(int n, int tile, int ept, int mode, const c32* in, c32*
                          const float* sup, long long sstride, float g, const c32* tw,
                          const int* fmap, int frames, hipStream_t st) {
   SPK_SIZE_SWITCH(rowfft, tile, ept, mode, in, out, sup, sstride, g, tw, fmap, frames, st)
+}
+
+hipError_t launch_persist(int n, int tile, int ept, const PassArgs& a, const unsigned char* stage,
+                          const PStep* steps, int nsteps, double* errp0, long long errp_step,
+                          unsigned* bar, int* flag, int grid, hipStream_t st) {
+  SPK_SIZE_SWITCH(persist, tile, ept, a, stage, steps, nsteps, errp0, errp_step, bar, flag, grid,
+                  st)
+}
+
+hipError_t persist_grid(int n, int tile, int ept, int* out) {
+  SPK_SIZE_SWITCH(persist_grid, tile, ept, out)
 }
 
 hipError_t launch_shrinkwrap(int n, int tile, const c32* psi, float* sup, long long sstride,

@@ -125,12 +125,14 @@ class Plan:
     """``batch`` resident frames of n x n (see spk_plan_create)."""
 
     def __init__(self, ctx: Context, n: int, batch: int = 1, shared: bool = True,
-                 tile: int = 0, ept: int = 0):
+                 tile: int = 0, ept: int = 0, persist: bool = False):
         self.ctx, self.lib = ctx, ctx.lib
         self.n, self.batch, self.shared = int(n), int(batch), bool(shared)
+        self.persist = bool(persist)
         tune = _capi.SpkTune()
         tune.tile = int(tile)
         tune.ept = int(ept)
+        tune.persist = int(self.persist)
         h = c_void_p()
         check(self.lib.spk_plan_create(ctx.handle, self.n, self.batch, int(self.shared),
                                        byref(tune), byref(h)))
@@ -259,6 +261,10 @@ class Plan:
         ms = c_float()
         check(self.lib.spk_time_passes(self.handle, which, reps, arr, byref(ms)))
         return ms.value
+
+    def check(self) -> None:
+        """Wait for queued work; raise if a persistent block's barrier gave up."""
+        check(self.lib.spk_plan_check(self.handle))
 
     def close(self) -> None:
         if getattr(self, "handle", None):

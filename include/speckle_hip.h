@@ -64,7 +64,8 @@ typedef struct spk_step {
 typedef struct spk_tune {
   int32_t tile;       /* tile edge of the internal layout: 2, 4 or 8         */
   int32_t ept;        /* values per thread of one transform: 8 or 16         */
-  int32_t reserved[6];
+  int32_t persist;    /* 1: run iteration blocks as one persistent launch    */
+  int32_t reserved[5];
 } spk_tune;
 
 /* -------------------------------------------------------------------------
@@ -85,6 +86,12 @@ const char* spk_last_error(void);
 /* library version string and the sizes built in (bitmask of log2 N) */
 const char* spk_version(void);
 uint32_t spk_supported_sizes(void);
+
+/* Waits for the plan's queued work and reports whether a grid barrier of a
+ * persistent iteration block gave up waiting (SPK_EHIP; the flag is then
+ * cleared).  No counterpart in the reference, whose kernels each end with a
+ * blocking wait (gpu.py:349). */
+int spk_plan_check(spk_plan* plan);
 
 /* -------------------------------------------------------------------------
  * Plans.  A plan holds `batch` frames of n x n (n a power of two, as pyfft
