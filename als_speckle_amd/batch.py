@@ -24,13 +24,18 @@ __all__ = ["phasing_batch"]
 class phasing_batch:
     """``trials`` random-start reconstructions of one modulus/support."""
 
-    def __init__(self, trials, gpu_info=None, tile=0, ept=0):
+    def __init__(self, trials, gpu_info=None, tile=0, ept=0, autotune=False):
+        """tile/ept: fixed FFT layout and values per thread (0: library
+        default); autotune: measure every built (tile, ept, launch mode) for
+        this frame size and trial count at ``load`` and use the fastest."""
         require_gpu()
         assert int(trials) >= 1, "trials must be >= 1"
         self.trials = int(trials)
         self.ctx = gpu_info if isinstance(gpu_info, Context) else Context(
             int(gpu_info) if gpu_info is not None else 0)
-        self._tune = (tile, ept)
+        self._tune = (tile, ept, False)
+        self._autotune = bool(autotune)
+        self.tuning = None
         self.N = self.shape = None
         self._plan = None
         self._trace = None
@@ -47,8 +52,12 @@ class phasing_batch:
         sup *= 1.0 / sup.max()
         if self._plan is None or self.N != n:
             self.N, self.shape = n, modulus.shape
+            if self._autotune:
+                from .tune import autotune
+                self.tuning = autotune(self.ctx, n, self.trials, shared=True)
+                self._tune = (self.tuning.tile, self.tuning.ept, self.tuning.persist)
             self._plan = Plan(self.ctx, n, batch=self.trials, shared=True,
-                              tile=self._tune[0], ept=self._tune[1])
+                              tile=self._tune[0], ept=self._tune[1], persist=self._tune[2])
         dev = self.ctx.device
         self._m = to_device(modulus.astype(np.float32), dev, torch.float32)
         self._s = to_device(sup, dev, torch.float32)

@@ -81,11 +81,19 @@ _NOOP = (_capi.NOOP, 0, 0.0, 0.0, 0.0)
 class _Pool:
     """Resident slots of one frame size on their own stream."""
 
-    def __init__(self, device_index, n, slots):
+    def __init__(self, device_index, n, slots, autotune=False):
         self.n, self.slots = n, slots
         self.stream = torch.cuda.Stream(device=device_index)
         self.ctx = Context(device_index, stream=self.stream)
-        self.plan = Plan(self.ctx, n, batch=slots, shared=False)
+        tile = ept = 0
+        persist = False
+        if autotune:
+            from .tune import autotune as tune_fn
+            with torch.cuda.stream(self.stream):
+                r = tune_fn(self.ctx, n, slots, shared=False)
+            tile, ept, persist = r.tile, r.ept, r.persist
+        self.plan = Plan(self.ctx, n, batch=slots, shared=False, tile=tile, ept=ept,
+                         persist=persist)
         self.free = list(range(slots))
         self.active = {}          # slot -> [request, steps, pos, trace chunks, admitted]
         self.event = None
@@ -152,17 +160,19 @@ class Server:
     slots: {n: slot count}; or budget_bytes plus mix {n: request fraction},
     which sizes each pool in proportion to its expected work (fraction x n^2)
     within the budget.  block: iterations per launch round (also the
-    admission granularity).
+    admission granularity).  autotune: pick each pool's FFT layout, radix
+    schedule and launch mode by measurement (tune.autotune).
     """
 
-    def __init__(self, device=0, slots=None, budget_bytes=None, mix=None, block=20):
+    def __init__(self, device=0, slots=None, budget_bytes=None, mix=None, block=20,
+                 autotune=False):
         require_gpu()
         self.device_index = int(device)
         self.block = int(block)
         if slots is None:
             assert budget_bytes and mix, "give slots or budget_bytes and mix"
             slots = self.size_slots(budget_bytes, mix)
-        self.pools = {int(n): _Pool(self.device_index, int(n), int(k))
+        self.pools = {int(n): _Pool(self.device_index, int(n), int(k), autotune)
                       for n, k in sorted(slots.items()) if k > 0}
         self.queue = collections.deque()
         self.results = []

@@ -196,14 +196,17 @@ def bench_trials_512(ctx):
     from als_speckle_amd.batch import phasing_batch
     n, T, iters = 512, 64, 500
     f = synth.config_frame("trials_512", 0)
-    b = phasing_batch(T, gpu_info=ctx)
+    b = phasing_batch(T, gpu_info=ctx, autotune=True)
     b.load(f.modulus, f.support)
+    log(f"trials_512 autotuned schedule: tile {b.tuning.tile}, {b.tuning.ept} values/thread, "
+        f"persistent {b.tuning.persist} ({b.tuning.us_per_iter} us per batched HIO iteration)")
     g = np.empty((T, n, n), np.complex64)
     for t in range(T):
         np.random.seed(t)
         g[t] = (np.random.rand(n, n) + 1j * np.random.rand(n, n)).astype(np.complex64)
     g_d = torch.from_numpy(g).to(ctx.device)
-    out = {}
+    out = {"schedule": {"tile": b.tuning.tile, "values_per_thread": b.tuning.ept,
+                        "persistent": b.tuning.persist, "autotune_table": b.tuning.table}}
     for alg, beta, bpx in (("raar", 0.87, B_COL + B_ROW), ("dm", 0.7, 92)):
         b.seed(guesses=g_d)
         b.iterate(10, order=((alg, 1),), beta=beta)           # warm-up

@@ -93,3 +93,29 @@ def test_server_matches_dropin_object():
     r.iterate(60, order=req.order, beta=req.beta, shrinkwrap=True)
     assert np.array_equal(r.get(r.psi_in), res.psi.cpu().numpy())
     assert np.array_equal(r.get_error_trace(), res.trace)
+
+
+def test_autotune_picks_a_built_shape_and_caches(tmp_path):
+    from als_speckle_amd.device import Context
+    from als_speckle_amd.tune import autotune, candidates
+    ctx = Context(0)
+    path = str(tmp_path / "tune.json")
+    r = autotune(ctx, 64, batch=3, shared=True, iters=4, cache_path=path)
+    assert (r.tile, r.ept, r.persist) in candidates(64)
+    assert len(r.table) == len(candidates(64))
+    assert r.us_per_iter == min(row["us_per_iter"] for row in r.table)
+    assert autotune(ctx, 64, batch=3, shared=True) is r          # memory cache
+    import json
+    disk = json.load(open(path))
+    assert any(k.endswith("n=64|batch=3|shared=1") for k in disk)
+    # a tuned batch gives the same trials as a default one to FP32 rounding
+    # (different radix schedules round differently)
+    f = synth.make_frame(64, 4, shape="disk", radius=10, photons=1e6)
+    runs = []
+    for auto in (False, True):
+        b = phasing_batch(3, gpu_info=ctx, autotune=auto)
+        b.load(f.modulus, f.support)
+        b.seed([5, 6, 7])
+        b.iterate(5, order=(("hio", 1),), beta=0.9)
+        runs.append(b.get_error_trace())
+    assert np.allclose(runs[0], runs[1], rtol=1e-5)

@@ -126,3 +126,16 @@ def test_server_budget_sizing():
     assert slots[2048] >= 1 and slots[256] >= 1
     # larger frames get more of the budget, smaller frames more slots
     assert slots[256] >= slots[2048]
+
+
+def test_autotune_candidates_are_built_shapes():
+    from als_speckle_amd.tune import candidates
+    for n in (16, 64, 256, 1024, 2048, 4096):
+        cands = candidates(n)
+        assert cands and all(p in (False, True) for _, _, p in cands)
+        for tile, ept, _ in cands:
+            # every candidate is a shape the library builds
+            assert (n // ept) * tile <= 1024 and not (n == 4096 and tile == 8)
+        assert (4, 16, False) in cands
+    assert (2, 8, False) in candidates(4096) and (4, 16, False) in candidates(4096)
+    assert all(t != 8 for t, _, _ in candidates(4096))
