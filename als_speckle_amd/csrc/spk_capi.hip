@@ -121,6 +121,12 @@ struct spk_plan {
   int persist = 0;
   int pgrid = 0;
   unsigned* pbar = nullptr;
+  // partial coherence: ipsf (tiled, shared by the frames) and the per-frame
+  // work buffers of the |Psi|^2 convolution
+  c32* ipsf = nullptr;
+  c32* dbuf = nullptr;
+  c32* dbufc = nullptr;
+  bool ipsf_on = false;
   float phib_g = 0.0f;
   long long nn() const { return (long long)n * n; }
   // Phi kept in two layouts (see spk_fft_core.hpp dual_layout): n >= 4096
@@ -214,6 +220,8 @@ PassArgs base_args(spk_plan* p) {
   a.errp = nullptr;
   a.tab = nullptr;
   a.inv_nn = (float)(1.0 / ((double)p->n * p->n));
+  a.dbuf = p->dbuf;
+  a.dbufc = p->dual() ? p->dbufc : p->dbuf;
   return a;
 }
 
@@ -462,6 +470,7 @@ int spk_plan_destroy(spk_plan* p) {
   dfree(p->psi); dfree(p->phi); dfree(p->phic); dfree(p->phib); dfree(p->phibc);
   dfree(p->mod); dfree(p->sup);
   dfree(p->tw); dfree(p->summ2); dfree(p->sumpart); dfree(p->errp); dfree(p->pbar);
+  dfree(p->ipsf); dfree(p->dbuf); dfree(p->dbufc);
   dfree(p->sw_tmp); dfree(p->sw_blur); dfree(p->sw_maxp); dfree(p->fftbuf);
   dfree(p->stage_d);
   prof_clear(p, 0);
@@ -561,6 +570,26 @@ int spk_get_psi(spk_plan* p, int frame, float* out) {
   return SPK_OK;
 }
 
+int spk_set_ipsf(spk_plan* p, const float* ipsf) {
+  SPK_REQUIRE(p, SPK_EINVAL, "null plan");
+  int rc = set_device(p->ctx);
+  if (rc) return rc;
+  if (!ipsf) {
+    p->ipsf_on = false;
+    return SPK_OK;
+  }
+  const size_t nn = (size_t)p->nn();
+  if (!p->ipsf) {
+    if ((rc = dalloc(&p->ipsf, nn)) || (rc = dalloc(&p->dbuf, nn * p->batch)) ||
+        (p->dual() && (rc = dalloc(&p->dbufc, nn * p->batch))))
+      return rc;
+  }
+  SPK_HIP(launch_pack_c64(p->n, p->tile, p->ipsf, (const c32*)ipsf, nullptr, 0, 1,
+                          p->ctx->stream));
+  p->ipsf_on = true;
+  return SPK_OK;
+}
+
 int spk_iterate(spk_plan* p, const spk_step* steps, int nsteps, int per_frame,
                 double* err_trace) {
   SPK_REQUIRE(p, SPK_EINVAL, "null plan");
@@ -602,6 +631,8 @@ int spk_iterate(spk_plan* p, const spk_step* steps, int nsteps, int per_frame,
   }
   bool dm_any = false;
   for (int k = 0; k < nsteps; ++k) dm_any |= any_dm[k] != 0;
+  SPK_REQUIRE(!(dm_any && p->ipsf_on), SPK_EUNSUP,
+              "difference map with a partial-coherence ipsf is not built");
   if (dm_any && (rc = ensure_phib(p))) return rc;
   bool sw_any = false;
   for (int k = 0; k < nsteps; ++k) sw_any |= any_sw[k] != 0;
@@ -680,7 +711,7 @@ int spk_iterate(spk_plan* p, const spk_step* steps, int nsteps, int per_frame,
     for (int f = 0; f < W; ++f) emits[k] |= (coefs[(size_t)k * W + f].flags & kEmitB) != 0;
   std::vector<int> block_end(nsteps, 0);
   std::vector<size_t> block_off(nsteps, 0);
-  if (p->persist) {
+  if (p->persist && !p->ipsf_on) {
     auto fits = [&](int j) { return !any_dm[j] && !emits[j]; };
     for (int k = 0; k < nsteps;) {
       if (!fits(k)) { ++k; continue; }
@@ -769,8 +800,19 @@ int spk_iterate(spk_plan* p, const spk_step* steps, int nsteps, int per_frame,
                             per_frame ? dm_n[k] : B, st));
     }
     a.errp = err_trace ? p->errp + (size_t)k * B * p->bands : nullptr;
-    if ((rc = prof_arm(p, 0))) return rc;
-    SPK_HIP(launch_colpass(p->n, p->tile, p->ept, 0, a, nact, any_dm[k] ? 2 : 1, st));
+    if (p->ipsf_on) {
+      // partially coherent projection: column pass A (|Psi|^2 -> column
+      // FFT), the row convolution with the ipsf, column pass B (divisor,
+      // replacement, error, column IFFT)
+      SPK_HIP(launch_colpass(p->n, p->tile, p->ept, 4, a, nact, 1, st));
+      SPK_HIP(launch_rowconv(p->n, p->tile, p->ept, p->dbuf, p->dual() ? p->dbufc : p->dbuf,
+                             p->ipsf, p->tw, a.fmap, nact, st));
+      if ((rc = prof_arm(p, 0))) return rc;
+      SPK_HIP(launch_colpass(p->n, p->tile, p->ept, 5, a, nact, 1, st));
+    } else {
+      if ((rc = prof_arm(p, 0))) return rc;
+      SPK_HIP(launch_colpass(p->n, p->tile, p->ept, 0, a, nact, any_dm[k] ? 2 : 1, st));
+    }
     a.errp = nullptr;
     if ((rc = prof_arm(p, 1))) return rc;
     SPK_HIP(launch_rowpass(p->n, p->tile, p->ept, any_dm[k] != 0, emits[k] != 0, a, nact, st));

@@ -44,6 +44,10 @@ struct PassArgs {
   const int* fmap;        // frames to process: blockIdx.y -> frame (nullptr: identity)
   Coef c;
   float inv_nn;           // 1/(n*n)
+  // partial coherence (ipsf): per-frame work buffer for the transform of
+  // |Psi|^2, tiled (dbuf) and, in the dual layout, column-band-major (dbufc)
+  c32* dbuf;
+  c32* dbufc;
 };
 
 // One step of a persistent iteration block (k_persist).  Offsets are bytes
@@ -87,6 +91,11 @@ hipError_t launch_rowpass(int n, int tile, int ept, bool use_t, bool emit_b,
 hipError_t launch_persist(int n, int tile, int ept, const PassArgs& a, const unsigned char* stage,
                           const PStep* steps, int nsteps, double* errp0, long long errp_step,
                           unsigned* bar, int* flag, int grid, hipStream_t st);
+// partial coherence: row pass of the |Psi|^2 convolution,
+// out = rowIFFT(ipsf * rowFFT(in)) (unscaled); ipsf tiled, shared by frames;
+// out is tiled (in place allowed) or, in the dual layout, column-band-major
+hipError_t launch_rowconv(int n, int tile, int ept, const c32* in, c32* out, const c32* ipsf,
+                          const c32* tw, const int* fmap, int frames, hipStream_t st);
 // workgroups of the persistent kernel that fit on the current device at once
 hipError_t persist_grid(int n, int tile, int ept, int* out);
 // mode 0: out = rowFFT(in); 1: out = rowIFFT(in) (unscaled);

@@ -191,6 +191,8 @@ __device__ __forceinline__ int xcd_band(int bid) {
 // ---------------------------------------------------------------------------
 // Column pass.  MODE 0: FFT -> projection (+ error) -> IFFT
 //               MODE 2: forward FFT only; MODE 3: inverse FFT only
+//               MODE 4, 5: the two column passes of the partially coherent
+//               projection (see colpass_ipsf_a / colpass_ipsf_b)
 // One item = one band of TT columns of frame f; z = 1 processes phib (DM)
 // without error.  OWN_TW: the item stages the twiddle table into LDS itself
 // (a one-item kernel); otherwise the table is already there (persistent
@@ -267,6 +269,88 @@ __device__ __forceinline__ void colpass_item(const PassArgs& a, int f, int band,
   }
 }
 
+// Partial coherence, column pass A (phasing.py:1175-1180 with ipsf):
+// Psi = colFFT(Phi); Q = colFFT(|Psi|^2) -> dbuf (tiled).  Phi is kept.
+template <int N, int TT, int E>
+__device__ __forceinline__ void colpass_ipsf_a(const PassArgs& a, int f, int band, c32* lds) {
+  using L = LdsPlan<N, TT>;
+  constexpr int NT = (N / E) * TT;
+  constexpr bool DUAL = dual_layout<N>();
+  const size_t fo = (size_t)f * N * N;
+  const c32* src = (DUAL ? a.phic : a.phi) + fo;
+  const int tid = threadIdx.x, b = tid % TT, t = tid / TT;
+  TwStage<N, TT, NT> tws;
+  tws.fetch(a.tw);
+  c32 x[E];
+#pragma unroll
+  for (int i = 0; i < E; ++i)
+    x[i] = gld(src, DUAL ? mcol_pos<N, TT, E>(band, b, t, i) : col_pos<N, TT, E>(band, b, t, i));
+  c32* twl = lds + L::NBUF * L::BUF;
+  tws.put(twl);
+  const auto tw = twiddle_view<N, TT>(twl);
+  fft_regs<N, E, TT, false, true>(x, t, b, lds, tw);
+#pragma unroll
+  for (int i = 0; i < E; ++i) x[i] = c32{fmaf(x[i].x, x[i].x, x[i].y * x[i].y), 0.0f};
+  fft_regs<N, E, TT, false, false>(x, t, b, lds, tw);
+#pragma unroll
+  for (int i = 0; i < E; ++i) gst(a.dbuf + fo, col_pos<N, TT, E>(band, b, t, i), x[i]);
+}
+
+// Partial coherence, column pass B: div = sqrt(|colIFFT(D)| / n^2) with D the
+// row pass's rowIFFT(ipsf * rowFFT(Q)), i.e. div = sqrt(|ifft2(fft2(|Psi|^2)
+// * ipsf)|) (phasing.py:1181-1183); then Psi' = m * Psi / div (0 where
+// div = 0; phasing.py:1217), the FP64 error partial of (div - m)^2, and
+// Phi = colIFFT(Psi').
+template <int N, int TT, int E>
+__device__ __forceinline__ void colpass_ipsf_b(const PassArgs& a, int f, int band,
+                                               double* errp, c32* lds) {
+  using L = LdsPlan<N, TT>;
+  constexpr int NB = N / TT;
+  constexpr int NT = (N / E) * TT;
+  constexpr bool DUAL = dual_layout<N>();
+  const size_t fo = (size_t)f * N * N;
+  const c32* src = (DUAL ? a.phic : a.phi) + fo;
+  const c32* dsrc = (DUAL ? a.dbufc : a.dbuf) + fo;
+  const int tid = threadIdx.x, b = tid % TT, t = tid / TT;
+  TwStage<N, TT, NT> tws;
+  tws.fetch(a.tw);
+  c32 d[E];
+#pragma unroll
+  for (int i = 0; i < E; ++i)
+    d[i] = gld(dsrc, DUAL ? mcol_pos<N, TT, E>(band, b, t, i) : col_pos<N, TT, E>(band, b, t, i));
+  c32 x[E];
+#pragma unroll
+  for (int i = 0; i < E; ++i)
+    x[i] = gld(src, DUAL ? mcol_pos<N, TT, E>(band, b, t, i) : col_pos<N, TT, E>(band, b, t, i));
+  const float* __restrict__ mod = a.mod + (size_t)f * a.mstride;
+  float m[E];
+#pragma unroll
+  for (int i = 0; i < E; ++i) m[i] = gld(mod, mcol_pos<N, TT, E>(band, b, t, i));
+  c32* twl = lds + L::NBUF * L::BUF;
+  tws.put(twl);
+  const auto tw = twiddle_view<N, TT>(twl);
+  fft_regs<N, E, TT, true, true>(d, t, b, lds, tw);
+  float div[E];
+#pragma unroll
+  for (int i = 0; i < E; ++i)
+    div[i] = sqrtf(sqrtf(fmaf(d[i].x, d[i].x, d[i].y * d[i].y)) * a.inv_nn);
+  fft_regs<N, E, TT, false, false>(x, t, b, lds, tw);
+  double err = 0.0;
+#pragma unroll
+  for (int i = 0; i < E; ++i) {
+    const double dd = (double)div[i] - (double)m[i];
+    err = fma(dd, dd, err);
+    x[i] = scale(x[i], div[i] > 0.0f ? m[i] / div[i] : 0.0f);
+  }
+  fft_regs<N, E, TT, true, false>(x, t, b, lds, tw);
+#pragma unroll
+  for (int i = 0; i < E; ++i) gst(a.phi + fo, col_pos<N, TT, E>(band, b, t, i), x[i]);
+  if (errp) {
+    const double s = block_sum(err, reinterpret_cast<double*>(lds));
+    if (tid == 0) errp[(size_t)f * NB + band] = s;
+  }
+}
+
 // grid (n/TT bands, frames, z)
 template <int N, int TT, int E, int MODE>
 __global__ void __launch_bounds__((N / E) * TT) k_colpass(PassArgs a) {
@@ -277,7 +361,13 @@ __global__ void __launch_bounds__((N / E) * TT) k_colpass(PassArgs a) {
   const Coef c = a.tab ? a.tab[f] : a.c;
   if (!(c.flags & kActive)) return;
   if (z && !(c.flags & kUseT)) return;
-  colpass_item<N, TT, E, MODE, true>(a, f, band, z, a.errp, lds);
+  if constexpr (MODE == 4) {
+    colpass_ipsf_a<N, TT, E>(a, f, band, lds);
+  } else if constexpr (MODE == 5) {
+    colpass_ipsf_b<N, TT, E>(a, f, band, a.errp, lds);
+  } else {
+    colpass_item<N, TT, E, MODE, true>(a, f, band, z, a.errp, lds);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -380,6 +470,43 @@ __global__ void __launch_bounds__((N / E) * TT) k_rowpass(PassArgs a) {
   const Coef c = a.tab ? a.tab[f] : a.c;
   if (!(c.flags & kActive)) return;
   rowpass_item<N, TT, E, USE_T, EMIT_B, true>(a, f, blockIdx.x, c, lds);
+}
+
+// ---------------------------------------------------------------------------
+// Row pass of the partial-coherence convolution: out = rowIFFT(ipsf *
+// rowFFT(in)), unscaled (the 1/n^2 of the 2-D inverse is applied in the
+// second column pass).  ipsf is tiled and shared by the frames.
+// grid (n/TT bands, frames)
+// ---------------------------------------------------------------------------
+template <int N, int TT, int E>
+__global__ void __launch_bounds__((N / E) * TT)
+    k_rowconv(const c32* in, c32* out, const c32* __restrict__ ipsf, const c32* __restrict__ tw,
+              const int* fmap) {
+  using L = LdsPlan<N, TT>;
+  constexpr int NT = (N / E) * TT;
+  __shared__ c32 lds[L::TOTAL];
+  const int band = blockIdx.x, f = fmap ? fmap[blockIdx.y] : blockIdx.y;
+  const size_t fo = (size_t)f * N * N;
+  const int tid = threadIdx.x, b = tid % TT, t = tid / TT;
+  TwStage<N, TT, NT> tws;
+  tws.fetch(tw);
+  c32 x[E], w[E];
+#pragma unroll
+  for (int i = 0; i < E; ++i) x[i] = gld(in + fo, row_pos<N, TT, E>(band, b, t, i));
+#pragma unroll
+  for (int i = 0; i < E; ++i) w[i] = gld(ipsf, row_pos<N, TT, E>(band, b, t, i));
+  c32* twl = lds + L::NBUF * L::BUF;
+  tws.put(twl);
+  const auto tv = twiddle_view<N, TT>(twl);
+  fft_regs<N, E, TT, false, true>(x, t, b, lds, tv);
+#pragma unroll
+  for (int i = 0; i < E; ++i) x[i] = mul(x[i], w[i]);
+  fft_regs<N, E, TT, true, false>(x, t, b, lds, tv);
+#pragma unroll
+  for (int i = 0; i < E; ++i) {
+    if constexpr (dual_layout<N>()) gst(out + fo, cbrow_pos<N, TT, E>(band, b, t, i), x[i]);
+    else gst(out + fo, row_pos<N, TT, E>(band, b, t, i), x[i]);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1023,6 +1150,8 @@ static hipError_t colpass_nt(int mode, const PassArgs& a, int frames, int zdim,
     case 0: launch_pass(k_colpass<N, TT, E, 0>, grid, block, st, a); break;
     case 2: launch_pass(k_colpass<N, TT, E, 2>, grid, block, st, a); break;
     case 3: launch_pass(k_colpass<N, TT, E, 3>, grid, block, st, a); break;
+    case 4: launch_pass(k_colpass<N, TT, E, 4>, grid, block, st, a); break;
+    case 5: launch_pass(k_colpass<N, TT, E, 5>, grid, block, st, a); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
@@ -1066,6 +1195,14 @@ static hipError_t persist_grid_nt(int* out) {
     return e;
   *out = occ * cus;
   return hipSuccess;
+}
+
+template <int N, int TT, int E>
+static hipError_t rowconv_nt(const c32* in, c32* out, const c32* ipsf, const c32* tw,
+                             const int* fmap, int frames, hipStream_t st) {
+  hipLaunchKernelGGL((k_rowconv<N, TT, E>), dim3(N / TT, frames), dim3((N / E) * TT), 0, st, in,
+                     out, ipsf, tw, fmap);
+  return hipGetLastError();
 }
 
 template <int N, int TT, int E>
@@ -1136,7 +1273,9 @@ constexpr bool shape_ok() {
   hipError_t persist_##n(int tile, int ept, const PassArgs& a, const unsigned char* stage,  \
                          const PStep* steps, int nsteps, double* errp0, long long errp_step,\
                          unsigned* bar, int* flag, int grid, hipStream_t st);               \
-  hipError_t persist_grid_##n(int tile, int ept, int* out);
+  hipError_t persist_grid_##n(int tile, int ept, int* out);                                 \
+  hipError_t rowconv_##n(int tile, int ept, const c32* in, c32* out, const c32* ipsf,       \
+                         const c32* tw, const int* fmap, int frames, hipStream_t st);
 
 // a shape that is not built compiles to an error return, not a kernel
 #define SPK_CALL_COL(TT_, E_)                                                   \
@@ -1155,6 +1294,11 @@ constexpr bool shape_ok() {
        ? persist_grid_nt<SPK_N, (shape_ok<SPK_N, TT_, E_>() ? TT_ : 4),           \
                          (shape_ok<SPK_N, TT_, E_>() ? E_ : 16)>(out)             \
        : hipErrorInvalidValue)
+#define SPK_CALL_CONV(TT_, E_)                                                  \
+  (shape_ok<SPK_N, TT_, E_>() ? rowconv_nt<SPK_N, (shape_ok<SPK_N, TT_, E_>() ? TT_ : 4), \
+                                           (shape_ok<SPK_N, TT_, E_>() ? E_ : 16)>(     \
+                                    in, out, ipsf, tw, fmap, frames, st)                \
+                              : hipErrorInvalidValue)
 #define SPK_CALL_ROW(TT_, E_)                                                   \
   (shape_ok<SPK_N, TT_, E_>() ? rowpass_nt<SPK_N, (shape_ok<SPK_N, TT_, E_>() ? TT_ : 4), \
                                            (shape_ok<SPK_N, TT_, E_>() ? E_ : 16)>(     \
@@ -1205,6 +1349,11 @@ constexpr bool shape_ok() {
   hipError_t persist_grid_##n(int tile, int ept, int* out) {                                \
     constexpr int SPK_N = n;                                                                \
     SPK_SHAPE_SWITCH(n, SPK_CALL_PGRID)                                                     \
+  }                                                                                         \
+  hipError_t rowconv_##n(int tile, int ept, const c32* in, c32* out, const c32* ipsf,       \
+                         const c32* tw, const int* fmap, int frames, hipStream_t st) {      \
+    constexpr int SPK_N = n;                                                                \
+    SPK_SHAPE_SWITCH(n, SPK_CALL_CONV)                                                      \
   }
 
 SPK_SIZES(SPK_SIZED_DECL)
@@ -1289,6 +1438,11 @@ hipError_t launch_persist(int n, int tile, int ept, const PassArgs& a, const uns
 
 hipError_t persist_grid(int n, int tile, int ept, int* out) {
   SPK_SIZE_SWITCH(persist_grid, tile, ept, out)
+}
+
+hipError_t launch_rowconv(int n, int tile, int ept, const c32* in, c32* out, const c32* ipsf,
+                          const c32* tw, const int* fmap, int frames, hipStream_t st) {
+  SPK_SIZE_SWITCH(rowconv, tile, ept, in, out, ipsf, tw, fmap, frames, st)
 }
 
 hipError_t launch_shrinkwrap(int n, int tile, const c32* psi, float* sup, long long sstride,

@@ -262,6 +262,14 @@ class Plan:
         check(self.lib.spk_time_passes(self.handle, which, reps, arr, byref(ms)))
         return ms.value
 
+    def set_ipsf(self, ipsf: torch.Tensor | None) -> None:
+        """Partial-coherence ipsf (n x n complex64, corner-centred), or None."""
+        if ipsf is None:
+            check(self.lib.spk_set_ipsf(self.handle, None))
+            return
+        _expect(ipsf, torch.complex64, (self.n, self.n), self.ctx.device, "ipsf")
+        check(self.lib.spk_set_ipsf(self.handle, _ptr(ipsf)))
+
     def check(self) -> None:
         """Wait for queued work; raise if a persistent block's barrier gave up."""
         check(self.lib.spk_plan_check(self.handle))

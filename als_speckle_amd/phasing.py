@@ -171,9 +171,17 @@ class phasing:
         assert isinstance(modulus, types), "modulus must be ndarray if supplied"
         assert isinstance(support, types), "support must be ndarray if supplied"
         assert isinstance(ipsf, types), "psf must be ndarray if supplied"
-        if ipsf is not None or spectrum is not None:
-            raise NotImplementedError(
-                "partial-coherence (ipsf / spectrum) loading is not built yet")
+        if spectrum is not None:
+            raise NotImplementedError("the spectral (longitudinal) rescale is not built yet")
+        if ipsf is not None:
+            # phasing.py:414-425: transverse coherence, same size as the modulus
+            assert ipsf.ndim == 2
+            assert ipsf.shape[0] == ipsf.shape[1]
+            ipsf = ipsf.astype(np.complex64)
+            if self.ipsf_state == 2:
+                assert ipsf.shape == self._ipsf_host.shape
+            self._ipsf_host = ipsf
+            self.ipsf_state = 1
 
         if modulus is not None:
             assert modulus.ndim == 2
@@ -189,6 +197,16 @@ class phasing:
             self._modulus_dev = to_device(modulus, self.ctx.device, torch.float32)
             self._plan.set_modulus(self._modulus_dev, -1)
             self.modulus_state = 2
+
+        if self.modulus_state == 2 and self.ipsf_state > 0:
+            # phasing.py:458-460
+            assert self._ipsf_host.shape == self.shape
+            if self.ipsf_state == 1:
+                self._ipsf_dev = to_device(self._ipsf_host, self.ctx.device, torch.complex64)
+                self._plan.set_ipsf(self._ipsf_dev)
+                dev = self._ipsf_dev
+                self.ipsf = DeviceArray("ipsf", self.shape, np.complex64, lambda: dev)
+            self.ipsf_state = 2
 
         if support is not None:
             assert support.ndim == 2, "support is %s-dim" % support.ndim

@@ -127,6 +127,14 @@ int spk_set_psi(spk_plan* plan, int frame, const float* guess_c64,
 /* phasing.get(psi_in) (gpu.py:389-397) */
 int spk_get_psi(spk_plan* plan, int frame, float* psi_c64);
 
+/* Transverse partial coherence (phasing.py:414-425 load(ipsf=), divisor at
+ * phasing.py:1175-1183): ipsf is an n x n c64 array (row-major, the transform
+ * of the point-spread function, corner-centred), shared by the plan's
+ * frames.  While set, every iteration's modulus replacement divides by
+ * div = sqrt(|ifft2(fft2(|Psi|^2) * ipsf)|) instead of |Psi| and the error
+ * trace compares div with m.  NULL switches it off.  Not combined with DM. */
+int spk_set_ipsf(spk_plan* plan, const float* ipsf_c64);
+
 /* -------------------------------------------------------------------------
  * The loop.  Replaces phasing.iterate's per-iteration work
  * (speckle/phasing.py:260-268 -> _iteration, 1088-1353: pyfft forward,

@@ -271,17 +271,57 @@ def dm_modified(psi_in, s, beta):
     return s * psi_in - gamma_m * (1 - s) * psi_in
 
 
-def iteration(psi_in, m, s, alg, beta=0.8, semantics="numpy", zero_guard=True):
+def coherence_divisor(Psi, ipsf):
+    """Partially coherent modulus, ``_build_divisor`` with an ipsf, CPU branch
+    without numexpr (``phasing.py:1175-1183``): div = |Psi|**2, forward
+    transform, times ipsf, inverse transform, sqrt(abs(.)).  ``ipsf`` is the
+    transform of the point-spread function, corner-centred, as loaded
+    (``phasing.py:414-425``)."""
+    return np.sqrt(np.abs(ifft2(fft2(np.abs(Psi) ** 2) * ipsf)))
+
+
+def fourier_constraint_ipsf(psi, m, ipsf, zero_guard=True):
+    """P_M with the partial-coherence divisor: ``m * Psi / div``
+    (``phasing.py:1217``, ``fd`` = ``coherence_divisor``).  Returns
+    (psi_out, Psi, div); 0 where div == 0 with ``zero_guard``."""
+    Psi = fft2(psi)
+    div = coherence_divisor(Psi, ipsf)
+    m = np.asarray(m, np.float64)
+    if zero_guard:
+        with np.errstate(divide="ignore", invalid="ignore"):
+            r = np.where(div > 0, m / np.where(div > 0, div, 1.0), 0.0)
+        Pp = Psi * r
+    else:
+        Pp = m * Psi / div
+    return ifft2(Pp), Psi, div
+
+
+def modulus_error_div(div, m):
+    """Error of the modelled modulus: sqrt(sum((div - m)^2) / sum(m^2)); with
+    a coherent divisor (div = |Psi|) it is ``modulus_error``."""
+    m = np.asarray(m, np.float64)
+    return float(np.sqrt(np.sum((div - m) ** 2) / np.sum(m ** 2)))
+
+
+def iteration(psi_in, m, s, alg, beta=0.8, semantics="numpy", zero_guard=True, ipsf=None):
     """One iteration of ``_iteration`` (``phasing.py:1088-1353``).
 
     Returns (psi_new, psi_out, err) where err is the modulus error of
-    Psi = fft2(psi_in) before replacement.
+    Psi = fft2(psi_in) before replacement (of the partially coherent divisor
+    when an ipsf is given).
     """
-    psi_out, Psi = fourier_constraint(psi_in, m, zero_guard)
-    err = modulus_error(Psi, m)
+    if ipsf is None:
+        psi_out, Psi = fourier_constraint(psi_in, m, zero_guard)
+        err = modulus_error(Psi, m)
+    else:
+        psi_out, Psi, div = fourier_constraint_ipsf(psi_in, m, ipsf, zero_guard)
+        err = modulus_error_div(div, m)
     t = None
     if alg == "dm":
-        t, _ = fourier_constraint(dm_modified(psi_in, s, beta), m, zero_guard)
+        if ipsf is None:
+            t, _ = fourier_constraint(dm_modified(psi_in, s, beta), m, zero_guard)
+        else:
+            t, _, _ = fourier_constraint_ipsf(dm_modified(psi_in, s, beta), m, ipsf, zero_guard)
     return update(alg, psi_in, psi_out, s, beta, semantics, t), psi_out, err
 
 

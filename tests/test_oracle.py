@@ -157,6 +157,37 @@ def test_dm_solution_doubles_inside_support():
     assert np.allclose(new, 2 * obj, atol=1e-12)
 
 
+def test_coherence_divisor_closed_forms():
+    rng = np.random.default_rng(21)
+    n = 16
+    Psi = crand((n, n), rng)
+    # ipsf = 1 everywhere: a delta point-spread function, the coherent divisor
+    assert np.allclose(O.coherence_divisor(Psi, np.ones((n, n))), np.abs(Psi), rtol=1e-12)
+    # a constant ipsf c scales the intensity: div = sqrt(c) |Psi|
+    assert np.allclose(O.coherence_divisor(Psi, 4.0 * np.ones((n, n))), 2 * np.abs(Psi))
+    # general ipsf: circular convolution of |Psi|^2 with psf = ifft2(ipsf),
+    # by brute force (a dropped square, sqrt or abs fails one of these)
+    psf = np.exp(-((np.arange(n)[:, None] - 3) ** 2 + (np.arange(n)[None, :] - 5) ** 2) / 4.0)
+    ipsf = np.fft.fft2(psf)
+    I = np.abs(Psi) ** 2
+    conv = np.zeros((n, n))
+    for y in range(n):
+        for x in range(n):
+            conv[y, x] = sum(psf[a, b] * I[(y - a) % n, (x - b) % n]
+                             for a in range(n) for b in range(n))
+    assert np.allclose(O.coherence_divisor(Psi, ipsf), np.sqrt(np.abs(conv)), rtol=1e-10)
+    # the P_M with a coherent ipsf is the coherent P_M
+    psi = crand((n, n), rng)
+    m = np.abs(crand((n, n), rng))
+    a, _, div = O.fourier_constraint_ipsf(psi, m, np.ones((n, n)))
+    b, _ = O.fourier_constraint(psi, m)
+    assert np.allclose(a, b) and np.allclose(div, np.abs(np.fft.fft2(psi)))
+    # |P_M| = m * |Psi| / div
+    _, Psi2, div2 = O.fourier_constraint_ipsf(psi, m, ipsf)
+    assert np.allclose(np.abs(np.fft.fft2(O.fourier_constraint_ipsf(psi, m, ipsf)[0])),
+                       m * np.abs(Psi2) / div2)
+
+
 def test_modulus_error_closed_form():
     Psi = np.array([[3 + 4j, 0], [1, -2j]])
     m = np.array([[4.0, 1.0], [1.0, 1.0]])
