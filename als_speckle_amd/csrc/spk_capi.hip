@@ -127,6 +127,13 @@ struct spk_plan {
   c32* dbuf = nullptr;
   c32* dbufc = nullptr;
   bool ipsf_on = false;
+  // spectral rescale: factors and weights (device), per-frame intensity maps
+  float* spec = nullptr;
+  int nspec = 0;
+  int spec_k = 0;
+  float* imap = nullptr;
+  float* dmap = nullptr;
+  bool spec_on = false;
   float phib_g = 0.0f;
   long long nn() const { return (long long)n * n; }
   // Phi kept in two layouts (see spk_fft_core.hpp dual_layout): n >= 4096
@@ -222,6 +229,8 @@ PassArgs base_args(spk_plan* p) {
   a.inv_nn = (float)(1.0 / ((double)p->n * p->n));
   a.dbuf = p->dbuf;
   a.dbufc = p->dual() ? p->dbufc : p->dbuf;
+  a.imap = p->imap;
+  a.dmap = p->dmap;
   return a;
 }
 
@@ -471,6 +480,7 @@ int spk_plan_destroy(spk_plan* p) {
   dfree(p->mod); dfree(p->sup);
   dfree(p->tw); dfree(p->summ2); dfree(p->sumpart); dfree(p->errp); dfree(p->pbar);
   dfree(p->ipsf); dfree(p->dbuf); dfree(p->dbufc);
+  dfree(p->spec); dfree(p->imap); dfree(p->dmap);
   dfree(p->sw_tmp); dfree(p->sw_blur); dfree(p->sw_maxp); dfree(p->fftbuf);
   dfree(p->stage_d);
   prof_clear(p, 0);
@@ -590,6 +600,41 @@ int spk_set_ipsf(spk_plan* p, const float* ipsf) {
   return SPK_OK;
 }
 
+int spk_set_spectrum(spk_plan* p, const float* rescales, const float* weights, int K) {
+  SPK_REQUIRE(p, SPK_EINVAL, "null plan");
+  int rc = set_device(p->ctx);
+  if (rc) return rc;
+  if (!rescales || K == 0) {
+    p->spec_on = false;
+    return SPK_OK;
+  }
+  SPK_REQUIRE(weights && K > 0 && K <= 4096, SPK_EINVAL, "spectrum needs 1..4096 samples");
+  for (int k = 0; k < K; ++k)
+    SPK_REQUIRE(std::isfinite(rescales[k]) && rescales[k] != 0.0f && std::isfinite(weights[k]),
+                SPK_EINVAL, "spectrum rescale factors must be finite and non-zero");
+  const size_t nn = (size_t)p->nn();
+  if (!p->imap) {
+    if ((rc = dalloc(&p->imap, nn * p->batch)) || (rc = dalloc(&p->dmap, nn * p->batch)))
+      return rc;
+  }
+  if (K > p->nspec) {
+    SPK_HIP(hipStreamSynchronize(p->ctx->stream));
+    dfree(p->spec);
+    if ((rc = dalloc(&p->spec, (size_t)2 * K))) return rc;
+  }
+  std::vector<float> tab(2 * (size_t)K);
+  std::memcpy(tab.data(), rescales, K * sizeof(float));
+  std::memcpy(tab.data() + K, weights, K * sizeof(float));
+  // synchronous: the host table goes out of scope at return
+  SPK_HIP(hipMemcpyAsync(p->spec, tab.data(), tab.size() * sizeof(float), hipMemcpyHostToDevice,
+                         p->ctx->stream));
+  SPK_HIP(hipStreamSynchronize(p->ctx->stream));
+  p->nspec = std::max(p->nspec, K);
+  p->spec_k = K;
+  p->spec_on = true;
+  return SPK_OK;
+}
+
 int spk_iterate(spk_plan* p, const spk_step* steps, int nsteps, int per_frame,
                 double* err_trace) {
   SPK_REQUIRE(p, SPK_EINVAL, "null plan");
@@ -631,8 +676,8 @@ int spk_iterate(spk_plan* p, const spk_step* steps, int nsteps, int per_frame,
   }
   bool dm_any = false;
   for (int k = 0; k < nsteps; ++k) dm_any |= any_dm[k] != 0;
-  SPK_REQUIRE(!(dm_any && p->ipsf_on), SPK_EUNSUP,
-              "difference map with a partial-coherence ipsf is not built");
+  SPK_REQUIRE(!(dm_any && (p->ipsf_on || p->spec_on)), SPK_EUNSUP,
+              "difference map with a partial-coherence ipsf or spectrum is not built");
   if (dm_any && (rc = ensure_phib(p))) return rc;
   bool sw_any = false;
   for (int k = 0; k < nsteps; ++k) sw_any |= any_sw[k] != 0;
@@ -711,7 +756,7 @@ int spk_iterate(spk_plan* p, const spk_step* steps, int nsteps, int per_frame,
     for (int f = 0; f < W; ++f) emits[k] |= (coefs[(size_t)k * W + f].flags & kEmitB) != 0;
   std::vector<int> block_end(nsteps, 0);
   std::vector<size_t> block_off(nsteps, 0);
-  if (p->persist && !p->ipsf_on) {
+  if (p->persist && !p->ipsf_on && !p->spec_on) {
     auto fits = [&](int j) { return !any_dm[j] && !emits[j]; };
     for (int k = 0; k < nsteps;) {
       if (!fits(k)) { ++k; continue; }
@@ -800,15 +845,24 @@ int spk_iterate(spk_plan* p, const spk_step* steps, int nsteps, int per_frame,
                             per_frame ? dm_n[k] : B, st));
     }
     a.errp = err_trace ? p->errp + (size_t)k * B * p->bands : nullptr;
-    if (p->ipsf_on) {
-      // partially coherent projection: column pass A (|Psi|^2 -> column
-      // FFT), the row convolution with the ipsf, column pass B (divisor,
-      // replacement, error, column IFFT)
-      SPK_HIP(launch_colpass(p->n, p->tile, p->ept, 4, a, nact, 1, st));
-      SPK_HIP(launch_rowconv(p->n, p->tile, p->ept, p->dbuf, p->dual() ? p->dbufc : p->dbuf,
-                             p->ipsf, p->tw, a.fmap, nact, st));
+    if (p->ipsf_on || p->spec_on) {
+      // partially coherent projection.  ipsf: column pass A (|Psi|^2 ->
+      // column FFT) and the row convolution with the ipsf.  spectrum: the
+      // squared divisor as a map (coherent: MODE 6; after the ipsf: MODE 7),
+      // its spectral blur.  Then column pass B (divisor, replacement,
+      // error, column IFFT).
+      if (p->ipsf_on) {
+        SPK_HIP(launch_colpass(p->n, p->tile, p->ept, 4, a, nact, 1, st));
+        SPK_HIP(launch_rowconv(p->n, p->tile, p->ept, p->dbuf, p->dual() ? p->dbufc : p->dbuf,
+                               p->ipsf, p->tw, a.fmap, nact, st));
+      }
+      if (p->spec_on) {
+        SPK_HIP(launch_colpass(p->n, p->tile, p->ept, p->ipsf_on ? 7 : 6, a, nact, 1, st));
+        SPK_HIP(launch_rescale(p->n, p->tile, p->imap, p->dmap, p->spec, p->spec_k, a.fmap,
+                               nact, st));
+      }
       if ((rc = prof_arm(p, 0))) return rc;
-      SPK_HIP(launch_colpass(p->n, p->tile, p->ept, 5, a, nact, 1, st));
+      SPK_HIP(launch_colpass(p->n, p->tile, p->ept, p->spec_on ? 8 : 5, a, nact, 1, st));
     } else {
       if ((rc = prof_arm(p, 0))) return rc;
       SPK_HIP(launch_colpass(p->n, p->tile, p->ept, 0, a, nact, any_dm[k] ? 2 : 1, st));

@@ -48,6 +48,10 @@ struct PassArgs {
   // |Psi|^2, tiled (dbuf) and, in the dual layout, column-band-major (dbufc)
   c32* dbuf;
   c32* dbufc;
+  // spectral rescale: per-frame intensity map and its blurred map, float,
+  // column-band-major like the modulus
+  float* imap;
+  float* dmap;
 };
 
 // One step of a persistent iteration block (k_persist).  Offsets are bytes
@@ -96,6 +100,11 @@ hipError_t launch_persist(int n, int tile, int ept, const PassArgs& a, const uns
 // out is tiled (in place allowed) or, in the dual layout, column-band-major
 hipError_t launch_rowconv(int n, int tile, int ept, const c32* in, c32* out, const c32* ipsf,
                           const c32* tw, const int* fmap, int frames, hipStream_t st);
+// spectral (longitudinal) blur of the intensity maps (phasing_radial_rescale.cl):
+// dmap = sum_k w_k bilinear(imap at coordinates / r_k); spec = K rescale
+// factors then K weights; maps column-band-major with the plan's tile
+hipError_t launch_rescale(int n, int tile, const float* imap, float* dmap, const float* spec,
+                          int K, const int* fmap, int frames, hipStream_t st);
 // workgroups of the persistent kernel that fit on the current device at once
 hipError_t persist_grid(int n, int tile, int ept, int* out);
 // mode 0: out = rowFFT(in); 1: out = rowIFFT(in) (unscaled);

@@ -192,7 +192,10 @@ __device__ __forceinline__ int xcd_band(int bid) {
 // Column pass.  MODE 0: FFT -> projection (+ error) -> IFFT
 //               MODE 2: forward FFT only; MODE 3: inverse FFT only
 //               MODE 4, 5: the two column passes of the partially coherent
-//               projection (see colpass_ipsf_a / colpass_ipsf_b)
+//               projection (colpass_ipsf_a, colpass_div_b)
+//               MODE 6, 7: intensity maps for the spectral blur; MODE 8: the
+//               projection with the blurred divisor (colpass_intensity,
+//               colpass_div_b)
 // One item = one band of TT columns of frame f; z = 1 processes phib (DM)
 // without error.  OWN_TW: the item stages the twiddle table into LDS itself
 // (a one-item kernel); otherwise the table is already there (persistent
@@ -296,28 +299,66 @@ __device__ __forceinline__ void colpass_ipsf_a(const PassArgs& a, int f, int ban
   for (int i = 0; i < E; ++i) gst(a.dbuf + fo, col_pos<N, TT, E>(band, b, t, i), x[i]);
 }
 
-// Partial coherence, column pass B: div = sqrt(|colIFFT(D)| / n^2) with D the
-// row pass's rowIFFT(ipsf * rowFFT(Q)), i.e. div = sqrt(|ifft2(fft2(|Psi|^2)
-// * ipsf)|) (phasing.py:1181-1183); then Psi' = m * Psi / div (0 where
-// div = 0; phasing.py:1217), the FP64 error partial of (div - m)^2, and
-// Phi = colIFFT(Psi').
-template <int N, int TT, int E>
-__device__ __forceinline__ void colpass_ipsf_b(const PassArgs& a, int f, int band,
-                                               double* errp, c32* lds) {
+// Intensity map for the spectral blur (MODE 6, 7): the squared divisor,
+// column-band-major in imap.  MODE 6: |Psi|^2 with Psi = colFFT(Phi) (the
+// coherent divisor squared); MODE 7: |colIFFT(D)| / n^2 with D from the ipsf
+// row convolution (the partially coherent divisor squared).
+template <int N, int TT, int E, int MODE>
+__device__ __forceinline__ void colpass_intensity(const PassArgs& a, int f, int band, c32* lds) {
+  using L = LdsPlan<N, TT>;
+  constexpr int NT = (N / E) * TT;
+  constexpr bool DUAL = dual_layout<N>();
+  const size_t fo = (size_t)f * N * N;
+  const c32* src = MODE == 6 ? (DUAL ? a.phic : a.phi) + fo : (DUAL ? a.dbufc : a.dbuf) + fo;
+  const int tid = threadIdx.x, b = tid % TT, t = tid / TT;
+  TwStage<N, TT, NT> tws;
+  tws.fetch(a.tw);
+  c32 x[E];
+#pragma unroll
+  for (int i = 0; i < E; ++i)
+    x[i] = gld(src, DUAL ? mcol_pos<N, TT, E>(band, b, t, i) : col_pos<N, TT, E>(band, b, t, i));
+  c32* twl = lds + L::NBUF * L::BUF;
+  tws.put(twl);
+  fft_regs<N, E, TT, MODE == 7, true>(x, t, b, lds, twiddle_view<N, TT>(twl));
+  float* dst = a.imap + fo;
+#pragma unroll
+  for (int i = 0; i < E; ++i) {
+    const float p2 = fmaf(x[i].x, x[i].x, x[i].y * x[i].y);
+    gst(dst, mcol_pos<N, TT, E>(band, b, t, i), MODE == 6 ? p2 : sqrtf(p2) * a.inv_nn);
+  }
+}
+
+// Column pass B of a partially coherent projection: the divisor, then
+// Psi' = m * Psi / div (0 where div = 0; phasing.py:1217), the FP64 error
+// partial of (div - m)^2, and Phi = colIFFT(Psi').
+//   SRC 0 (ipsf, MODE 5): div = sqrt(|colIFFT(D)| / n^2) with D the row
+//     pass's rowIFFT(ipsf * rowFFT(colFFT(|Psi|^2))), i.e.
+//     div = sqrt(|ifft2(fft2(|Psi|^2) * ipsf)|) (phasing.py:1175-1183);
+//   SRC 1 (spectrum, MODE 8): div = sqrt(dmap), the spectrally blurred
+//     squared divisor (phasing.py:1140-1156).
+template <int N, int TT, int E, int SRC>
+__device__ __forceinline__ void colpass_div_b(const PassArgs& a, int f, int band,
+                                              double* errp, c32* lds) {
   using L = LdsPlan<N, TT>;
   constexpr int NB = N / TT;
   constexpr int NT = (N / E) * TT;
   constexpr bool DUAL = dual_layout<N>();
   const size_t fo = (size_t)f * N * N;
   const c32* src = (DUAL ? a.phic : a.phi) + fo;
-  const c32* dsrc = (DUAL ? a.dbufc : a.dbuf) + fo;
   const int tid = threadIdx.x, b = tid % TT, t = tid / TT;
   TwStage<N, TT, NT> tws;
   tws.fetch(a.tw);
-  c32 d[E];
+  c32 d[SRC == 0 ? E : 1];
+  float div[E];
+  if constexpr (SRC == 0) {
+    const c32* dsrc = (DUAL ? a.dbufc : a.dbuf) + fo;
 #pragma unroll
-  for (int i = 0; i < E; ++i)
-    d[i] = gld(dsrc, DUAL ? mcol_pos<N, TT, E>(band, b, t, i) : col_pos<N, TT, E>(band, b, t, i));
+    for (int i = 0; i < E; ++i)
+      d[i] = gld(dsrc, DUAL ? mcol_pos<N, TT, E>(band, b, t, i) : col_pos<N, TT, E>(band, b, t, i));
+  } else {
+#pragma unroll
+    for (int i = 0; i < E; ++i) div[i] = gld(a.dmap + fo, mcol_pos<N, TT, E>(band, b, t, i));
+  }
   c32 x[E];
 #pragma unroll
   for (int i = 0; i < E; ++i)
@@ -329,12 +370,16 @@ __device__ __forceinline__ void colpass_ipsf_b(const PassArgs& a, int f, int ban
   c32* twl = lds + L::NBUF * L::BUF;
   tws.put(twl);
   const auto tw = twiddle_view<N, TT>(twl);
-  fft_regs<N, E, TT, true, true>(d, t, b, lds, tw);
-  float div[E];
+  if constexpr (SRC == 0) {
+    fft_regs<N, E, TT, true, true>(d, t, b, lds, tw);
 #pragma unroll
-  for (int i = 0; i < E; ++i)
-    div[i] = sqrtf(sqrtf(fmaf(d[i].x, d[i].x, d[i].y * d[i].y)) * a.inv_nn);
-  fft_regs<N, E, TT, false, false>(x, t, b, lds, tw);
+    for (int i = 0; i < E; ++i)
+      div[i] = sqrtf(sqrtf(fmaf(d[i].x, d[i].x, d[i].y * d[i].y)) * a.inv_nn);
+  } else {
+#pragma unroll
+    for (int i = 0; i < E; ++i) div[i] = sqrtf(div[i]);
+  }
+  fft_regs<N, E, TT, false, SRC != 0>(x, t, b, lds, tw);
   double err = 0.0;
 #pragma unroll
   for (int i = 0; i < E; ++i) {
@@ -364,7 +409,11 @@ __global__ void __launch_bounds__((N / E) * TT) k_colpass(PassArgs a) {
   if constexpr (MODE == 4) {
     colpass_ipsf_a<N, TT, E>(a, f, band, lds);
   } else if constexpr (MODE == 5) {
-    colpass_ipsf_b<N, TT, E>(a, f, band, a.errp, lds);
+    colpass_div_b<N, TT, E, 0>(a, f, band, a.errp, lds);
+  } else if constexpr (MODE == 6 || MODE == 7) {
+    colpass_intensity<N, TT, E, MODE>(a, f, band, lds);
+  } else if constexpr (MODE == 8) {
+    colpass_div_b<N, TT, E, 1>(a, f, band, a.errp, lds);
   } else {
     colpass_item<N, TT, E, MODE, true>(a, f, band, z, a.errp, lds);
   }
@@ -1086,6 +1135,59 @@ static __global__ void __launch_bounds__(256) k_rftf_stack(const c32* F, const f
 }
 
 // ---------------------------------------------------------------------------
+// Spectral (longitudinal) blur, phasing_radial_rescale.cl restated: output
+// pixel (row j, column i) has signed coordinates x = i (i < n/2) or i - n,
+// y likewise; for each sample k the intensity is bilinearly interpolated at
+// (x / r_k, y / r_k) (FP32, as the reference) and summed with weight w_k.
+// A node with floor coordinate <= -n/2 or >= n/2 (and its +1 neighbour
+// likewise) has weight 0 and is not read; kept nodes wrap by +n.  Maps are
+// column-band-major with tile tt.  spec: K factors, then K weights.
+// grid (ceil(n*n/256), frames)
+// ---------------------------------------------------------------------------
+static __device__ __forceinline__ float rescale_node(const float* img, int n, int tt, int yi,
+                                                     int xi) {
+  return img[(size_t)(xi / tt) * n * tt + (size_t)yi * tt + (xi % tt)];
+}
+
+static __global__ void __launch_bounds__(256) k_rescale(const float* imap, float* dmap,
+                                                        const float* spec, int K, int n, int tt,
+                                                        const int* fmap) {
+  const int f = fmap ? fmap[blockIdx.y] : blockIdx.y;
+  const long long nn = (long long)n * n;
+  const long long p = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (p >= nn) return;
+  const float* img = imap + (size_t)f * nn;
+  // column-band-major position p -> (row j, column i)
+  const int band = (int)(p / ((long long)n * tt)), rem = (int)(p % ((long long)n * tt));
+  const int j = rem / tt, i = band * tt + rem % tt;
+  const int h = n / 2;
+  const float x = (float)(i > h - 1 ? i - n : i);
+  const float y = (float)(j > h - 1 ? j - n : j);
+  float acc = 0.0f;
+  for (int k = 0; k < K; ++k) {
+    const float rv = spec[k], sw = spec[K + k];
+    const float rx = x / rv, ry = y / rv;
+    const int x1 = (int)floorf(rx), y1 = (int)floorf(ry);
+    const int x2 = x1 + 1, y2 = y1 + 1;
+    const bool kx1 = x1 > -h && x1 < h, kx2 = x2 > -h && x2 < h;
+    const bool ky1 = y1 > -h && y1 < h, ky2 = y2 > -h && y2 < h;
+    const float wx1 = kx1 ? 1.0f - fabsf((float)x1 - rx) : 0.0f;
+    const float wx2 = kx2 ? 1.0f - fabsf((float)x2 - rx) : 0.0f;
+    const float wy1 = ky1 ? 1.0f - fabsf((float)y1 - ry) : 0.0f;
+    const float wy2 = ky2 ? 1.0f - fabsf((float)y2 - ry) : 0.0f;
+    const int ix1 = x1 < 0 ? x1 + n : x1, ix2 = x2 < 0 ? x2 + n : x2;
+    const int iy1 = y1 < 0 ? y1 + n : y1, iy2 = y2 < 0 ? y2 + n : y2;
+    const float v11 = (kx1 && ky1) ? rescale_node(img, n, tt, iy1, ix1) : 0.0f;
+    const float v12 = (kx1 && ky2) ? rescale_node(img, n, tt, iy2, ix1) : 0.0f;
+    const float v21 = (kx2 && ky1) ? rescale_node(img, n, tt, iy1, ix2) : 0.0f;
+    const float v22 = (kx2 && ky2) ? rescale_node(img, n, tt, iy2, ix2) : 0.0f;
+    const float iv = v11 * wx1 * wy1 + v12 * wx1 * wy2 + v21 * wx2 * wy1 + v22 * wx2 * wy2;
+    acc = fmaf(iv, sw, acc);
+  }
+  dmap[(size_t)f * nn + p] = acc;
+}
+
+// ---------------------------------------------------------------------------
 // mean over a stack of frames (the trial average, phasing.py:960-961):
 // out[i] = (1/frames) sum_f data[f][i], summed in FP64 in frame order
 // ---------------------------------------------------------------------------
@@ -1152,6 +1254,9 @@ static hipError_t colpass_nt(int mode, const PassArgs& a, int frames, int zdim,
     case 3: launch_pass(k_colpass<N, TT, E, 3>, grid, block, st, a); break;
     case 4: launch_pass(k_colpass<N, TT, E, 4>, grid, block, st, a); break;
     case 5: launch_pass(k_colpass<N, TT, E, 5>, grid, block, st, a); break;
+    case 6: launch_pass(k_colpass<N, TT, E, 6>, grid, block, st, a); break;
+    case 7: launch_pass(k_colpass<N, TT, E, 7>, grid, block, st, a); break;
+    case 8: launch_pass(k_colpass<N, TT, E, 8>, grid, block, st, a); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
@@ -1604,6 +1709,14 @@ hipError_t launch_rftf(const c32* F, const float* goal, int n, double* partial, 
   hipLaunchKernelGGL(k_rftf_diff, dim3(nblk), dim3(256), 0, st, F, goal, n, partial);
   hipLaunchKernelGGL(k_rftf_out, dim3(grid_for((long long)n * n)), dim3(256), 0, st, F, goal, n,
                      partial, nblk, scale, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_rescale(int n, int tile, const float* imap, float* dmap, const float* spec,
+                          int K, const int* fmap, int frames, hipStream_t st) {
+  const long long nn = (long long)n * n;
+  hipLaunchKernelGGL(k_rescale, dim3((unsigned)((nn + 255) / 256), frames), dim3(256), 0, st,
+                     imap, dmap, spec, K, n, tile, fmap);
   return hipGetLastError();
 }
 

@@ -270,6 +270,16 @@ class Plan:
         _expect(ipsf, torch.complex64, (self.n, self.n), self.ctx.device, "ipsf")
         check(self.lib.spk_set_ipsf(self.handle, _ptr(ipsf)))
 
+    def set_spectrum(self, rescales, weights) -> None:
+        """Spectral blur: K rescale factors and K weights (host arrays), or None."""
+        if rescales is None:
+            check(self.lib.spk_set_spectrum(self.handle, None, None, 0))
+            return
+        r = np.ascontiguousarray(rescales, np.float32)
+        w = np.ascontiguousarray(weights, np.float32)
+        assert r.ndim == 1 and r.shape == w.shape, "rescales and weights: equal 1-d arrays"
+        check(self.lib.spk_set_spectrum(self.handle, r.ctypes.data, w.ctypes.data, r.size))
+
     def check(self) -> None:
         """Wait for queued work; raise if a persistent block's barrier gave up."""
         check(self.lib.spk_plan_check(self.handle))

@@ -155,6 +155,8 @@ class phasing:
     # ------------------------------------------------------------------
     def _make_plan(self):
         self._plan = Plan(self.ctx, self.N, batch=1, shared=True)
+        if getattr(self, "_spectrum_host", None) is not None:
+            self._plan.set_spectrum(*self._spectrum_host)
         plan = self._plan
         self.modulus = DeviceArray("modulus", self.shape, np.float32, lambda: self._modulus_dev)
         self.psi_in = DeviceArray("psi_in", self.shape, np.complex64, lambda: plan.get_psi(-1))
@@ -172,7 +174,19 @@ class phasing:
         assert isinstance(support, types), "support must be ndarray if supplied"
         assert isinstance(ipsf, types), "psf must be ndarray if supplied"
         if spectrum is not None:
-            raise NotImplementedError("the spectral (longitudinal) rescale is not built yet")
+            # phasing.py:430-448: rows (energies, weights); rescale factors are
+            # the energies over the energy of the largest weight
+            assert isinstance(spectrum, np.ndarray)
+            assert spectrum.ndim == 2
+            assert spectrum.shape[0] == 2
+            energies, weights = spectrum[0], spectrum[1]
+            center_e = energies[weights.argmax()]
+            self._spectrum_host = ((energies / center_e).astype(np.float32),
+                                   weights.astype(np.float32))
+            self.N_spctrm = np.int32(energies.shape[0])
+            self.spectrum_state = 2
+            if self._plan is not None:
+                self._plan.set_spectrum(*self._spectrum_host)
         if ipsf is not None:
             # phasing.py:414-425: transverse coherence, same size as the modulus
             assert ipsf.ndim == 2

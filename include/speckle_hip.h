@@ -135,6 +135,16 @@ int spk_get_psi(spk_plan* plan, int frame, float* psi_c64);
  * trace compares div with m.  NULL switches it off.  Not combined with DM. */
 int spk_set_ipsf(spk_plan* plan, const float* ipsf_c64);
 
+/* Longitudinal (spectral) partial coherence (phasing.py:430-448
+ * load(spectrum=), applied at phasing.py:1140-1156 with
+ * phasing_radial_rescale.cl): K rescale factors (energy / energy of the peak
+ * weight) and K weights, HOST arrays, copied.  While set, the squared
+ * divisor (|Psi|^2, or the ipsf-blurred intensity when an ipsf is also set)
+ * is replaced by sum_k w_k * (its bilinear interpolation at coordinates /
+ * r_k) before the square root.  rescales = NULL or K = 0 switches it off.
+ * Not combined with DM. */
+int spk_set_spectrum(spk_plan* plan, const float* rescales, const float* weights, int K);
+
 /* -------------------------------------------------------------------------
  * The loop.  Replaces phasing.iterate's per-iteration work
  * (speckle/phasing.py:260-268 -> _iteration, 1088-1353: pyfft forward,

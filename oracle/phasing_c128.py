@@ -280,12 +280,65 @@ def coherence_divisor(Psi, ipsf):
     return np.sqrt(np.abs(ifft2(fft2(np.abs(Psi) ** 2) * ipsf)))
 
 
-def fourier_constraint_ipsf(psi, m, ipsf, zero_guard=True):
+def radial_rescale(I, rescales, weights):
+    """Spectral (longitudinal) blur of an intensity, ``phasing_radial_rescale.cl``
+    (called at ``phasing.py:1140-1156``), in FP64.
+
+    Pixel (row j, column i) of an n x n corner-centred array has signed
+    coordinates x = i (i < n/2) or i - n, y likewise from j.  For every
+    spectral sample k the intensity is bilinearly interpolated at
+    (x / r_k, y / r_k) and summed with weight w_k.  Interpolation nodes with
+    floor coordinate <= -n/2 or >= n/2 (and their +1 neighbours likewise)
+    get weight 0 (``.cl`` lines 85-92); kept nodes wrap back by +n.
+    """
+    I = np.asarray(I, np.float64)
+    n = I.shape[0]
+    h = n // 2
+    idx = np.arange(n)
+    coord = np.where(idx > h - 1, idx - n, idx).astype(np.float64)
+    X = np.broadcast_to(coord[None, :], (n, n))
+    Y = np.broadcast_to(coord[:, None], (n, n))
+    out = np.zeros((n, n))
+
+    def node(c):
+        keep = (c > -h) & (c < h)
+        return np.where(keep, np.where(c < 0, c + n, c), 0).astype(int), keep
+
+    for rv, sw in zip(np.asarray(rescales, np.float64), np.asarray(weights, np.float64)):
+        rx, ry = X / rv, Y / rv
+        x1 = np.floor(rx)
+        y1 = np.floor(ry)
+        x2, y2 = x1 + 1, y1 + 1
+        (ix1, kx1), (ix2, kx2) = node(x1), node(x2)
+        (iy1, ky1), (iy2, ky2) = node(y1), node(y2)
+        wx1 = (1 - np.abs(x1 - rx)) * kx1
+        wx2 = (1 - np.abs(x2 - rx)) * kx2
+        wy1 = (1 - np.abs(y1 - ry)) * ky1
+        wy2 = (1 - np.abs(y2 - ry)) * ky2
+        iv = (I[iy1, ix1] * wx1 * wy1 + I[iy2, ix1] * wx1 * wy2 +
+              I[iy1, ix2] * wx2 * wy1 + I[iy2, ix2] * wx2 * wy2)
+        out += iv * sw
+    return out
+
+
+def spectrum_rescales(spectrum):
+    """``load(spectrum=)`` (``phasing.py:430-448``): rows (energies, weights);
+    rescale factors are energies / the energy of the largest weight."""
+    energies, weights = np.asarray(spectrum[0], np.float64), np.asarray(spectrum[1], np.float64)
+    return energies / energies[weights.argmax()], weights
+
+
+def fourier_constraint_ipsf(psi, m, ipsf, zero_guard=True, spectrum=None):
     """P_M with the partial-coherence divisor: ``m * Psi / div``
-    (``phasing.py:1217``, ``fd`` = ``coherence_divisor``).  Returns
-    (psi_out, Psi, div); 0 where div == 0 with ``zero_guard``."""
+    (``phasing.py:1217``, ``fd`` = ``coherence_divisor``; without an ipsf the
+    coherent |Psi|).  With ``spectrum`` = (rescales, weights) the divisor is
+    then squared, spectrally blurred and square-rooted (``phasing.py:1140-
+    1156``).  Returns (psi_out, Psi, div); 0 where div == 0 with
+    ``zero_guard``."""
     Psi = fft2(psi)
-    div = coherence_divisor(Psi, ipsf)
+    div = coherence_divisor(Psi, ipsf) if ipsf is not None else np.abs(Psi)
+    if spectrum is not None:
+        div = np.sqrt(radial_rescale(div ** 2, spectrum[0], spectrum[1]))
     m = np.asarray(m, np.float64)
     if zero_guard:
         with np.errstate(divide="ignore", invalid="ignore"):
@@ -303,25 +356,27 @@ def modulus_error_div(div, m):
     return float(np.sqrt(np.sum((div - m) ** 2) / np.sum(m ** 2)))
 
 
-def iteration(psi_in, m, s, alg, beta=0.8, semantics="numpy", zero_guard=True, ipsf=None):
+def iteration(psi_in, m, s, alg, beta=0.8, semantics="numpy", zero_guard=True, ipsf=None,
+              spectrum=None):
     """One iteration of ``_iteration`` (``phasing.py:1088-1353``).
 
     Returns (psi_new, psi_out, err) where err is the modulus error of
     Psi = fft2(psi_in) before replacement (of the partially coherent divisor
-    when an ipsf is given).
+    when an ipsf or a spectrum (rescales, weights) is given).
     """
-    if ipsf is None:
+    if ipsf is None and spectrum is None:
         psi_out, Psi = fourier_constraint(psi_in, m, zero_guard)
         err = modulus_error(Psi, m)
     else:
-        psi_out, Psi, div = fourier_constraint_ipsf(psi_in, m, ipsf, zero_guard)
+        psi_out, Psi, div = fourier_constraint_ipsf(psi_in, m, ipsf, zero_guard, spectrum)
         err = modulus_error_div(div, m)
     t = None
     if alg == "dm":
-        if ipsf is None:
+        if ipsf is None and spectrum is None:
             t, _ = fourier_constraint(dm_modified(psi_in, s, beta), m, zero_guard)
         else:
-            t, _, _ = fourier_constraint_ipsf(dm_modified(psi_in, s, beta), m, ipsf, zero_guard)
+            t, _, _ = fourier_constraint_ipsf(dm_modified(psi_in, s, beta), m, ipsf, zero_guard,
+                                              spectrum)
     return update(alg, psi_in, psi_out, s, beta, semantics, t), psi_out, err
 
 

@@ -164,3 +164,71 @@ def test_dropin_load_ipsf(gpu_ctx):
     assert abs(r.get_error_trace()[0] - e) / e < TOL
     with pytest.raises(AssertionError):
         r.load(ipsf=np.ones((64, 64)))     # must keep the modulus shape
+
+
+# ----------------------------------------------------- spectral (longitudinal)
+SPECTRUM = np.array([[96.0, 98.0, 100.0, 102.0, 104.0],
+                     [0.1, 0.5, 1.0, 0.6, 0.2]])
+
+
+def spectrum_args():
+    r, w = O.spectrum_rescales(SPECTRUM)
+    return r.astype(np.float32), w.astype(np.float32)
+
+
+@pytest.mark.parametrize("n,tile,ept", [(64, 4, 8), (256, 2, 16), (256, 8, 8), (1024, 4, 8),
+                                        (4096, 2, 8)])
+@pytest.mark.parametrize("with_ipsf", [False, True])
+def test_one_step_with_spectrum_matches_oracle(gpu_ctx, n, tile, ept, with_ipsf):
+    print(f"[spectrum one-step n={n} tile={tile} ept={ept} ipsf={with_ipsf}]", flush=True)
+    f, g = setup(n, seed=7)
+    ipsf = gaussian_ipsf(n, 1.5) if with_ipsf else None
+    r, w = spectrum_args()
+    p = plan_with_ipsf(gpu_ctx, f, g, ipsf, tile, ept)
+    p.set_spectrum(r, w)
+    err = p.iterate([step("hio", 0.8)]).cpu().numpy()[0, 0]
+    got = p.get_psi().cpu().numpy()
+    p.close()
+    want, _, e = O.iteration(g.astype(np.complex128), f.modulus, f.support, "hio", 0.8,
+                             ipsf=None if ipsf is None else ipsf.astype(np.complex128),
+                             spectrum=(r.astype(np.float64), w.astype(np.float64)))
+    # with an ipsf the FP32 convolution of |Psi|^2 bounds large frames (the
+    # reference's own complex64 path is 2e-5 / 9e-5 off at 1024 / 4096, see
+    # test_one_step_with_ipsf_large_frames)
+    tol = TOL if n <= 256 else 2e-4 if with_ipsf else 2e-5
+    print(f"  rel L2 {rel(got, want):.2e}, error {abs(err - e) / e:.2e}")
+    assert rel(got, want) < tol
+    assert abs(err - e) / e < 1e-5
+
+
+def test_spectrum_single_sample_at_one_is_coherent_away_from_the_edge(gpu_ctx):
+    """Factor 1, weight 1: the blur is the identity except the row and column
+    at -n/2, which the reference's bounds rule drops (div = 0 there, so those
+    pixels are zeroed by the guarded replacement)."""
+    n = 128
+    f, g = setup(n, seed=8)
+    a = plan_with_ipsf(gpu_ctx, f, g, None)
+    a.set_spectrum(np.array([1.0], np.float32), np.array([1.0], np.float32))
+    a.iterate([step("er")])
+    want, _, _ = O.iteration(g.astype(np.complex128), f.modulus, f.support, "er",
+                             spectrum=(np.array([1.0]), np.array([1.0])))
+    assert rel(a.get_psi().cpu().numpy(), want) < TOL
+
+
+def test_dropin_load_spectrum(gpu_ctx):
+    n = 128
+    f, g = setup(n, seed=9)
+    r = phasing()
+    r.load(spectrum=SPECTRUM)
+    assert r.spectrum_state == 2 and r.N_spctrm == 5
+    r.load(modulus=f.modulus, support=f.support, numtrials=1)
+    r._plan.set_psi(dev(g, torch.complex64), -1, mult_support=False)
+    r._seeded = True
+    r.iterate(2, order=(("er", 1),))
+    rr, ww = O.spectrum_rescales(SPECTRUM)
+    cur = g.astype(np.complex128)
+    for _ in range(2):
+        cur, _, _ = O.iteration(cur, f.modulus, f.support, "er", spectrum=(rr, ww))
+    assert rel(r.get(r.psi_in), cur) < 2e-5
+    with pytest.raises(AssertionError):
+        r.load(spectrum=np.ones((3, 5)))

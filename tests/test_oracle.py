@@ -188,6 +188,39 @@ def test_coherence_divisor_closed_forms():
                        m * np.abs(Psi2) / div2)
 
 
+def test_radial_rescale_closed_forms():
+    rng = np.random.default_rng(31)
+    n = 16
+    I = rng.random((n, n))
+    # one sample, factor 1, weight 1: every node is an integer, identity
+    # except the row/column at -n/2 (its floor node is dropped, .cl line 85)
+    out = O.radial_rescale(I, [1.0], [1.0])
+    keep = np.ones((n, n), bool)
+    keep[n // 2, :] = keep[:, n // 2] = False
+    assert np.allclose(out[keep], I[keep]) and np.all(out[~keep] == 0)
+    # linear in the weights and additive over samples
+    a = O.radial_rescale(I, [1.1], [1.0])
+    b = O.radial_rescale(I, [0.9], [1.0])
+    assert np.allclose(O.radial_rescale(I, [1.1, 0.9], [0.3, 0.7]), 0.3 * a + 0.7 * b)
+    # bilinear interpolation reproduces a bilinear function exactly: with
+    # I(y, x) = (2 + x)(3 + y) in signed coordinates and factor 2, pixels whose
+    # nodes stay inside (-n/2, n/2) read I(y/2, x/2)
+    c = np.where(np.arange(n) > n // 2 - 1, np.arange(n) - n, np.arange(n)).astype(float)
+    J = (3 + c[:, None]) * (2 + c[None, :])
+    got = O.radial_rescale(J, [2.0], [1.0])
+    want = (3 + c[:, None] / 2) * (2 + c[None, :] / 2)
+    inner = (np.abs(c) <= n // 2 - 2)
+    sel = inner[:, None] & inner[None, :]
+    assert np.allclose(got[sel], want[sel])
+    # by hand: the pixel at (y, x) = (1, 3) with factor 2 reads the midpoint of
+    # (0, 1), (0, 2), (1, 1), (1, 2) -> mean of those four values
+    got13 = O.radial_rescale(I, [2.0], [1.0])[1, 3]
+    assert np.isclose(got13, 0.25 * (I[0, 1] + I[0, 2] + I[1, 1] + I[1, 2]))
+    # spectrum rows: rescales are energies over the energy of the peak weight
+    r, w = O.spectrum_rescales(np.array([[90.0, 100.0, 110.0], [0.2, 1.0, 0.3]]))
+    assert np.allclose(r, [0.9, 1.0, 1.1]) and np.allclose(w, [0.2, 1.0, 0.3])
+
+
 def test_modulus_error_closed_form():
     Psi = np.array([[3 + 4j, 0], [1, -2j]])
     m = np.array([[4.0, 1.0], [1.0, 1.0]])
