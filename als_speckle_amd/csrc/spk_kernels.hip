@@ -29,16 +29,27 @@ namespace spk {
 // ---------------------------------------------------------------------------
 // LDS plan of an FFT kernel: exchange buffer(s) plus the twiddle table.
 // Double buffering (exchanges alternate between two buffers) needs one
-// barrier per exchange instead of two; it is used where both buffers and
-// the twiddle table fit in the 160 KB of a gfx950 CU.  The full twiddle
-// table (N entries) is used where it fits, else the two-level one.
+// barrier per exchange instead of two.  It is used where both buffers and
+// the twiddle table fit in the 160 KB of a gfx950 CU and the second buffer
+// does not lower the number of workgroups the LDS admits per CU (counted up
+// to 4): measured at 1024^2 with 16 frames, 3 resident workgroups with one
+// buffer beat 2 with two by 17 % (column pass) and 4 % (row pass).  The full
+// twiddle table (N entries) is used where it fits, else the two-level one.
 // ---------------------------------------------------------------------------
 constexpr int kLdsBytes = 160 * 1024;
 
 template <int N, int TT>
+constexpr bool db_pays() {
+  constexpr int buf = lds_elems<N, TT>();
+  constexpr int one = kLdsBytes / ((buf + N) * 8);
+  constexpr int two = kLdsBytes / ((2 * buf + N) * 8);
+  return two >= 1 && two >= (one < 4 ? one : 4);
+}
+
+template <int N, int TT>
 struct LdsPlan {
   static constexpr int BUF = lds_elems<N, TT>();
-  static constexpr bool DB = (2 * BUF + N) * 8 <= kLdsBytes;
+  static constexpr bool DB = db_pays<N, TT>();
   static constexpr int NBUF = DB ? 2 : 1;
   static constexpr bool FULL_TW = (NBUF * BUF + N) * 8 <= kLdsBytes;
   static constexpr int TW_ELEMS = FULL_TW ? N : (N >> 6) + 64;

@@ -216,12 +216,14 @@ def test_zero_guard(gpu_ctx):
 
 
 # ------------------------------------------------------------ shrinkwrap
-@pytest.mark.parametrize("n,sigma", [(64, 3.0), (256, 2.0), (512, 1.5)])
-def test_shrinkwrap_matches_oracle(gpu_ctx, n, sigma):
+@pytest.mark.parametrize("n,sigma,tile", [(64, 3.0, 0), (256, 2.0, 0), (512, 1.5, 0),
+                                          (256, 2.0, 2), (256, 3.0, 8), (2048, 2.5, 0)])
+def test_shrinkwrap_matches_oracle(gpu_ctx, n, sigma, tile):
+    print(f"[shrinkwrap n={n} sigma={sigma} tile={tile}]", flush=True)
     f = frame(n, seed=3)
     rng = np.random.default_rng(n)
     psi = (f.obj + 0.05 * (rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)))).astype(np.complex64)
-    p = load_plan(gpu_ctx, f, psi)
+    p = load_plan(gpu_ctx, f, psi, tile=tile)
     p.shrinkwrap(sigma, 0.2)
     got = p.get_support().cpu().numpy()
     want, b = O.shrinkwrap_support(psi.astype(np.complex128), f.support, sigma, 0.2)
