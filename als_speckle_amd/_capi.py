@@ -6,6 +6,8 @@ from __future__ import annotations
 
 import ctypes
 import os
+
+import numpy as np
 from ctypes import POINTER, c_char_p, c_double, c_float, c_int, c_int32, c_size_t, c_uint32, c_void_p
 
 LIB_NAME = "libspeckle_hip.so"
@@ -111,10 +113,32 @@ def last_error() -> str:
     return load().spk_last_error().decode()
 
 
-def make_steps(entries) -> "ctypes.Array":
-    """entries: iterable of (alg_code, sem_code, beta, sw_sigma, sw_tau)."""
+# numpy twin of SpkStep: step tables built with array operations (serving
+# builds one per launch for every slot)
+STEP_DTYPE = np.dtype([("alg", "<i4"), ("semantics", "<i4"), ("beta", "<f4"),
+                       ("sw_sigma", "<f4"), ("sw_tau", "<f4"), ("pad_", "<i4")])
+assert STEP_DTYPE.itemsize == ctypes.sizeof(SpkStep)
+
+
+def step_table(entries) -> np.ndarray:
+    """Contiguous STEP_DTYPE array from (alg, sem, beta, sw_sigma, sw_tau)
+    tuples or from a STEP_DTYPE array (returned as is when contiguous)."""
+    if isinstance(entries, np.ndarray) and entries.dtype == STEP_DTYPE:
+        return np.ascontiguousarray(entries).reshape(-1)
     entries = list(entries)
-    arr = (SpkStep * max(1, len(entries)))()
-    for i, (alg, sem, beta, sig, tau) in enumerate(entries):
-        arr[i] = SpkStep(alg, sem, float(beta), float(sig), float(tau), 0)
+    arr = np.zeros(len(entries), STEP_DTYPE)
+    if entries:
+        cols = list(zip(*entries))
+        arr["alg"], arr["semantics"] = cols[0], cols[1]
+        arr["beta"], arr["sw_sigma"], arr["sw_tau"] = cols[2], cols[3], cols[4]
+    return arr
+
+
+def make_steps(entries) -> "ctypes.Array":
+    """entries: iterable of (alg_code, sem_code, beta, sw_sigma, sw_tau), or a
+    STEP_DTYPE array."""
+    tab = step_table(entries)
+    arr = (SpkStep * max(1, tab.size))()
+    if tab.size:
+        ctypes.memmove(arr, tab.ctypes.data, tab.nbytes)
     return arr

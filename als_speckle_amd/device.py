@@ -174,20 +174,22 @@ class Plan:
     # -- the loop --------------------------------------------------------
     def iterate(self, steps, per_frame: bool = False, error_trace: bool = True,
                 err_out: torch.Tensor | None = None):
-        """steps: list of (alg, sem, beta, sw_sigma, sw_tau) codes.
+        """steps: list of (alg, sem, beta, sw_sigma, sw_tau) codes, or a
+        ``_capi.STEP_DTYPE`` array (step-major, frames fastest with per_frame).
 
         Returns a (nsteps, batch) float64 device tensor of modulus errors, or
         None.  With per_frame, ``steps`` has nsteps * batch entries.
         """
-        steps = list(steps)
-        nsteps = len(steps) // (self.batch if per_frame else 1)
-        arr = _capi.make_steps(steps)
+        tab = _capi.step_table(steps)
+        nsteps = tab.size // (self.batch if per_frame else 1)
         err = None
         if error_trace:
             err = err_out if err_out is not None else torch.empty(
                 (nsteps, self.batch), dtype=torch.float64, device=self.ctx.device)
-        check(self.lib.spk_iterate(self.handle, arr, nsteps, int(per_frame),
-                                   _ptr(err) if err is not None else None))
+        # the library copies the table into its staged upload before returning
+        check(self.lib.spk_iterate(self.handle,
+                                   tab.ctypes.data_as(ctypes.POINTER(_capi.SpkStep)), nsteps,
+                                   int(per_frame), _ptr(err) if err is not None else None))
         return err
 
     def shrinkwrap(self, sigma: float, tau: float, frame: int = -1) -> None:

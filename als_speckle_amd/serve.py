@@ -63,7 +63,8 @@ class Result:
         return self.finished - self.arrival
 
 
-def _steps_for(req: Request):
+def _steps_for(req: Request) -> np.ndarray:
+    """The request's whole schedule as a step table (one entry per iteration)."""
     sem = _capi.SEM_CODES[req.semantics]
     sw = {}
     if req.shrinkwrap:
@@ -71,11 +72,8 @@ def _steps_for(req: Request):
         if isinstance(req.shrinkwrap, dict):
             kw.update(req.shrinkwrap)
         sw = shrinkwrap_schedule(req.iterations, **kw)
-    return [(_capi.ALG_CODES[a], sem, req.beta, *sw.get(k, (0.0, 0.0)))
-            for k, a in enumerate(expand_order(req.order, req.iterations))]
-
-
-_NOOP = (_capi.NOOP, 0, 0.0, 0.0, 0.0)
+    return _capi.step_table([(_capi.ALG_CODES[a], sem, req.beta, *sw.get(k, (0.0, 0.0)))
+                             for k, a in enumerate(expand_order(req.order, req.iterations))])
 
 
 class _Pool:
@@ -114,16 +112,15 @@ class _Pool:
         """Queue the next `block` iterations of every active slot."""
         if not self.active:
             return False
-        steps = []
-        for k in range(block):
-            for f in range(self.slots):
-                st = self.active.get(f)
-                if st is None or st[2] + k >= len(st[1]):
-                    steps.append(_NOOP)
-                else:
-                    steps.append(st[1][st[2] + k])
+        # (block, slots) table, idle entries NOOP; each active slot's column
+        # is the next stretch of its own schedule
+        table = np.zeros((block, self.slots), _capi.STEP_DTYPE)
+        table["alg"] = _capi.NOOP
+        for f, st in self.active.items():
+            take = min(block, len(st[1]) - st[2])
+            table[:take, f] = st[1][st[2]:st[2] + take]
         with torch.cuda.stream(self.stream):
-            err = self.plan.iterate(steps, per_frame=True)
+            err = self.plan.iterate(table, per_frame=True)
         for f, st in self.active.items():
             take = min(block, len(st[1]) - st[2])
             st[3].append((err, f, take))
