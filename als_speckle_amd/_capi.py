@@ -71,6 +71,10 @@ SIGNATURES = {
     "spk_prtf": (c_int, [c_void_p, c_void_p, c_int, c_int, c_int, c_void_p]),
     "spk_rftf": (c_int, [c_void_p, c_void_p, c_int, c_int, c_void_p, c_int, c_void_p]),
     "spk_rftf_stack": (c_int, [c_void_p, c_void_p, c_int, c_int, c_int, c_void_p, c_void_p]),
+    "spk_coherence_rl": (c_int, [c_void_p, c_void_p, c_int, c_int, c_void_p, c_int, c_int, c_int,
+                                 c_void_p]),
+    "spk_coherence_gauss_prepare": (c_int, [c_void_p, c_void_p, c_int, c_int, c_void_p]),
+    "spk_coherence_gauss_eval": (c_int, [c_void_p, c_double, c_double, POINTER(c_double)]),
     "spk_time_passes": (c_int, [c_void_p, c_int, c_int, POINTER(SpkStep), POINTER(c_float)]),
     "spk_plan_profile": (c_int, [c_void_p, c_int]),
     "spk_graph_begin": (c_int, [c_void_p]),

@@ -141,6 +141,13 @@ struct spk_plan {
   float* imap = nullptr;
   float* dmap = nullptr;
   bool spec_on = false;
+  // Gaussian coherence fit: autocorrelation, tiled modulus, work frames
+  c32* coh_ac = nullptr;
+  c32* coh_w = nullptr;
+  c32* coh_cb = nullptr;
+  float* coh_m = nullptr;
+  double* coh_part = nullptr;
+  bool coh_ready = false;
   float phib_g = 0.0f;
   long long nn() const { return (long long)n * n; }
   // Phi kept in two layouts (see spk_fft_core.hpp dual_layout): n >= 4096
@@ -488,6 +495,7 @@ int spk_plan_destroy(spk_plan* p) {
   dfree(p->tw); dfree(p->summ2); dfree(p->sumpart); dfree(p->errp); dfree(p->pbar);
   dfree(p->ipsf); dfree(p->dbuf); dfree(p->dbufc);
   dfree(p->spec); dfree(p->imap); dfree(p->dmap);
+  dfree(p->coh_ac); dfree(p->coh_w); dfree(p->coh_cb); dfree(p->coh_m); dfree(p->coh_part);
   dfree(p->sw_tmp); dfree(p->sw_blur); dfree(p->sw_maxp); dfree(p->fftbuf);
   dfree(p->stage_d);
   prof_clear(p, 0);
@@ -1089,6 +1097,149 @@ int spk_rftf_stack(spk_plan* p, const float* stack, int frames, int rows, int co
   SPK_HIP(launch_embed((const c32*)stack, pad, rows, cols, p->n, frames, p->ctx->stream));
   if ((rc = spk_fft2(p, (const float*)pad, (float*)F, frames, 0))) return rc;
   SPK_HIP(launch_rftf_stack(F, goal, frames, p->n, out, p->ctx->stream));
+  return SPK_OK;
+}
+
+namespace {
+// 2-D transform of one tiled frame: out = F(in), or the unnormalised inverse.
+// In the single layout in may equal out; cb is a column-band work frame for
+// the dual layout (n >= 4096).
+int fft2_tiled(spk_plan* p, const c32* in, c32* out, c32* cb, bool inverse) {
+  hipStream_t st = p->ctx->stream;
+  c32* mid = p->dual() ? cb : out;
+  SPK_HIP(launch_rowfft(p->n, p->tile, p->ept, inverse ? 1 : 0, in, mid, nullptr, 0, 0.0f, p->tw,
+                        nullptr, 1, st));
+  PassArgs a = base_args(p);
+  a.phic = mid;
+  a.phi = out;
+  a.c = Coef{0, 0, 0, 0, 0, 0, 0, kActive};
+  a.tab = nullptr;
+  a.fmap = nullptr;
+  a.errp = nullptr;
+  SPK_HIP(launch_colpass(p->n, p->tile, p->ept, inverse ? 3 : 2, a, 1, 1, st));
+  return SPK_OK;
+}
+
+constexpr int kCohBlocks = 256;
+
+// active = the (rows, cols) c64 estimate embedded top-left, tiled, in dst
+// (row-major staging in tmp)
+int embed_tiled(spk_plan* p, const float* est, int rows, int cols, c32* tmp, c32* dst) {
+  SPK_HIP(launch_embed((const c32*)est, tmp, rows, cols, p->n, 1, p->ctx->stream));
+  SPK_HIP(launch_pack_c64(p->n, p->tile, dst, tmp, nullptr, 0, 1, p->ctx->stream));
+  return SPK_OK;
+}
+}  // namespace
+
+int spk_coherence_rl(spk_plan* p, const float* est, int rows, int cols, const float* modulus,
+                     int iterations, int semantics, int is_fourier, float* ipsf) {
+  SPK_REQUIRE(p && est && modulus && ipsf, SPK_EINVAL, "bad arguments");
+  SPK_REQUIRE(rows >= 1 && cols >= 1 && rows <= p->n && cols <= p->n, SPK_EINVAL,
+              "estimate larger than the modulus");
+  SPK_REQUIRE(iterations >= 0, SPK_EINVAL, "iterations must be >= 0");
+  SPK_REQUIRE(semantics == SPK_SEM_NUMPY || semantics == SPK_SEM_OPENCL, SPK_EINVAL,
+              "unknown semantics");
+  int rc = set_device(p->ctx);
+  if (rc) return rc;
+  const int n = p->n, tile = p->tile;
+  const size_t nn = (size_t)p->nn();
+  hipStream_t st = p->ctx->stream;
+  void* scr;
+  const size_t frames = p->dual() ? 6 : 5;
+  if ((rc = ctx_scratch(p->ctx, frames * nn * sizeof(c32) + nn * sizeof(float) +
+                                    kCohBlocks * sizeof(double), &scr)))
+    return rc;
+  c32* u = (c32*)scr;
+  c32* W = u + nn;
+  c32* w = W + nn;
+  c32* P = w + nn;
+  c32* Ph = P + nn;
+  c32* cb = p->dual() ? Ph + nn : nullptr;
+  float* d = (float*)(Ph + (p->dual() ? 2 : 1) * nn);
+  double* part = (double*)(d + nn);
+  const float inv_nn = (float)(1.0 / (double)nn);
+  // p = |fft2(active)|^2 / sum (|active|^2 for a Fourier-space estimate, the
+  // GPU branch), P = fft2(p), Ph = P reversed (phasing.py:806-854)
+  if ((rc = embed_tiled(p, est, rows, cols, W, P))) return rc;
+  if (!is_fourier && (rc = fft2_tiled(p, P, P, cb, false))) return rc;
+  SPK_HIP(launch_coh(3, n, tile, P, nullptr, nullptr, nullptr, nullptr, part, kCohBlocks, 0, 0, 0, st));
+  SPK_HIP(launch_coh(4, n, tile, P, nullptr, nullptr, nullptr, nullptr, part, kCohBlocks, 0, 0, 0, st));
+  if ((rc = fft2_tiled(p, P, P, cb, false))) return rc;
+  SPK_HIP(launch_coh(6, n, tile, nullptr, P, Ph, nullptr, nullptr, nullptr, 0, 0, 0, 0, st));
+  // d = m^2 / sum(m^2) (phasing.py:819-821)
+  SPK_HIP(launch_pack_f32(n, tile, d, modulus, 1, st));
+  SPK_HIP(launch_coh(5, n, tile, nullptr, nullptr, nullptr, nullptr, d, part, kCohBlocks, 0, 0, 0, st));
+  SPK_HIP(launch_coh(4, n, tile, nullptr, nullptr, nullptr, nullptr, d, part, kCohBlocks, 0, 0, 0, st));
+  // u = the start Gaussian of widths n/4 (phasing.py:818)
+  const double w0 = (double)(n / 4);
+  SPK_HIP(launch_coh(7, n, tile, nullptr, nullptr, u, nullptr, nullptr, nullptr, 0, 0, w0, w0, st));
+  for (int it = 0; it < iterations; ++it) {
+    // blur = ifft2(fft2(u) P); w = d / blur; w = ifft2(fft2(w) Ph); u *= w
+    // (phasing.py:700-736); the inverse transforms are unnormalised, 1/n^2
+    // is applied in the elementwise steps
+    if ((rc = fft2_tiled(p, u, W, cb, false))) return rc;
+    SPK_HIP(launch_coh(0, n, tile, W, P, W, nullptr, nullptr, nullptr, 0, 0, 0, 0, st));
+    if ((rc = fft2_tiled(p, W, W, cb, true))) return rc;
+    SPK_HIP(launch_coh(1, n, tile, nullptr, W, w, d, nullptr, nullptr, 0, inv_nn, 0, 0, st));
+    if ((rc = fft2_tiled(p, w, w, cb, false))) return rc;
+    SPK_HIP(launch_coh(0, n, tile, w, Ph, w, nullptr, nullptr, nullptr, 0, 0, 0, 0, st));
+    if ((rc = fft2_tiled(p, w, w, cb, true))) return rc;
+    SPK_HIP(launch_coh(2, n, tile, u, w, nullptr, nullptr, nullptr, nullptr, 0, inv_nn, 0, 0, st));
+  }
+  // ipsf = fft2(u), real part with numpy semantics, / |ipsf(0,0)|
+  if ((rc = fft2_tiled(p, u, u, cb, false))) return rc;
+  SPK_HIP(launch_coh(9, n, tile, u, nullptr, nullptr, nullptr, nullptr, part, 0, 0,
+                     semantics == SPK_SEM_NUMPY ? 1.0 : 0.0, 0, st));
+  SPK_HIP(launch_unpack_c64(n, tile, (c32*)ipsf, u, 1.0f, 1, st));
+  return SPK_OK;
+}
+
+int spk_coherence_gauss_prepare(spk_plan* p, const float* est, int rows, int cols,
+                                const float* modulus) {
+  SPK_REQUIRE(p && est && modulus, SPK_EINVAL, "bad arguments");
+  SPK_REQUIRE(rows >= 1 && cols >= 1 && rows <= p->n && cols <= p->n, SPK_EINVAL,
+              "estimate larger than the modulus");
+  int rc = set_device(p->ctx);
+  if (rc) return rc;
+  const size_t nn = (size_t)p->nn();
+  if (!p->coh_ac) {
+    if ((rc = dalloc(&p->coh_ac, nn)) || (rc = dalloc(&p->coh_w, nn)) ||
+        (p->dual() && (rc = dalloc(&p->coh_cb, nn))) || (rc = dalloc(&p->coh_m, nn)) ||
+        (rc = dalloc(&p->coh_part, (size_t)kCohBlocks)))
+      return rc;
+  }
+  hipStream_t st = p->ctx->stream;
+  // ac = ifft2(|fft2(active)|^2) (phasing.py:785-803)
+  if ((rc = embed_tiled(p, est, rows, cols, p->coh_w, p->coh_ac))) return rc;
+  if ((rc = fft2_tiled(p, p->coh_ac, p->coh_ac, p->coh_cb, false))) return rc;
+  SPK_HIP(launch_coh(3, p->n, p->tile, p->coh_ac, nullptr, nullptr, nullptr, nullptr, p->coh_part,
+                     kCohBlocks, 0, 0, 0, st));
+  if ((rc = fft2_tiled(p, p->coh_ac, p->coh_ac, p->coh_cb, true))) return rc;
+  SPK_HIP(launch_coh(10, p->n, p->tile, p->coh_ac, nullptr, nullptr, nullptr, nullptr, nullptr, 0,
+                     (float)(1.0 / (double)nn), 0, 0, st));
+  SPK_HIP(launch_pack_f32(p->n, p->tile, p->coh_m, modulus, 1, st));
+  p->coh_ready = true;
+  return SPK_OK;
+}
+
+int spk_coherence_gauss_eval(spk_plan* p, double cl0, double cl1, double* value) {
+  SPK_REQUIRE(p && value, SPK_EINVAL, "bad arguments");
+  SPK_REQUIRE(p->coh_ready, SPK_ESTATE, "spk_coherence_gauss_prepare first");
+  int rc = set_device(p->ctx);
+  if (rc) return rc;
+  hipStream_t st = p->ctx->stream;
+  // sum (sqrt|fft2(g * ac)| - m)^2 (phasing.py:657-698)
+  SPK_HIP(launch_coh(7, p->n, p->tile, nullptr, p->coh_ac, p->coh_w, nullptr, nullptr, nullptr, 0,
+                     0, cl0, cl1, st));
+  if ((rc = fft2_tiled(p, p->coh_w, p->coh_w, p->coh_cb, false))) return rc;
+  SPK_HIP(launch_coh(8, p->n, p->tile, nullptr, p->coh_w, nullptr, p->coh_m, nullptr, p->coh_part,
+                     kCohBlocks, 0, 0, 0, st));
+  double h[kCohBlocks];
+  SPK_HIP(hipMemcpyAsync(h, p->coh_part, sizeof(h), hipMemcpyDeviceToHost, st));
+  SPK_HIP(hipStreamSynchronize(st));
+  double s = 0.0;
+  for (int b = 0; b < kCohBlocks; ++b) s += h[b];
+  *value = s;
   return SPK_OK;
 }
 

@@ -105,6 +105,14 @@ hipError_t launch_rowconv(int n, int tile, int ept, const c32* in, c32* out, con
 // factors then K weights; maps column-band-major with the plan's tile
 hipError_t launch_rescale(int n, int tile, const float* imap, float* dmap, const float* spec,
                           int K, const int* fmap, int frames, hipStream_t st);
+// coherence estimation pieces (op: 0 out = a*b; 1 out = d/(b*scale);
+// 2 a *= b*scale; 3 a = |a|^2 with partial sums; 4 a (or fr) /= sum of
+// partials; 5 fr = fr^2 with partial sums; 6 out = reversal of b (tiled);
+// 7 out = b * Gaussian(s0, s1) (b = nullptr: the Gaussian); 8 partial sums
+// of (sqrt|b| - d)^2; 9 a = a (real part if s0 != 0) / |a(0,0)|; 10 a *= sc)
+hipError_t launch_coh(int op, int n, int tile, c32* a, const c32* b, c32* out, const float* d,
+                      float* fr, double* partial, int nblk, float scale, double s0, double s1,
+                      hipStream_t st);
 // workgroups of the persistent kernel that fit on the current device at once
 hipError_t persist_grid(int n, int tile, int ept, int* out);
 // mode 0: out = rowFFT(in); 1: out = rowIFFT(in) (unscaled);

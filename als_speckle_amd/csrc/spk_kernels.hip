@@ -1199,6 +1199,169 @@ static __global__ void __launch_bounds__(256) k_rescale(const float* imap, float
 }
 
 // ---------------------------------------------------------------------------
+// Coherence estimation (optimize_coherence, phasing.py:507-930): elementwise
+// pieces around the library's 2-D transforms.  Arrays are in the plan's
+// tiled layout (tile tt); position p of a tiled array is row-major (r, c)
+// through tile_rc.
+// ---------------------------------------------------------------------------
+// signed coordinate of index i after fftshift: ((i + n/2) mod n) - n/2
+static __device__ __forceinline__ int shifted_coord(int i, int n) {
+  return ((i + n / 2) & (n - 1)) - n / 2;
+}
+
+static __device__ __forceinline__ void tiled_rc(int n, int tt, long long p, int& r, int& c) {
+  const long long tile = p / (tt * tt);
+  const int w = (int)(p % (tt * tt));
+  const int nb = n / tt;
+  r = (int)(tile / nb) * tt + w / tt;
+  c = (int)(tile % nb) * tt + w % tt;
+}
+
+static __device__ __forceinline__ long long tiled_pos(int n, int tt, int r, int c) {
+  return ((long long)(r / tt) * (n / tt) + c / tt) * (tt * tt) + (r % tt) * tt + (c % tt);
+}
+
+// a *= sc
+static __global__ void __launch_bounds__(256) k_scale_c(c32* a, long long nn, float sc) {
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < nn;
+       i += (long long)gridDim.x * 256)
+    a[i] = scale(a[i], sc);
+}
+
+// out = a * b
+static __global__ void __launch_bounds__(256) k_cmul(const c32* a, const c32* b, c32* out,
+                                                     long long nn) {
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < nn;
+       i += (long long)gridDim.x * 256)
+    out[i] = mul(a[i], b[i]);
+}
+
+// out = d / (w * scale), d real (the Richardson-Lucy quotient, phasing.py:720-723)
+static __global__ void __launch_bounds__(256) k_rl_divide(const float* d, const c32* w, c32* out,
+                                                          long long nn, float sc) {
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < nn;
+       i += (long long)gridDim.x * 256) {
+    const c32 v = scale(w[i], sc);
+    const float den = fmaf(v.x, v.x, v.y * v.y);
+    out[i] = c32{d[i] * v.x / den, -d[i] * v.y / den};
+  }
+}
+
+// u *= w * scale (phasing.py:733-736)
+static __global__ void __launch_bounds__(256) k_rl_update(c32* u, const c32* w, long long nn,
+                                                          float sc) {
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < nn;
+       i += (long long)gridDim.x * 256)
+    u[i] = mul(u[i], scale(w[i], sc));
+}
+
+// x = |x|^2 (+0i) with FP64 block partial sums of |x|^2
+static __global__ void __launch_bounds__(256) k_abs2_sum(c32* x, long long nn, double* partial) {
+  __shared__ double sh[256];
+  double acc = 0.0;
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < nn;
+       i += (long long)gridDim.x * 256) {
+    const float v = fmaf(x[i].x, x[i].x, x[i].y * x[i].y);
+    x[i] = c32{v, 0.0f};
+    acc += v;
+  }
+  const double s = block_sum(acc, sh);
+  if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+// x /= sum of the nblk partials (each thread adds them in order: deterministic)
+static __global__ void __launch_bounds__(256) k_div_by_partials(c32* x, float* xr, long long nn,
+                                                                const double* partial, int nblk) {
+  double s = 0.0;
+  for (int b = 0; b < nblk; ++b) s += partial[b];
+  const float inv = (float)(1.0 / s);
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < nn;
+       i += (long long)gridDim.x * 256) {
+    if (x) x[i] = scale(x[i], inv);
+    if (xr) xr[i] *= inv;
+  }
+}
+
+// d = m^2 with FP64 partial sums (m tiled, in place)
+static __global__ void __launch_bounds__(256) k_sq_sum(float* m, long long nn, double* partial) {
+  __shared__ double sh[256];
+  double acc = 0.0;
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < nn;
+       i += (long long)gridDim.x * 256) {
+    const float v = m[i] * m[i];
+    m[i] = v;
+    acc += v;
+  }
+  const double s = block_sum(acc, sh);
+  if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+// out(r, c) = in(n-1-r, n-1-c): the plain reversal P[::-1, ::-1]
+// (common_flip_f2.cl), tiled in and out
+static __global__ void __launch_bounds__(256) k_flip(const c32* in, c32* out, int n, int tt) {
+  const long long nn = (long long)n * n;
+  for (long long p = (long long)blockIdx.x * 256 + threadIdx.x; p < nn;
+       p += (long long)gridDim.x * 256) {
+    int r, c;
+    tiled_rc(n, tt, p, r, c);
+    out[p] = in[tiled_pos(n, tt, n - 1 - r, n - 1 - c)];
+  }
+}
+
+// out = ac * exp(-y^2/(2 s0^2)) exp(-x^2/(2 s1^2)) on fftshifted signed
+// coordinates (the Gaussian of _opt_iter_g, phasing.py:663-670), FP64 exp;
+// with ac = nullptr, out = the Gaussian itself (the Richardson-Lucy start,
+// phasing.py:818, widths n/4)
+static __global__ void __launch_bounds__(256) k_gauss_mul(const c32* ac, c32* out, int n, int tt,
+                                                          double s0, double s1) {
+  const long long nn = (long long)n * n;
+  for (long long p = (long long)blockIdx.x * 256 + threadIdx.x; p < nn;
+       p += (long long)gridDim.x * 256) {
+    int r, c;
+    tiled_rc(n, tt, p, r, c);
+    const double y = shifted_coord(r, n), x = shifted_coord(c, n);
+    const float g = (float)(exp(-y * y / (2.0 * s0 * s0)) * exp(-x * x / (2.0 * s1 * s1)));
+    out[p] = ac ? scale(ac[p], g) : c32{g, 0.0f};
+  }
+}
+
+// FP64 partials of sum (sqrt|W| - m)^2 (phasing.py:672-698)
+static __global__ void __launch_bounds__(256) k_gauss_resid(const c32* W, const float* m,
+                                                            long long nn, double* partial) {
+  __shared__ double sh[256];
+  double acc = 0.0;
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < nn;
+       i += (long long)gridDim.x * 256) {
+    const double a = sqrt(sqrt((double)W[i].x * W[i].x + (double)W[i].y * W[i].y));
+    const double d = a - (double)m[i];
+    acc = fma(d, d, acc);
+  }
+  const double s = block_sum(acc, sh);
+  if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+// final ipsf: optionally the real part (the CPU branch's float32 cast,
+// phasing.py:778-783), divided by |u(0,0)| (phasing.py:641-643); position 0
+// of a tiled array is (0,0).  norm[0] is written by the first launch
+// (which = 0) and applied by the second (which = 1).
+static __global__ void __launch_bounds__(256) k_rl_finish(c32* u, long long nn, int real_only,
+                                                          double* norm, int which) {
+  if (which == 0) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      const c32 v = u[0];
+      norm[0] = real_only ? fabs((double)v.x) : sqrt((double)v.x * v.x + (double)v.y * v.y);
+    }
+    return;
+  }
+  const float inv = (float)(1.0 / norm[0]);
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < nn;
+       i += (long long)gridDim.x * 256) {
+    const c32 v = u[i];
+    u[i] = real_only ? c32{v.x * inv, 0.0f} : scale(v, inv);
+  }
+}
+
+// ---------------------------------------------------------------------------
 // mean over a stack of frames (the trial average, phasing.py:960-961):
 // out[i] = (1/frames) sum_f data[f][i], summed in FP64 in frame order
 // ---------------------------------------------------------------------------
@@ -1735,6 +1898,29 @@ hipError_t launch_rftf_stack(const c32* F, const float* goal, int frames, int n,
                              hipStream_t st) {
   hipLaunchKernelGGL(k_rftf_stack, dim3(grid_for((long long)n * n)), dim3(256), 0, st, F, goal,
                      frames, n, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_coh(int op, int n, int tile, c32* a, const c32* b, c32* out, const float* d,
+                      float* fr, double* partial, int nblk, float sc, double s0, double s1,
+                      hipStream_t st) {
+  const long long nn = (long long)n * n;
+  const dim3 g(grid_for(nn)), b256(256);
+  switch (op) {
+    case 0: hipLaunchKernelGGL(k_cmul, g, b256, 0, st, a, b, out, nn); break;
+    case 1: hipLaunchKernelGGL(k_rl_divide, g, b256, 0, st, d, b, out, nn, sc); break;
+    case 2: hipLaunchKernelGGL(k_rl_update, g, b256, 0, st, a, b, nn, sc); break;
+    case 3: hipLaunchKernelGGL(k_abs2_sum, dim3(nblk), b256, 0, st, a, nn, partial); break;
+    case 4: hipLaunchKernelGGL(k_div_by_partials, g, b256, 0, st, a, fr, nn, partial, nblk); break;
+    case 5: hipLaunchKernelGGL(k_sq_sum, dim3(nblk), b256, 0, st, fr, nn, partial); break;
+    case 6: hipLaunchKernelGGL(k_flip, g, b256, 0, st, b, out, n, tile); break;
+    case 7: hipLaunchKernelGGL(k_gauss_mul, g, b256, 0, st, b, out, n, tile, s0, s1); break;
+    case 8: hipLaunchKernelGGL(k_gauss_resid, dim3(nblk), b256, 0, st, b, d, nn, partial); break;
+    case 9: hipLaunchKernelGGL(k_rl_finish, dim3(1), b256, 0, st, a, nn, (int)s0, partial, 0);
+            hipLaunchKernelGGL(k_rl_finish, g, b256, 0, st, a, nn, (int)s0, partial, 1); break;
+    case 10: hipLaunchKernelGGL(k_scale_c, g, b256, 0, st, a, nn, sc); break;
+    default: return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
 

@@ -248,6 +248,31 @@ class Plan:
         check(self.lib.spk_rftf_stack(self.handle, _ptr(stack), f, r, c, _ptr(goal), _ptr(out)))
         return out
 
+    def coherence_rl(self, estimate: torch.Tensor, modulus: torch.Tensor, iterations: int,
+                     semantics: str = "numpy", is_fourier: bool = False) -> torch.Tensor:
+        """Richardson-Lucy ipsf (n x n complex64, not centre-shifted) of one frame."""
+        r, c = estimate.shape
+        _expect(estimate, torch.complex64, (r, c), self.ctx.device, "estimate")
+        _expect(modulus, torch.float32, (self.n, self.n), self.ctx.device, "modulus")
+        out = torch.empty((self.n, self.n), dtype=torch.complex64, device=self.ctx.device)
+        check(self.lib.spk_coherence_rl(self.handle, _ptr(estimate), r, c, _ptr(modulus),
+                                        int(iterations), _capi.SEM_CODES[semantics],
+                                        int(bool(is_fourier)), _ptr(out)))
+        return out
+
+    def gauss_prepare(self, estimate: torch.Tensor, modulus: torch.Tensor) -> None:
+        r, c = estimate.shape
+        _expect(estimate, torch.complex64, (r, c), self.ctx.device, "estimate")
+        _expect(modulus, torch.float32, (self.n, self.n), self.ctx.device, "modulus")
+        check(self.lib.spk_coherence_gauss_prepare(self.handle, _ptr(estimate), r, c,
+                                                   _ptr(modulus)))
+
+    def gauss_eval(self, cl0: float, cl1: float) -> float:
+        v = ctypes.c_double()
+        check(self.lib.spk_coherence_gauss_eval(self.handle, float(cl0), float(cl1),
+                                                ctypes.byref(v)))
+        return v.value
+
     def profile(self, enable: bool) -> None:
         check(self.lib.spk_plan_profile(self.handle, int(enable)))
 

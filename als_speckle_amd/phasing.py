@@ -425,8 +425,70 @@ class phasing:
         print("  spectrm %s" % ro[self.spectrum_state])
         print("  device  %s" % self.compute_device)
 
-    def optimize_coherence(self, *args, **kwargs):
-        raise NotImplementedError("partial-coherence optimisation is not built yet")
+    def optimize_coherence(self, best_estimate, modulus=None, force_cpu=False, load=True,
+                           silent=False, iterations=500, method="richardson_lucy",
+                           is_fourier=False, semantics="numpy"):
+        """Estimate the coherence function from reconstructions,
+        ``phasing.py:507-930``, on the GPU.
+
+        best_estimate: 2-d or 3-d (frames) real-space reconstruction(s),
+        embedded top-left in the modulus shape.  method:
+          'richardson_lucy': ``iterations`` Richardson-Lucy updates from the
+            start Gaussian; the ipsf is fft2 of the result (its real part with
+            semantics 'numpy', the reference's CPU branch; complex with
+            'opencl') over |ipsf[0, 0]|.  The reference's result depends only
+            on the last frame (its running sum ``rl_sum`` is never read), so
+            only the last frame is computed.  Loaded as the ipsf if ``load``;
+            returns it centre-shifted, as the reference.
+          'gaussian': per frame, scipy ``fmin`` (ftol 0.1, xtol 0.1, from
+            (n/2, n/2)) of sum (sqrt|fft2(g * ac)| - m)^2 over the Gaussian's
+            two coherence lengths, each objective evaluated on the GPU; the
+            mean lengths' Gaussian is loaded as the ipsf if ``load``; returns
+            the (2, frames) lengths.
+        """
+        assert method in ("richardson_lucy", "gaussian"), "unknown coherence method %s" % method
+        if force_cpu:
+            raise RuntimeError("there is no CPU path; optimize_coherence runs on the GPU")
+        if modulus is None:
+            assert self.modulus_state == 2, "load a modulus or pass one"
+            m_dev = self._modulus_dev
+        else:
+            assert isinstance(modulus, np.ndarray) and modulus.ndim == 2
+            assert modulus.shape[0] == modulus.shape[1]
+            m_dev = to_device(modulus.astype(np.float32), self.ctx.device, torch.float32)
+        n = int(m_dev.shape[0])
+        if isinstance(best_estimate, DeviceArray):
+            best_estimate = best_estimate.get()
+        est = np.asarray(best_estimate)
+        assert est.ndim in (2, 3), "best_estimate must be 2d or 3d"
+        if est.ndim == 2:
+            est = est[None]
+        assert est.shape[1] <= n and est.shape[2] <= n
+        plan = self._plan if (self._plan is not None and self.N == n) else Plan(self.ctx, n)
+        if method == "richardson_lucy":
+            frame = to_device(est[-1].astype(np.complex64), self.ctx.device, torch.complex64)
+            ipsf = plan.coherence_rl(frame, m_dev, int(iterations), semantics, is_fourier)
+            out = ipsf.cpu().numpy()
+            if load:
+                self.load(ipsf=out)
+            return np.fft.fftshift(out)
+        from scipy.optimize import fmin
+        optcls = np.zeros((2, est.shape[0]), np.float32)
+        for k, fr in enumerate(est):
+            plan.gauss_prepare(to_device(fr.astype(np.complex64), self.ctx.device,
+                                         torch.complex64), m_dev)
+            optcls[:, k] = fmin(lambda c: plan.gauss_eval(c[0], c[1]), (n // 2, n // 2),
+                                disp=False, ftol=.1, xtol=0.1)
+        ave = np.average(optcls, axis=1)
+        if not silent:
+            print("recovered coherence lengths")
+            print(ave)
+        if load:
+            rows, cols = np.indices((n, n)) - n // 2
+            rows, cols = np.fft.fftshift(rows), np.fft.fftshift(cols)
+            g = np.exp(-rows ** 2 / (2 * ave[0] ** 2)) * np.exp(-cols ** 2 / (2 * ave[1] ** 2))
+            self.load(ipsf=g)
+        return optcls
 
 
 _default_ctx = None

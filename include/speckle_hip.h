@@ -232,6 +232,24 @@ int spk_rftf(spk_plan* plan, const float* estimate_c64, int rows, int cols,
 int spk_rftf_stack(spk_plan* plan, const float* stack_c64, int frames, int rows,
                    int cols, const float* goal, float* out);
 
+/* Coherence estimation, optimize_coherence (phasing.py:507-930).
+ * Richardson-Lucy (method 'richardson_lucy', phasing.py:700-854): one
+ * estimate frame (rows, cols) c64 embedded top-left, the n x n modulus;
+ * `iterations` updates u *= ifft2(fft2(d / ifft2(fft2(u) P)) Ph) from the
+ * start Gaussian; ipsf = fft2(u) (real part with SPK_SEM_NUMPY, the CPU
+ * branch) / |ipsf(0,0)|, n x n c64 row-major, not centre-shifted.
+ * is_fourier: the estimate is a Fourier-space array (the GPU branch). */
+int spk_coherence_rl(spk_plan* plan, const float* estimate_c64, int rows, int cols,
+                     const float* modulus, int iterations, int semantics, int is_fourier,
+                     float* ipsf_c64);
+/* Gaussian fit (method 'gaussian', phasing.py:657-698, 785-803): prepare the
+ * autocorrelation of one estimate frame, then evaluate the objective
+ * sum (sqrt|fft2(g * ac)| - m)^2 for coherence lengths (cl0, cl1) in pixels
+ * (value: a HOST double; the call waits for the result). */
+int spk_coherence_gauss_prepare(spk_plan* plan, const float* estimate_c64, int rows, int cols,
+                                const float* modulus);
+int spk_coherence_gauss_eval(spk_plan* plan, double cl0, double cl1, double* value);
+
 /* trial average: out (rows, cols) c64 = mean over frames of data; the
  * reference averages aligned trials with np.mean / np.sum on the host
  * (phasing.py:960-961, basic_phasing_example.py:43); FP64 sums */

@@ -655,3 +655,94 @@ def rftf_stack(estimates, goal_modulus):
         d = g - np.abs(np.fft.fft2(new))
         acc += 1.0 / np.sqrt(1.0 + d * d / (g * g))
     return np.fft.fftshift(acc / len(est))
+
+
+# ---------------------------------------------------------------------------
+# optimize_coherence (phasing.py:507-930), CPU branch
+# ---------------------------------------------------------------------------
+def rl_start_gaussian(n):
+    """Initial coherence estimate of the Richardson-Lucy method
+    (``phasing.py:818``): fftshift(shape.gaussian((n, n), (n/4, n/4))),
+    unnormalised, centre n/2 (Python 2 integer division of the shape)."""
+    x = np.arange(n, dtype=float) - n // 2
+    g1 = np.exp(-x ** 2 / (2.0 * (n // 4) ** 2))
+    return np.fft.fftshift(np.outer(g1, g1))
+
+
+def rl_coherence(estimate, modulus, iterations=500, semantics="numpy", is_fourier=False):
+    """Richardson-Lucy coherence estimate of ONE frame (``phasing.py:702-837``).
+
+    p = |fft2(estimate embedded top-left)|^2 / sum (or |estimate|^2 when
+    is_fourier, the GPU branch), P = fft2(p), Ph = P[::-1, ::-1] (a plain
+    reversal, ``common_flip_f2.cl``), d = m^2 / sum(m^2), u = rl_start_gaussian;
+    per iteration u *= ifft2(fft2(d / ifft2(fft2(u) P)) Ph); then u = fft2(u),
+    real part with numpy semantics (the CPU branch casts to float32,
+    ``phasing.py:778-783``), divided by |u[0, 0]| (``phasing.py:641-643``).
+    The reference returns fftshift of this and loads the unshifted array;
+    only the last frame of a stack determines it (``rl_sum`` is never read).
+    Returns the unshifted ipsf.
+    """
+    m = np.asarray(modulus, np.float64)
+    n = m.shape[0]
+    est = np.asarray(estimate)
+    if is_fourier:
+        act = np.zeros((n, n), np.complex128)
+        act[:est.shape[0], :est.shape[1]] = est
+        p = np.abs(act) ** 2
+    else:
+        act = np.zeros((n, n), np.complex128)
+        act[:est.shape[0], :est.shape[1]] = est
+        p = np.abs(np.fft.fft2(act)) ** 2
+    p = p / p.sum()
+    P = np.fft.fft2(p)
+    Ph = P[::-1, ::-1]
+    d = m ** 2
+    d = d / d.sum()
+    u = rl_start_gaussian(n).astype(np.complex128)
+    for _ in range(iterations):
+        blur = np.fft.ifft2(np.fft.fft2(u) * P)
+        blur = d / blur
+        blur = np.fft.ifft2(np.fft.fft2(blur) * Ph)
+        u = u * blur
+    u = np.fft.fft2(u)
+    if semantics == "numpy":
+        u = u.real
+    return u / abs(u[0, 0])
+
+
+def gaussian_coherence_objective(ac, m, cl):
+    """``_opt_iter_g`` (``phasing.py:657-698``): sum (sqrt|fft2(g * ac)| - m)^2
+    with g = exp(-rows^2/(2 cl0^2)) exp(-cols^2/(2 cl1^2)) on fftshifted
+    signed coordinates."""
+    n = m.shape[0]
+    rows, cols = np.indices((n, n)) - n // 2
+    rows, cols = np.fft.fftshift(rows), np.fft.fftshift(cols)
+    g = np.exp(-rows ** 2 / (2.0 * cl[0] ** 2)) * np.exp(-cols ** 2 / (2.0 * cl[1] ** 2))
+    bl = np.sqrt(np.abs(np.fft.fft2(g * ac)))
+    return float(np.sum(np.abs(bl - m) ** 2))
+
+
+def autocorrelation(estimate, n):
+    """``_preprocess_g`` (``phasing.py:785-803``): ifft2(|fft2(active)|^2)."""
+    est = np.asarray(estimate)
+    act = np.zeros((n, n), np.complex128)
+    act[:est.shape[0], :est.shape[1]] = est
+    return np.fft.ifft2(np.abs(np.fft.fft2(act)) ** 2)
+
+
+def gaussian_coherence(estimates, modulus):
+    """``optimize_coherence(method='gaussian')``: per frame, scipy ``fmin``
+    from (n/2, n/2) with ftol 0.1, xtol 0.1 (``phasing.py:897-906``); returns
+    the (2, frames) coherence lengths."""
+    from scipy.optimize import fmin
+    m = np.asarray(modulus, np.float64)
+    n = m.shape[0]
+    est = np.asarray(estimates)
+    if est.ndim == 2:
+        est = est[None]
+    out = np.zeros((2, est.shape[0]))
+    for k, fr in enumerate(est):
+        ac = autocorrelation(fr, n)
+        out[:, k] = fmin(lambda c: gaussian_coherence_objective(ac, m, c), (n // 2, n // 2),
+                         disp=False, ftol=.1, xtol=0.1)
+    return out

@@ -221,6 +221,45 @@ def test_radial_rescale_closed_forms():
     assert np.allclose(r, [0.9, 1.0, 1.1]) and np.allclose(w, [0.2, 1.0, 0.3])
 
 
+def test_richardson_lucy_closed_forms():
+    rng = np.random.default_rng(41)
+    n = 16
+    est = crand((n, n), rng)
+    # the start is the fftshifted unnormalised Gaussian of width n/4
+    g = O.rl_start_gaussian(n)
+    assert g[0, 0] == 1.0 and np.isclose(g[0, 4], np.exp(-0.5))
+    # zero iterations: the ipsf is fft2 of the start, real part, over |.[0,0]|
+    want = np.fft.fft2(g).real
+    assert np.allclose(O.rl_coherence(est, np.abs(crand((n, n), rng)), 0), want / abs(want[0, 0]))
+    # a datum consistent with the start, m^2 proportional to u (*) p, makes
+    # d / (u (*) p) a constant; with the plain reversal Ph = P[::-1, ::-1] the
+    # correction ifft2(fft2(const) Ph) is const * P[n-1, n-1] (the proper
+    # adjoint would give const * sum(p)), so one iteration multiplies u by a
+    # complex constant whose phase survives the |u[0,0]| normalisation
+    p = np.abs(np.fft.fft2(est)) ** 2
+    p /= p.sum()
+    P = np.fft.fft2(p)
+    conv = np.fft.ifft2(np.fft.fft2(g) * P).real     # u (*) p, real and > 0
+    m = np.sqrt(conv) * 3.0                           # the scale cancels in d
+    one = O.rl_coherence(est, m, 1, semantics="opencl")
+    want = np.fft.fft2(g * P[n - 1, n - 1])
+    assert np.allclose(one, want / abs(want[0, 0]), atol=1e-9)
+
+
+def test_gaussian_coherence_objective_limits():
+    rng = np.random.default_rng(42)
+    n = 16
+    est = crand((n, n), rng)
+    m = np.abs(crand((n, n), rng))
+    ac = O.autocorrelation(est, n)
+    # very long coherence: g = 1, sqrt|fft2(ac)| = |fft2(est)|
+    far = O.gaussian_coherence_objective(ac, m, (1e9, 1e9))
+    assert np.isclose(far, np.sum((np.abs(np.fft.fft2(est)) - m) ** 2))
+    # very short coherence: g = delta at 0, a flat pattern of sqrt(sum|est|^2)
+    near = O.gaussian_coherence_objective(ac, m, (1e-3, 1e-3))
+    assert np.isclose(near, np.sum((np.sqrt(np.sum(np.abs(est) ** 2)) - m) ** 2))
+
+
 def test_modulus_error_closed_form():
     Psi = np.array([[3 + 4j, 0], [1, -2j]])
     m = np.array([[4.0, 1.0], [1.0, 1.0]])
