@@ -69,8 +69,9 @@ SIGNATURES = {
     "spk_stack_mean": (c_int, [c_void_p, c_void_p, c_int, c_int, c_int, c_void_p]),
     "spk_blur2d": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_int, c_float]),
     "spk_prtf": (c_int, [c_void_p, c_void_p, c_int, c_int, c_int, c_void_p]),
-    "spk_rftf": (c_int, [c_void_p, c_void_p, c_int, c_int, c_void_p, c_int, c_void_p]),
-    "spk_rftf_stack": (c_int, [c_void_p, c_void_p, c_int, c_int, c_int, c_void_p, c_void_p]),
+    "spk_rftf": (c_int, [c_void_p, c_void_p, c_int, c_int, c_void_p, c_void_p, c_int, c_void_p]),
+    "spk_rftf_stack": (c_int, [c_void_p, c_void_p, c_int, c_int, c_int, c_void_p, c_void_p,
+                               c_void_p]),
     "spk_coherence_rl": (c_int, [c_void_p, c_void_p, c_int, c_int, c_void_p, c_int, c_int, c_int,
                                  c_void_p]),
     "spk_coherence_gauss_prepare": (c_int, [c_void_p, c_void_p, c_int, c_int, c_void_p]),

@@ -1061,8 +1061,31 @@ int spk_prtf(spk_plan* p, const float* stack, int frames, int rows, int cols, fl
   return SPK_OK;
 }
 
-int spk_rftf(spk_plan* p, const float* est, int rows, int cols, const float* goal, int scale,
-             float* out) {
+namespace {
+// partially coherent speckle of the RFTF functions: F <- sqrt(|ifft2(fft2(A)
+// * ipsf)|) with A = |F| (rftf, phasing.py:1567-1573) or |F|^2 (gpu_rftf,
+// phasing.py:1676-1690), per frame; ipsf n x n c64 row-major; work: one frame
+int rftf_blur(spk_plan* p, c32* F, int frames, const c32* ipsf, int squared, c32* work) {
+  const int n = p->n;
+  const size_t nn = (size_t)p->nn();
+  hipStream_t st = p->ctx->stream;
+  for (int f = 0; f < frames; ++f) {
+    c32* x = F + (size_t)f * nn;
+    SPK_HIP(launch_coh(11, n, p->tile, x, nullptr, nullptr, nullptr, nullptr, nullptr, 0, 0,
+                       squared ? 1.0 : 0.0, 0, st));
+    int rc = spk_fft2(p, (const float*)x, (float*)work, 1, 0);
+    if (rc) return rc;
+    SPK_HIP(launch_coh(0, n, p->tile, work, ipsf, work, nullptr, nullptr, nullptr, 0, 0, 0, 0, st));
+    if ((rc = spk_fft2(p, (const float*)work, (float*)x, 1, 1))) return rc;
+    SPK_HIP(launch_coh(11, n, p->tile, x, nullptr, nullptr, nullptr, nullptr, nullptr, 0, 0, 2.0, 0,
+                       st));
+  }
+  return SPK_OK;
+}
+}  // namespace
+
+int spk_rftf(spk_plan* p, const float* est, int rows, int cols, const float* goal,
+             const float* ipsf, int scale, float* out) {
   SPK_REQUIRE(p && est && goal && out, SPK_EINVAL, "bad arguments");
   SPK_REQUIRE(rows >= 1 && cols >= 1 && rows <= p->n && cols <= p->n, SPK_EINVAL,
               "estimate larger than the goal modulus");
@@ -1078,12 +1101,13 @@ int spk_rftf(spk_plan* p, const float* est, int rows, int cols, const float* goa
   double* part = (double*)(F + nn);
   SPK_HIP(launch_embed((const c32*)est, pad, rows, cols, p->n, 1, p->ctx->stream));
   if ((rc = spk_fft2(p, (const float*)pad, (float*)F, 1, 0))) return rc;
+  if (ipsf && (rc = rftf_blur(p, F, 1, (const c32*)ipsf, 0, pad))) return rc;
   SPK_HIP(launch_rftf(F, goal, p->n, part, nblk, scale, out, p->ctx->stream));
   return SPK_OK;
 }
 
 int spk_rftf_stack(spk_plan* p, const float* stack, int frames, int rows, int cols,
-                   const float* goal, float* out) {
+                   const float* goal, const float* ipsf, float* out) {
   SPK_REQUIRE(p && stack && goal && out && frames >= 1, SPK_EINVAL, "bad arguments");
   SPK_REQUIRE(rows >= 1 && cols >= 1 && rows <= p->n && cols <= p->n, SPK_EINVAL,
               "estimates larger than the goal modulus");
@@ -1096,6 +1120,7 @@ int spk_rftf_stack(spk_plan* p, const float* stack, int frames, int rows, int co
   c32* F = pad + frames * nn;
   SPK_HIP(launch_embed((const c32*)stack, pad, rows, cols, p->n, frames, p->ctx->stream));
   if ((rc = spk_fft2(p, (const float*)pad, (float*)F, frames, 0))) return rc;
+  if (ipsf && (rc = rftf_blur(p, F, frames, (const c32*)ipsf, 1, pad))) return rc;
   SPK_HIP(launch_rftf_stack(F, goal, frames, p->n, out, p->ctx->stream));
   return SPK_OK;
 }

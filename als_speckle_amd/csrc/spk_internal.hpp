@@ -109,7 +109,8 @@ hipError_t launch_rescale(int n, int tile, const float* imap, float* dmap, const
 // 2 a *= b*scale; 3 a = |a|^2 with partial sums; 4 a (or fr) /= sum of
 // partials; 5 fr = fr^2 with partial sums; 6 out = reversal of b (tiled);
 // 7 out = b * Gaussian(s0, s1) (b = nullptr: the Gaussian); 8 partial sums
-// of (sqrt|b| - d)^2; 9 a = a (real part if s0 != 0) / |a(0,0)|; 10 a *= sc)
+// of (sqrt|b| - d)^2; 9 a = a (real part if s0 != 0) / |a(0,0)|; 10 a *= sc;
+// 11 a = |a|, |a|^2 or sqrt|a| for s0 = 0, 1, 2)
 hipError_t launch_coh(int op, int n, int tile, c32* a, const c32* b, c32* out, const float* d,
                       float* fr, double* partial, int nblk, float scale, double s0, double s1,
                       hipStream_t st);

@@ -1221,6 +1221,16 @@ static __device__ __forceinline__ long long tiled_pos(int n, int tt, int r, int 
   return ((long long)(r / tt) * (n / tt) + c / tt) * (tt * tt) + (r % tt) * tt + (c % tt);
 }
 
+// a = |a|, |a|^2 or sqrt|a| (mode 0, 1, 2), imaginary part 0
+static __global__ void __launch_bounds__(256) k_abs_mode(c32* a, long long nn, int mode) {
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < nn;
+       i += (long long)gridDim.x * 256) {
+    const float p2 = fmaf(a[i].x, a[i].x, a[i].y * a[i].y);
+    const float v = mode == 1 ? p2 : mode == 0 ? sqrtf(p2) : sqrtf(sqrtf(p2));
+    a[i] = c32{v, 0.0f};
+  }
+}
+
 // a *= sc
 static __global__ void __launch_bounds__(256) k_scale_c(c32* a, long long nn, float sc) {
   for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < nn;
@@ -1919,6 +1929,7 @@ hipError_t launch_coh(int op, int n, int tile, c32* a, const c32* b, c32* out, c
     case 9: hipLaunchKernelGGL(k_rl_finish, dim3(1), b256, 0, st, a, nn, (int)s0, partial, 0);
             hipLaunchKernelGGL(k_rl_finish, g, b256, 0, st, a, nn, (int)s0, partial, 1); break;
     case 10: hipLaunchKernelGGL(k_scale_c, g, b256, 0, st, a, nn, sc); break;
+    case 11: hipLaunchKernelGGL(k_abs_mode, g, b256, 0, st, a, nn, (int)s0); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();

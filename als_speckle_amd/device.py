@@ -231,21 +231,28 @@ class Plan:
         check(self.lib.spk_prtf(self.handle, _ptr(stack), f, r, c, _ptr(out)))
         return out
 
-    def rftf(self, estimate: torch.Tensor, goal: torch.Tensor, scale: bool = False):
+    def rftf(self, estimate: torch.Tensor, goal: torch.Tensor, scale: bool = False,
+             ipsf: torch.Tensor | None = None):
         r, c = estimate.shape
         _expect(estimate, torch.complex64, (r, c), self.ctx.device, "estimate")
         _expect(goal, torch.float32, (self.n, self.n), self.ctx.device, "goal_modulus")
+        if ipsf is not None:
+            _expect(ipsf, torch.complex64, (self.n, self.n), self.ctx.device, "ipsf")
         out = torch.empty((self.n, self.n), dtype=torch.float32, device=self.ctx.device)
-        check(self.lib.spk_rftf(self.handle, _ptr(estimate), r, c, _ptr(goal), int(scale),
-                                _ptr(out)))
+        check(self.lib.spk_rftf(self.handle, _ptr(estimate), r, c, _ptr(goal),
+                                _ptr(ipsf) if ipsf is not None else None, int(scale), _ptr(out)))
         return out
 
-    def rftf_stack(self, stack: torch.Tensor, goal: torch.Tensor) -> torch.Tensor:
+    def rftf_stack(self, stack: torch.Tensor, goal: torch.Tensor,
+                   ipsf: torch.Tensor | None = None) -> torch.Tensor:
         f, r, c = stack.shape
         _expect(stack, torch.complex64, (f, r, c), self.ctx.device, "estimates")
         _expect(goal, torch.float32, (self.n, self.n), self.ctx.device, "goal_modulus")
+        if ipsf is not None:
+            _expect(ipsf, torch.complex64, (self.n, self.n), self.ctx.device, "ipsf")
         out = torch.empty((self.n, self.n), dtype=torch.float32, device=self.ctx.device)
-        check(self.lib.spk_rftf_stack(self.handle, _ptr(stack), f, r, c, _ptr(goal), _ptr(out)))
+        check(self.lib.spk_rftf_stack(self.handle, _ptr(stack), f, r, c, _ptr(goal),
+                                      _ptr(ipsf) if ipsf is not None else None, _ptr(out)))
         return out
 
     def coherence_rl(self, estimate: torch.Tensor, modulus: torch.Tensor, iterations: int,

@@ -607,25 +607,31 @@ def prtf(estimates, N=None, silent=True, prtfq=False):
 def rftf(estimate, goal_modulus, hot_pixels=False, ipsf=None, rftfq=False, scale=False):
     """Fourier-error transfer function, ``phasing.py:1521-1609``.
 
-    |F| of the estimate zero-padded into the goal's shape is compared with the
-    goal modulus at the corner and the centre alignment (the closer one is
-    used, ``phasing.py:1564-1573``); returns sqrt(1/(1+err)) with err =
-    (|F| - g)^2 / g^2, centre-shifted, optionally rescaled from
-    [1/sqrt(2), 1] to [0, 1].  The goal must be square with a power-of-2 side
-    in [16, 4096].  ``hot_pixels``, ``ipsf`` and ``rftfq`` are not provided.
+    |F| of the estimate zero-padded into the goal's shape (with an ipsf:
+    sqrt(|ifft2(fft2(|F|) * ipsf)|), as the reference blurs the modulus) is
+    compared with the goal modulus at the corner and the centre alignment
+    (the closer one is used, ``phasing.py:1564-1573``); returns
+    sqrt(1/(1+err)) with err = (|F| - g)^2 / g^2, centre-shifted, optionally
+    rescaled from [1/sqrt(2), 1] to [0, 1].  The goal must be square with a
+    power-of-2 side in [16, 4096].  ``hot_pixels`` (the conditioning module)
+    and ``rftfq`` (``wrapping.unwrap``) are not provided.
     """
     assert isinstance(estimate, np.ndarray), "estimate must be ndarray"
     assert isinstance(goal_modulus, np.ndarray), "goal_modulus must be ndarray"
     assert estimate.shape <= goal_modulus.shape, "estimate must be smaller than goal_modulus"
-    if hot_pixels or ipsf is not None or rftfq:
-        raise NotImplementedError("hot_pixels, ipsf and rftfq are not provided")
+    if hot_pixels or rftfq:
+        raise NotImplementedError("hot_pixels and rftfq are not provided")
     n = goal_modulus.shape[0]
     assert goal_modulus.shape == (n, n), "goal_modulus must be square"
+    if ipsf is not None:
+        assert isinstance(ipsf, np.ndarray) and ipsf.ndim == 2
+        assert ipsf.shape == goal_modulus.shape
     plan = _post_plan(n)
     dev = plan.ctx.device
     e = to_device(estimate.astype(np.complex64), dev, torch.complex64)
     g = to_device(goal_modulus.astype(np.float32), dev, torch.float32)
-    return plan.rftf(e, g, scale=bool(scale)).cpu().numpy()
+    ip = None if ipsf is None else to_device(ipsf.astype(np.complex64), dev, torch.complex64)
+    return plan.rftf(e, g, scale=bool(scale), ipsf=ip).cpu().numpy()
 
 
 def gpu_prtf(gpuinfo, estimates, N=None, prtfq=False):
@@ -639,16 +645,22 @@ def gpu_rftf(gpuinfo, estimates, goal_modulus, ipsf=None, rftfq=False):
     Unlike ``rftf`` this averages the per-frame value
     1/sqrt(1 + (g - |F_f|)^2 / g^2) over the estimates (frames, r, c), with
     the goal taken at the corner (no alignment search), then centre-shifts.
-    ``gpuinfo`` is accepted for the signature and ignored.  ``ipsf`` and
-    ``rftfq`` are not provided.
+    With an ipsf |F_f| becomes sqrt(|ifft2(fft2(|F_f|^2) * Re ipsf)|): the
+    reference blurs the intensity here and uploads the ipsf as float32 (its
+    real part).  ``gpuinfo`` is accepted for the signature and ignored;
+    ``rftfq`` is not provided.
     """
     assert isinstance(estimates, np.ndarray) and estimates.ndim == 3
     assert isinstance(goal_modulus, np.ndarray) and goal_modulus.ndim == 2
     assert goal_modulus.shape[0] == goal_modulus.shape[1]
-    if ipsf is not None or rftfq:
-        raise NotImplementedError("ipsf and rftfq are not provided")
+    if rftfq:
+        raise NotImplementedError("rftfq is not provided")
+    if ipsf is not None:
+        assert isinstance(ipsf, np.ndarray) and ipsf.shape == goal_modulus.shape
     plan = _post_plan(goal_modulus.shape[0])
     dev = plan.ctx.device
     e = to_device(estimates.astype(np.complex64), dev, torch.complex64)
     g = to_device(goal_modulus.astype(np.float32), dev, torch.float32)
-    return plan.rftf_stack(e, g).cpu().numpy()
+    ip = None if ipsf is None else to_device(np.real(ipsf).astype(np.complex64), dev,
+                                              torch.complex64)
+    return plan.rftf_stack(e, g, ipsf=ip).cpu().numpy()

@@ -218,19 +218,23 @@ int spk_blur2d(spk_ctx* ctx, const float* in, float* out, int rows, int cols,
 int spk_prtf(spk_plan* plan, const float* stack_c64, int frames, int rows,
              int cols, float* out);
 
-/* RFTF (phasing.py:1521-1609 without hot-pixel removal, ipsf or q-average):
+/* RFTF (phasing.py:1521-1609 without hot-pixel removal or q-average):
  * estimate (rows, cols) c64 zero-padded into n x n, |F| compared with the
  * goal modulus at both alignments, out = sqrt(1/(1+err)) centre-shifted as
- * the reference returns it; scale != 0 maps [1/sqrt(2), 1] to [0, 1]. */
+ * the reference returns it; scale != 0 maps [1/sqrt(2), 1] to [0, 1].
+ * ipsf (n x n c64, or NULL): |F| is first replaced by
+ * sqrt(|ifft2(fft2(|F|) * ipsf)|) (phasing.py:1567-1573). */
 int spk_rftf(spk_plan* plan, const float* estimate_c64, int rows, int cols,
-             const float* goal, int scale, float* out);
+             const float* goal, const float* ipsf_c64, int scale, float* out);
 
 /* gpu_rftf (phasing.py:1611-1708, phasing_rftf_accumulate.cl:15-24): the
  * per-frame value 1/sqrt(1 + (g - |F_f|)^2 / g^2) with the goal at the corner,
  * averaged over the stack (frames, rows, cols) c64 zero-padded into n x n;
- * out n x n float, centre-shifted as the reference returns it. */
+ * out n x n float, centre-shifted as the reference returns it.  ipsf (or
+ * NULL): |F_f| is first replaced by sqrt(|ifft2(fft2(|F_f|^2) * ipsf)|)
+ * (phasing.py:1676-1690). */
 int spk_rftf_stack(spk_plan* plan, const float* stack_c64, int frames, int rows,
-                   int cols, const float* goal, float* out);
+                   int cols, const float* goal, const float* ipsf_c64, float* out);
 
 /* Coherence estimation, optimize_coherence (phasing.py:507-930).
  * Richardson-Lucy (method 'richardson_lucy', phasing.py:700-854): one

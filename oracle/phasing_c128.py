@@ -614,18 +614,22 @@ def prtf(estimates, N=None):
     return np.fft.fftshift(np.abs(acc / f))
 
 
-def rftf(estimate, goal_modulus, scale=False):
-    """RFTF, ``phasing.py:1521-1586`` (no ipsf, no hot-pixel removal).
+def rftf(estimate, goal_modulus, scale=False, ipsf=None):
+    """RFTF, ``phasing.py:1521-1586`` (no hot-pixel removal).
 
-    Zero-pads the estimate into the modulus shape, compares |F| with the goal
-    at both corner and centre alignments (``phasing.py:1564-1573``) and returns
-    sqrt(1/(1+err)), centre-shifted.
+    Zero-pads the estimate into the modulus shape; with an ipsf |F| becomes
+    sqrt(|ifft2(fft2(|F|) * ipsf)|) (``phasing.py:1567-1573``: the modulus, not
+    the intensity, is blurred); compares it with the goal at both corner and
+    centre alignments (``phasing.py:1564-1573``) and returns sqrt(1/(1+err)),
+    centre-shifted.
     """
     est = np.asarray(estimate)
     g = np.asarray(goal_modulus, np.float64)
     new = np.zeros(g.shape, np.complex128)
     new[:est.shape[0], :est.shape[1]] = est
     fourier = np.abs(np.fft.fft2(new))
+    if ipsf is not None:
+        fourier = np.sqrt(np.abs(np.fft.ifft2(np.fft.fft2(fourier) * ipsf)))
     shift = np.fft.fftshift
     d1 = np.sum(np.abs(fourier - g))
     d2 = np.sum(np.abs(shift(fourier) - g))
@@ -640,19 +644,24 @@ def rftf(estimate, goal_modulus, scale=False):
     return out
 
 
-def rftf_stack(estimates, goal_modulus):
+def rftf_stack(estimates, goal_modulus, ipsf=None):
     """Stack RFTF of ``gpu_rftf``, ``phasing.py:1611-1708`` with
     ``phasing_rftf_accumulate.cl:15-24``: per pixel, the mean over frames of
-    1/sqrt(1 + (g - |F_f|)^2 / g^2), goal at the corner, then fftshift.
+    1/sqrt(1 + (g - |F_f|)^2 / g^2), goal at the corner, then fftshift.  With
+    an ipsf |F_f| becomes sqrt(|ifft2(fft2(|F_f|^2) * Re ipsf)|): the GPU
+    branch blurs the intensity and uploads the ipsf as float32, i.e. its real
+    part (``phasing.py:1668-1690``).
     """
     est = np.asarray(estimates)
     g = np.asarray(goal_modulus, np.float64)
-    n = g.shape[0]
     acc = np.zeros(g.shape, np.float64)
     for e in est:
         new = np.zeros(g.shape, np.complex128)
         new[:e.shape[0], :e.shape[1]] = e
-        d = g - np.abs(np.fft.fft2(new))
+        fa = np.abs(np.fft.fft2(new))
+        if ipsf is not None:
+            fa = np.sqrt(np.abs(np.fft.ifft2(np.fft.fft2(fa ** 2) * np.real(ipsf))))
+        d = g - fa
         acc += 1.0 / np.sqrt(1.0 + d * d / (g * g))
     return np.fft.fftshift(acc / len(est))
 

@@ -135,3 +135,22 @@ def test_gpu_rftf_stack_matches_oracle():
     # one frame at the corner alignment equals rftf
     one = P.gpu_rftf(None, est[:1], g)
     assert np.max(np.abs(one - O.rftf(est[0], g))) < 2e-5
+
+
+@pytest.mark.parametrize("stack", [False, True])
+def test_rftf_with_ipsf_matches_oracle(stack):
+    rng = np.random.default_rng(11)
+    obj, s = two_part_object(64, rng)
+    g = np.abs(np.fft.fft2(obj)) * (1 + 0.2 * rng.random((64, 64))) + 0.5
+    x = np.fft.fftfreq(64) * 64
+    psf = np.exp(-(x[:, None] ** 2 + x[None, :] ** 2) / 4.0)
+    ipsf = np.fft.fft2(psf / psf.sum()) * np.exp(0.2j)     # complex, to see Re taken
+    if stack:
+        est = np.stack([(obj + 0.05 * crand(obj.shape, rng))[:50, :50] for _ in range(3)])
+        got = P.gpu_rftf(None, est, g, ipsf=ipsf)
+        want = O.rftf_stack(est, g, ipsf=ipsf)
+    else:
+        est = (obj + 0.05 * crand(obj.shape, rng))[:50, :56]
+        got = P.rftf(est, g, ipsf=ipsf)
+        want = O.rftf(est, g, ipsf=ipsf)
+    assert np.max(np.abs(got - want)) < 2e-5

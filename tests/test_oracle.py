@@ -410,6 +410,24 @@ def test_rftf_alignment_and_scale():
     assert np.allclose(O.rftf_stack(np.stack([a, b]), g), want)
 
 
+def test_rftf_ipsf_closed_forms():
+    obj, s, m = known_object(32)
+    g = m + 1e-3
+    ones = np.ones((32, 32))
+    # gpu_rftf blurs the intensity: a delta point-spread function changes nothing
+    assert np.allclose(O.rftf_stack(obj[None], g, ipsf=ones), O.rftf_stack(obj[None], g))
+    # rftf blurs the modulus and takes a square root (phasing.py:1573): with a
+    # delta point-spread function the compared pattern is sqrt|F|
+    fa = np.abs(np.fft.fft2(obj))
+    sq = np.sqrt(fa)
+    goal = sq * 2 + 1e-30                     # err = 1/4 against the sqrt|F| pattern
+    assert np.allclose(O.rftf(obj, goal, ipsf=ones), np.sqrt(1 / 1.25))
+    # only the real part of the ipsf enters the stack form
+    ip = np.fft.fft2(np.exp(-np.arange(32)[:, None] ** 2 / 3.0) * np.ones((1, 32)))
+    assert np.allclose(O.rftf_stack(obj[None], g, ipsf=ip * np.exp(0.7j)),
+                       O.rftf_stack(obj[None], g, ipsf=(ip * np.exp(0.7j)).real))
+
+
 def test_blur_kernel_odd_size_peak():
     # centre at n//2 then fftshift: the peak lands at n - 1 for odd n
     k = O.gaussian_kernel((9, 7), 1.5)
