@@ -67,6 +67,7 @@ SIGNATURES = {
     "spk_align_global_phase": (c_int, [c_void_p, c_void_p, c_int, c_int, c_int]),
     "spk_copy_bytes": (c_int, [c_void_p, c_void_p, c_void_p, c_size_t]),
     "spk_stack_mean": (c_int, [c_void_p, c_void_p, c_int, c_int, c_int, c_void_p]),
+    "spk_com_average": (c_int, [c_void_p, c_void_p, c_int, c_int, c_int, c_void_p]),
     "spk_blur2d": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_int, c_float]),
     "spk_prtf": (c_int, [c_void_p, c_void_p, c_int, c_int, c_int, c_void_p]),
     "spk_rftf": (c_int, [c_void_p, c_void_p, c_int, c_int, c_void_p, c_void_p, c_int, c_void_p]),

@@ -1268,6 +1268,49 @@ int spk_coherence_gauss_eval(spk_plan* p, double cl0, double cl1, double* value)
   return SPK_OK;
 }
 
+int spk_com_average(spk_ctx* ctx, const float* data, int frames, int rows, int cols,
+                    float* out) {
+  SPK_REQUIRE(data && out && frames >= 1 && rows >= 1 && cols >= 1, SPK_EINVAL, "bad stack");
+  int rc = set_device(ctx);
+  if (rc) return rc;
+  const long long per = (long long)rows * cols;
+  const int nblk = (int)std::min<long long>(64, (per + 255) / 256);
+  void* scr;
+  if ((rc = ctx_scratch(ctx, (size_t)frames * nblk * 3 * sizeof(double) +
+                                 (size_t)2 * frames * sizeof(int), &scr)))
+    return rc;
+  double* part = (double*)scr;
+  int* dshift = (int*)(part + (size_t)frames * nblk * 3);
+  hipStream_t st = ctx->stream;
+  SPK_HIP(launch_com_partial((const c32*)data, frames, rows, cols, part, nblk, st));
+  std::vector<double> h((size_t)frames * nblk * 3);
+  SPK_HIP(hipMemcpyAsync(h.data(), part, h.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+  SPK_HIP(hipStreamSynchronize(st));
+  // centre of mass, truncated to int (phasing.py:1731-1734), relative to frame 0
+  std::vector<int> shift(2 * (size_t)frames);
+  int r0 = 0, c0 = 0;
+  for (int f = 0; f < frames; ++f) {
+    double s0 = 0.0, sr = 0.0, sc = 0.0;
+    for (int b = 0; b < nblk; ++b) {
+      const double* q = &h[((size_t)f * nblk + b) * 3];
+      s0 += q[0];
+      sr += q[1];
+      sc += q[2];
+    }
+    SPK_REQUIRE(s0 > 0.0, SPK_EINVAL, "a frame of the stack is all zero");
+    const int r = (int)(sr / s0), c = (int)(sc / s0);
+    if (f == 0) { r0 = r; c0 = c; }
+    shift[2 * f] = r - r0;
+    shift[2 * f + 1] = c - c0;
+  }
+  // synchronous: the host vector goes out of scope at return
+  SPK_HIP(hipMemcpyAsync(dshift, shift.data(), shift.size() * sizeof(int), hipMemcpyHostToDevice,
+                         st));
+  SPK_HIP(launch_roll_sum((const c32*)data, frames, rows, cols, dshift, (c32*)out, st));
+  SPK_HIP(hipStreamSynchronize(st));
+  return SPK_OK;
+}
+
 int spk_stack_mean(spk_ctx* ctx, const float* data, int frames, int rows, int cols,
                    float* out) {
   SPK_REQUIRE(data && out && frames >= 1 && rows >= 1 && cols >= 1, SPK_EINVAL, "bad stack");

@@ -114,6 +114,12 @@ hipError_t launch_rescale(int n, int tile, const float* imap, float* dmap, const
 hipError_t launch_coh(int op, int n, int tile, c32* a, const c32* b, c32* out, const float* d,
                       float* fr, double* partial, int nblk, float scale, double s0, double s1,
                       hipStream_t st);
+// center of mass average: per-frame FP64 sums of |x|, row |x|, col |x|
+// (partial [frame][block][3]) and the sum of rolled frames
+hipError_t launch_com_partial(const c32* data, int frames, int rows, int cols, double* partial,
+                              int nblk, hipStream_t st);
+hipError_t launch_roll_sum(const c32* data, int frames, int rows, int cols, const int* shift,
+                           c32* out, hipStream_t st);
 // workgroups of the persistent kernel that fit on the current device at once
 hipError_t persist_grid(int n, int tile, int ept, int* out);
 // mode 0: out = rowFFT(in); 1: out = rowIFFT(in) (unscaled);

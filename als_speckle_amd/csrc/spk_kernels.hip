@@ -1372,6 +1372,52 @@ static __global__ void __launch_bounds__(256) k_rl_finish(c32* u, long long nn, 
 }
 
 // ---------------------------------------------------------------------------
+// center_of_mass_average (phasing.py:1710-1745): per frame the FP64 sums of
+// |img|, row * |img| and col * |img| (block partials [frame][block][3]), then
+// out = sum_f roll(img_f, -(dr_f, dc_f)) with FP64 accumulation.
+// ---------------------------------------------------------------------------
+static __global__ void __launch_bounds__(256) k_com_partial(const c32* data, int rows, int cols,
+                                                            double* partial, int nblk) {
+  __shared__ double sh[256];
+  const long long per = (long long)rows * cols;
+  const c32* d = data + (size_t)blockIdx.y * per;
+  double s0 = 0.0, sr = 0.0, sc = 0.0;
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < per;
+       i += (long long)nblk * 256) {
+    const double a = sqrt((double)d[i].x * d[i].x + (double)d[i].y * d[i].y);
+    s0 += a;
+    sr += a * (double)(i / cols);
+    sc += a * (double)(i % cols);
+  }
+  double* out = partial + ((size_t)blockIdx.y * nblk + blockIdx.x) * 3;
+  const double t0 = block_sum(s0, sh);
+  if (threadIdx.x == 0) out[0] = t0;
+  const double t1 = block_sum(sr, sh);
+  if (threadIdx.x == 0) out[1] = t1;
+  const double t2 = block_sum(sc, sh);
+  if (threadIdx.x == 0) out[2] = t2;
+}
+
+static __global__ void __launch_bounds__(256) k_roll_sum(const c32* data, int frames, int rows,
+                                                         int cols, const int* shift, c32* out) {
+  const long long per = (long long)rows * cols;
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < per;
+       i += (long long)gridDim.x * 256) {
+    const int r = (int)(i / cols), c = (int)(i % cols);
+    double re = 0.0, im = 0.0;
+    for (int f = 0; f < frames; ++f) {
+      int rr = (r + shift[2 * f]) % rows, cc = (c + shift[2 * f + 1]) % cols;
+      if (rr < 0) rr += rows;
+      if (cc < 0) cc += cols;
+      const c32 v = data[(size_t)f * per + (size_t)rr * cols + cc];
+      re += v.x;
+      im += v.y;
+    }
+    out[i] = c32{(float)re, (float)im};
+  }
+}
+
+// ---------------------------------------------------------------------------
 // mean over a stack of frames (the trial average, phasing.py:960-961):
 // out[i] = (1/frames) sum_f data[f][i], summed in FP64 in frame order
 // ---------------------------------------------------------------------------
@@ -1932,6 +1978,20 @@ hipError_t launch_coh(int op, int n, int tile, c32* a, const c32* b, c32* out, c
     case 11: hipLaunchKernelGGL(k_abs_mode, g, b256, 0, st, a, nn, (int)s0); break;
     default: return hipErrorInvalidValue;
   }
+  return hipGetLastError();
+}
+
+hipError_t launch_com_partial(const c32* data, int frames, int rows, int cols, double* partial,
+                              int nblk, hipStream_t st) {
+  hipLaunchKernelGGL(k_com_partial, dim3(nblk, frames), dim3(256), 0, st, data, rows, cols,
+                     partial, nblk);
+  return hipGetLastError();
+}
+
+hipError_t launch_roll_sum(const c32* data, int frames, int rows, int cols, const int* shift,
+                           c32* out, hipStream_t st) {
+  hipLaunchKernelGGL(k_roll_sum, dim3(grid_for((long long)rows * cols)), dim3(256), 0, st, data,
+                     frames, rows, cols, shift, out);
   return hipGetLastError();
 }
 

@@ -97,6 +97,14 @@ class Context:
         check(self.lib.spk_blur2d(self.handle, _ptr(img), _ptr(out), r, c, float(sigma)))
         return out
 
+    def com_average(self, stack: torch.Tensor) -> torch.Tensor:
+        """Sum of the frames rolled onto frame 0's centre of mass (complex64)."""
+        f, r, c = stack.shape
+        _expect(stack, torch.complex64, (f, r, c), self.device, "stack")
+        out = torch.empty((r, c), dtype=torch.complex64, device=self.device)
+        check(self.lib.spk_com_average(self.handle, _ptr(stack), f, r, c, _ptr(out)))
+        return out
+
     def copy_bytes(self, dst: torch.Tensor, src: torch.Tensor) -> None:
         nb = src.numel() * src.element_size()
         check(self.lib.spk_copy_bytes(self.handle, _ptr(dst), _ptr(src), nb))

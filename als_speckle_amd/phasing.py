@@ -25,7 +25,8 @@ from .device import Context, Plan, require_gpu, to_device
 from .masking import bounding_box, find_all_objects
 
 __all__ = ["phasing", "expand_order", "shrinkwrap_schedule", "DeviceArray",
-           "align_global_phase", "refine_support", "prtf", "rftf", "gpu_prtf", "gpu_rftf"]
+           "align_global_phase", "refine_support", "prtf", "rftf", "gpu_prtf", "gpu_rftf",
+           "center_of_mass_average"]
 
 
 def expand_order(order, iterations):
@@ -664,3 +665,21 @@ def gpu_rftf(gpuinfo, estimates, goal_modulus, ipsf=None, rftfq=False):
     ip = None if ipsf is None else to_device(np.real(ipsf).astype(np.complex64), dev,
                                               torch.complex64)
     return plan.rftf_stack(e, g, ipsf=ip).cpu().numpy()
+
+
+def center_of_mass_average(imgs):
+    """Sum of reconstructions aligned by the centre of mass of their
+    magnitudes, ``phasing.py:1710-1745``, on the GPU.
+
+    Each frame is rolled by minus the integer (truncated) offset of its
+    centre of mass from frame 0's and the rolled frames are summed (the
+    reference returns the sum, not the mean) in FP64; returns complex64 (the
+    reference accumulates in the input dtype).  Unlike the reference, a 2-d
+    input is not reshaped in place.
+    """
+    assert isinstance(imgs, np.ndarray)
+    assert imgs.ndim in (2, 3)
+    stack = imgs[None] if imgs.ndim == 2 else imgs
+    ctx = _context()
+    t = to_device(stack.astype(np.complex64), ctx.device, torch.complex64)
+    return ctx.com_average(t).cpu().numpy()

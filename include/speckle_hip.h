@@ -254,6 +254,12 @@ int spk_coherence_gauss_prepare(spk_plan* plan, const float* estimate_c64, int r
                                 const float* modulus);
 int spk_coherence_gauss_eval(spk_plan* plan, double cl0, double cl1, double* value);
 
+/* center_of_mass_average (phasing.py:1710-1745): out (rows, cols) c64 = the
+ * SUM over frames of each frame rolled by minus its integer centre-of-mass
+ * offset (of |frame|, truncated) from frame 0's.  Waits for the result. */
+int spk_com_average(spk_ctx* ctx, const float* data, int frames, int rows, int cols,
+                    float* out);
+
 /* trial average: out (rows, cols) c64 = mean over frames of data; the
  * reference averages aligned trials with np.mean / np.sum on the host
  * (phasing.py:960-961, basic_phasing_example.py:43); FP64 sums */

@@ -755,3 +755,22 @@ def gaussian_coherence(estimates, modulus):
         out[:, k] = fmin(lambda c: gaussian_coherence_objective(ac, m, c), (n // 2, n // 2),
                          disp=False, ftol=.1, xtol=0.1)
     return out
+
+
+def center_of_mass_average(imgs):
+    """``phasing.py:1710-1745``: the sum over frames of each frame rolled by
+    minus its truncated centre-of-mass offset from frame 0's (|frame|
+    weights)."""
+    imgs = np.asarray(imgs)
+    if imgs.ndim == 2:
+        imgs = imgs[None]
+    rows, cols = np.indices(imgs.shape[1:])
+    total = np.zeros(imgs.shape[1:], np.complex128)
+    for n, img in enumerate(imgs):
+        a = np.abs(img)
+        f = np.sum(a)
+        r, c = int(np.sum(rows * a) / f), int(np.sum(cols * a) / f)
+        if n == 0:
+            r0, c0 = r, c
+        total += np.roll(np.roll(img, -(r - r0), axis=0), -(c - c0), axis=1)
+    return total

@@ -154,3 +154,15 @@ def test_rftf_with_ipsf_matches_oracle(stack):
         got = P.rftf(est, g, ipsf=ipsf)
         want = O.rftf(est, g, ipsf=ipsf)
     assert np.max(np.abs(got - want)) < 2e-5
+
+
+def test_center_of_mass_average_matches_oracle():
+    rng = np.random.default_rng(12)
+    obj, s = two_part_object(64, rng)
+    frames = [np.roll(np.roll(obj, int(rng.integers(-6, 7)), 0), int(rng.integers(-6, 7)), 1)
+              * np.exp(1j * rng.uniform(0, 6)) + 0.01 * crand(obj.shape, rng) for _ in range(5)]
+    stack = np.stack(frames)[:, :60, :50]
+    got = P.center_of_mass_average(stack)
+    want = O.center_of_mass_average(stack)
+    assert got.shape == (60, 50)
+    assert np.max(np.abs(got - want)) < 1e-5 * np.abs(want).max()

@@ -428,6 +428,17 @@ def test_rftf_ipsf_closed_forms():
                        O.rftf_stack(obj[None], g, ipsf=(ip * np.exp(0.7j)).real))
 
 
+def test_center_of_mass_average_closed_forms():
+    rng = np.random.default_rng(51)
+    a = np.zeros((20, 24), complex)
+    a[5:9, 6:11] = crand((4, 5), rng)
+    # copies of one frame shifted by whole pixels sum back onto frame 0
+    stack = np.stack([a, np.roll(np.roll(a, 3, 0), -2, 1), np.roll(a, 5, 1)])
+    assert np.allclose(O.center_of_mass_average(stack), 3 * a)
+    # a single frame is returned as is
+    assert np.allclose(O.center_of_mass_average(a), a)
+
+
 def test_blur_kernel_odd_size_peak():
     # centre at n//2 then fftshift: the peak lands at n - 1 for odd n
     k = O.gaussian_kernel((9, 7), 1.5)
