@@ -52,6 +52,7 @@ SIGNATURES = {
     "spk_plan_create": (c_int, [c_void_p, c_int, c_int, c_int, POINTER(SpkTune), POINTER(c_void_p)]),
     "spk_plan_destroy": (c_int, [c_void_p]),
     "spk_plan_check": (c_int, [c_void_p]),
+    "spk_plan_info": (c_int, [c_void_p, POINTER(c_int), POINTER(c_int), POINTER(c_int)]),
     "spk_set_ipsf": (c_int, [c_void_p, c_void_p]),
     "spk_set_spectrum": (c_int, [c_void_p, c_void_p, c_void_p, c_int]),
     "spk_plan_bytes": (c_int, [c_int, c_int, c_int, c_int, POINTER(c_size_t)]),

@@ -35,6 +35,9 @@ COMMON = [
     "-I", os.path.join(ROOT, "include"), "-I", HERE,
     "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
 ]
+# extra -D definitions for layout experiments (space separated), e.g.
+# SPK_BUILD_DEFS="-DSPK_DUAL_MIN=1024"
+COMMON += os.environ.get("SPK_BUILD_DEFS", "").split()
 
 
 def _units():

@@ -150,8 +150,8 @@ struct spk_plan {
   bool coh_ready = false;
   float phib_g = 0.0f;
   long long nn() const { return (long long)n * n; }
-  // Phi kept in two layouts (see spk_fft_core.hpp dual_layout): n >= 4096
-  bool dual() const { return n >= 4096; }
+  // Phi kept in two layouts (see spk_fft_core.hpp dual_layout)
+  bool dual() const { return n >= kDualMin; }
   c32* phi_in() const { return dual() ? phic : phi; }    // column pass input
   c32* phib_in() const { return dual() ? phibc : phib; }
   long long mstride() const { return shared ? 0 : nn(); }
@@ -464,6 +464,14 @@ int spk_plan_create(spk_ctx* ctx, int n, int batch, int shared, const spk_tune* 
     return fail(SPK_EHIP, std::string("plan setup: ") + hipGetErrorString(e));
   }
   *out = p;
+  return SPK_OK;
+}
+
+int spk_plan_info(const spk_plan* p, int* tile, int* ept, int* persist) {
+  SPK_REQUIRE(p, SPK_EINVAL, "null plan");
+  if (tile) *tile = p->tile;
+  if (ept) *ept = p->ept;
+  if (persist) *persist = p->persist;
   return SPK_OK;
 }
 

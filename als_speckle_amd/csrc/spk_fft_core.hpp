@@ -251,9 +251,13 @@ SPK_HD int col_pos(int band, int b, int t, int i) {
 //    memory (at 4096 the tile stride of the single layout is 128 KB).
 // Measured on MI355X: strided writes cost more than strided reads up to
 // 2048, so the dual layout only pays at 4096.
+#ifndef SPK_DUAL_MIN
+#define SPK_DUAL_MIN 4096
+#endif
+constexpr int kDualMin = SPK_DUAL_MIN;
 template <int N>
 constexpr bool dual_layout() {
-  return N >= 4096;
+  return N >= kDualMin;
 }
 template <int TT>
 SPK_HD int mcb_pos(int n, int r, int c) {

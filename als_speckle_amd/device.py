@@ -146,6 +146,9 @@ class Plan:
                                        byref(tune), byref(h)))
         self.handle = h
         _live_plans.add(self)
+        t, e, pe = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        check(self.lib.spk_plan_info(h, byref(t), byref(e), byref(pe)))
+        self.tile, self.ept = t.value, e.value       # the resolved shape
 
     # -- helpers ---------------------------------------------------------
     def _frames_shape(self, frame, per_frame):

@@ -51,7 +51,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--cpu-iters", type=int, default=100)
+    ap.add_argument("--cpu-iters", type=int, default=300)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
@@ -259,9 +259,25 @@ def bench_batched_1024(ctx, frame, trials=16):
     ctx.sync()
     dt = time.perf_counter() - t0
     fe = err[-1].cpu().numpy()
+    # the passes of one more run, timed per launch (dispatch-recorded events):
+    # the FFT + projection (column) pass and the update (row) pass at their
+    # batched shape, against the 8 TB/s roofline (m and s per trial here, so
+    # the model bytes are the bytes moved)
+    plan.profile(True)
+    run()
+    ctx.sync()
+    nc, tc = plan.profile_read(0)
+    nr, tr = plan.profile_read(1)
+    plan.profile(False)
+    px = trials * n * n
+    col_us, row_us = tc / nc * 1e3, tr / nr * 1e3
     return {"trials": trials, "recons_per_s": round(trials / dt, 3),
             "iterations_per_s": round(trials * iters / dt, 1),
-            "final_error_mean": float(fe.mean()), "final_error_min": float(fe.min())}
+            "final_error_mean": float(fe.mean()), "final_error_min": float(fe.min()),
+            "tile": plan.tile, "values_per_thread": plan.ept,
+            "column_pass_us": round(col_us, 2), "row_pass_us": round(row_us, 2),
+            "column_pass_hbm_frac": round(B_COL * px / (col_us * 1e-6) / HBM_PEAK, 4),
+            "row_pass_hbm_frac": round(B_ROW * px / (row_us * 1e-6) / HBM_PEAK, 4)}
 
 
 def bench_serving(a):

@@ -87,6 +87,10 @@ const char* spk_last_error(void);
 const char* spk_version(void);
 uint32_t spk_supported_sizes(void);
 
+/* The layout tile, values per thread and launch mode a plan was built with
+ * (the defaults resolved).  No reference counterpart. */
+int spk_plan_info(const spk_plan* plan, int* tile, int* ept, int* persist);
+
 /* Waits for the plan's queued work and reports whether a grid barrier of a
  * persistent iteration block gave up waiting (SPK_EHIP; the flag is then
  * cleared).  No counterpart in the reference, whose kernels each end with a
