@@ -29,7 +29,7 @@ class SpkStep(ctypes.Structure):
 
 class SpkTune(ctypes.Structure):
     _fields_ = [("tile", c_int32), ("ept", c_int32), ("persist", c_int32),
-                ("reserved", c_int32 * 5)]
+                ("sw_separate", c_int32), ("reserved", c_int32 * 4)]
 
 
 class SpkError(RuntimeError):
@@ -53,6 +53,8 @@ SIGNATURES = {
     "spk_plan_destroy": (c_int, [c_void_p]),
     "spk_plan_check": (c_int, [c_void_p]),
     "spk_plan_info": (c_int, [c_void_p, POINTER(c_int), POINTER(c_int), POINTER(c_int)]),
+    "spk_plan_sw_counts": (c_int, [c_void_p, POINTER(ctypes.c_longlong),
+                                   POINTER(ctypes.c_longlong)]),
     "spk_set_ipsf": (c_int, [c_void_p, c_void_p]),
     "spk_set_spectrum": (c_int, [c_void_p, c_void_p, c_void_p, c_int]),
     "spk_plan_bytes": (c_int, [c_int, c_int, c_int, c_int, POINTER(c_size_t)]),

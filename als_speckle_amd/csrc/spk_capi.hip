@@ -116,6 +116,13 @@ struct spk_plan {
   // parameters) staged in pinned memory and copied in one transfer
   unsigned char* stage_d = nullptr;
   unsigned char* stage_h = nullptr;
+  unsigned char* stage_cur = nullptr;  // device tables of the latest spk_iterate
+  // Under graph capture the tables go to buffers of the capture's own (one
+  // pair per captured graph, kept until the plan is destroyed): a replay
+  // copies them from pinned memory that later calls do not overwrite.
+  bool capturing = false;
+  std::vector<std::pair<unsigned char*, unsigned char*>> cap_stages;  // (device, pinned)
+  size_t cap_used = 0;
   size_t stage_cap = 0;
   hipEvent_t upload_done = nullptr;
   bool upload_pending = false;
@@ -127,6 +134,9 @@ struct spk_plan {
   // (0 = not queried yet), barrier counter [0] and give-up flag [1]
   int persist = 0;
   int pgrid = 0;
+  // shrinkwraps fused into the passes where possible (spk_tune.sw_separate = 0)
+  bool sw_fuse = true;
+  long long sw_fused_count = 0, sw_separate_count = 0;
   unsigned* pbar = nullptr;
   // partial coherence: ipsf (tiled, shared by the frames) and the per-frame
   // work buffers of the |Psi|^2 convolution
@@ -245,6 +255,9 @@ PassArgs base_args(spk_plan* p) {
   a.dbufc = p->dual() ? p->dbufc : p->dbuf;
   a.imap = p->imap;
   a.dmap = p->dmap;
+  a.sw_tmp = p->sw_tmp;
+  a.sw_blur = p->sw_blur;
+  a.sw_maxp = p->sw_maxp;
   return a;
 }
 
@@ -284,9 +297,24 @@ struct Stage {
   }
 };
 
+// bytes of the table buffers of one captured graph
+constexpr size_t kCaptureStage = (size_t)4 << 20;
+
 int upload_stage(spk_plan* p, const Stage& stg) {
   const size_t need = stg.buf.size();
   if (need == 0) return SPK_OK;
+  if (p->capturing) {
+    SPK_REQUIRE(!p->cap_stages.empty() && p->cap_used + need <= kCaptureStage, SPK_EUNSUP,
+                "the step tables of a captured block exceed " +
+                    std::to_string(kCaptureStage >> 20) + " MiB");
+    unsigned char* d = p->cap_stages.back().first + p->cap_used;
+    unsigned char* h = p->cap_stages.back().second + p->cap_used;
+    std::memcpy(h, stg.buf.data(), need);
+    SPK_HIP(hipMemcpyAsync(d, h, need, hipMemcpyHostToDevice, p->ctx->stream));
+    p->cap_used += (need + 255) & ~(size_t)255;
+    p->stage_cur = d;
+    return SPK_OK;
+  }
   if (p->upload_pending) SPK_HIP(hipEventSynchronize(p->upload_done));
   p->upload_pending = false;
   if (need > p->stage_cap) {
@@ -306,6 +334,7 @@ int upload_stage(spk_plan* p, const Stage& stg) {
   SPK_HIP(hipMemcpyAsync(p->stage_d, p->stage_h, need, hipMemcpyHostToDevice, p->ctx->stream));
   SPK_HIP(hipEventRecord(p->upload_done, p->ctx->stream));
   p->upload_pending = true;
+  p->stage_cur = p->stage_d;
   return SPK_OK;
 }
 
@@ -440,6 +469,7 @@ int spk_plan_create(spk_ctx* ctx, int n, int batch, int shared, const spk_tune* 
   p->ept = ept;
   p->bands = n / tile;
   p->persist = (tune && tune->persist > 0) ? 1 : 0;
+  p->sw_fuse = !(tune && tune->sw_separate > 0);
   const size_t nn = (size_t)n * n;
   const size_t fms = shared ? 1 : (size_t)batch;
   if ((rc = dalloc(&p->psi, nn * batch)) || (rc = dalloc(&p->phi, nn * batch)) ||
@@ -475,6 +505,13 @@ int spk_plan_info(const spk_plan* p, int* tile, int* ept, int* persist) {
   return SPK_OK;
 }
 
+int spk_plan_sw_counts(const spk_plan* p, long long* fused, long long* separate) {
+  SPK_REQUIRE(p, SPK_EINVAL, "null plan");
+  if (fused) *fused = p->sw_fused_count;
+  if (separate) *separate = p->sw_separate_count;
+  return SPK_OK;
+}
+
 int spk_plan_check(spk_plan* p) {
   SPK_REQUIRE(p, SPK_EINVAL, "null plan");
   int rc = set_device(p->ctx);
@@ -506,6 +543,10 @@ int spk_plan_destroy(spk_plan* p) {
   dfree(p->coh_ac); dfree(p->coh_w); dfree(p->coh_cb); dfree(p->coh_m); dfree(p->coh_part);
   dfree(p->sw_tmp); dfree(p->sw_blur); dfree(p->sw_maxp); dfree(p->fftbuf);
   dfree(p->stage_d);
+  for (auto& cs : p->cap_stages) {
+    dfree(cs.first);
+    (void)hipHostFree(cs.second);
+  }
   prof_clear(p, 0);
   prof_clear(p, 1);
   if (p->stage_h) (void)hipHostFree(p->stage_h);
@@ -718,9 +759,9 @@ int spk_iterate(spk_plan* p, const spk_step* steps, int nsteps, int per_frame,
   std::vector<int> map_n(nsteps, B), dm_n(nsteps, 0), sw_n(nsteps, 0), sw_r(nsteps, 0);
   std::vector<float> dm_g(nsteps, 0.0f);
   std::vector<SwParam> sw_first(nsteps);
+  std::vector<std::vector<SwParam>> sw_prm(per_frame ? nsteps : 0);
   size_t coef_off = 0, act_off = 0;
   if (per_frame) {
-    coef_off = stg.add(coefs.data(), coefs.size());
     act_off = stg.add(act.data(), act.size());
     std::vector<int> lst;
     std::vector<SwParam> prm;
@@ -759,6 +800,7 @@ int spk_iterate(spk_plan* p, const spk_step* steps, int nsteps, int per_frame,
         sw_n[k] = (int)lst.size();
         sw_off[k] = stg.add(lst.data(), lst.size());
         swp_off[k] = stg.add(prm.data(), prm.size());
+        sw_prm[k] = prm;
       }
     }
   } else {
@@ -771,12 +813,53 @@ int spk_iterate(spk_plan* p, const spk_step* steps, int nsteps, int per_frame,
       }
     }
   }
-  // Persistent blocks: runs of at least two steps with no DM step and no DM
-  // successor, each run ending at a step with a shrinkwrap (which follows the
-  // block).  block_end[k0] = k1 for a block of steps [k0, k1).
   std::vector<char> emits(nsteps, 0);
   for (int k = 0; k < nsteps; ++k)
     for (int f = 0; f < W; ++f) emits[k] |= (coefs[(size_t)k * W + f].flags & kEmitB) != 0;
+  // Fused shrinkwraps: the shrinkwrap after step k runs inside the passes of
+  // steps k and k+1 (row blur in row pass k, column blur in column pass k+1,
+  // threshold in row pass k+1, the only reader of the support in between)
+  // when both steps are single projections without DM, every frame with the
+  // shrinkwrap is active at k+1 and the blur radius is within the fused
+  // form's window.  Parameter tables by frame (per_frame) or one entry.
+  std::vector<char> sw_fuse(nsteps, 0);
+  std::vector<size_t> swt_off(nsteps, 0);
+  if (p->sw_fuse && !p->persist && !p->ipsf_on && !p->spec_on) {
+    for (int k = 0; k + 1 < nsteps; ++k) {
+      if (!any_sw[k] || sw_r[k] > kSwFuseRmax || any_dm[k] || any_dm[k + 1] || emits[k + 1])
+        continue;
+      const size_t row = (size_t)k * W, nrow = (size_t)(k + 1) * W;
+      std::vector<SwParam> tab(per_frame ? B : 1);
+      if (per_frame) {
+        bool ok = true;
+        int j = 0;
+        for (int f = 0; f < B; ++f) {
+          const spk_step& sp = steps[row + f];
+          if (sp.alg == SPK_NOOP || !(sp.sw_sigma > 0.0f)) continue;
+          ok &= act[nrow + f] != 0;
+          tab[f] = sw_prm[k][j++];
+        }
+        if (!ok) continue;
+        for (int f = 0; f < B; ++f) {
+          const spk_step& sp = steps[row + f];
+          if (sp.alg == SPK_NOOP || !(sp.sw_sigma > 0.0f)) continue;
+          coefs[row + f].flags |= kSwRow;
+          coefs[nrow + f].flags |= kSwFin;
+        }
+      } else {
+        if (!act[nrow]) continue;
+        tab[0] = sw_first[k];
+        coefs[row].flags |= kSwRow;
+        coefs[nrow].flags |= kSwFin;
+      }
+      sw_fuse[k] = 1;
+      swt_off[k] = stg.add(tab.data(), tab.size());
+    }
+  }
+  if (per_frame) coef_off = stg.add(coefs.data(), coefs.size());
+  // Persistent blocks: runs of at least two steps with no DM step and no DM
+  // successor, each run ending at a step with a shrinkwrap (which follows the
+  // block).  block_end[k0] = k1 for a block of steps [k0, k1).
   std::vector<int> block_end(nsteps, 0);
   std::vector<size_t> block_off(nsteps, 0);
   if (p->persist && !p->ipsf_on && !p->spec_on) {
@@ -805,7 +888,7 @@ int spk_iterate(spk_plan* p, const spk_step* steps, int nsteps, int per_frame,
     }
   }
   if ((rc = upload_stage(p, stg))) return rc;
-  unsigned char* dbase = p->stage_d;
+  unsigned char* dbase = p->stage_cur;
   auto dptr = [&](size_t off) { return dbase + off; };
 
   // error partials [step][frame][band]
@@ -833,6 +916,7 @@ int spk_iterate(spk_plan* p, const spk_step* steps, int nsteps, int per_frame,
                                 sw_r[k], st));
     }
     p->phib_fresh = false;
+    p->sw_separate_count++;
     return SPK_OK;
   };
   for (int k = 0; k < nsteps;) {
@@ -868,6 +952,10 @@ int spk_iterate(spk_plan* p, const spk_step* steps, int nsteps, int per_frame,
                             per_frame ? dm_n[k] : B, st));
     }
     a.errp = err_trace ? p->errp + (size_t)k * B * p->bands : nullptr;
+    const int swbits = (sw_fuse[k] ? 1 : 0) | (k > 0 && sw_fuse[k - 1] ? 2 : 0);
+    a.swr_tab = (swbits & 1) ? (const SwParam*)dptr(swt_off[k]) : nullptr;
+    a.swf_tab = (swbits & 2) ? (const SwParam*)dptr(swt_off[k - 1]) : nullptr;
+    a.sw_pf = per_frame ? 1 : 0;
     if (p->ipsf_on || p->spec_on) {
       // partially coherent projection.  ipsf: column pass A (|Psi|^2 ->
       // column FFT) and the row convolution with the ipsf.  spectrum: the
@@ -888,13 +976,16 @@ int spk_iterate(spk_plan* p, const spk_step* steps, int nsteps, int per_frame,
       SPK_HIP(launch_colpass(p->n, p->tile, p->ept, p->spec_on ? 8 : 5, a, nact, 1, st));
     } else {
       if ((rc = prof_arm(p, 0))) return rc;
-      SPK_HIP(launch_colpass(p->n, p->tile, p->ept, 0, a, nact, any_dm[k] ? 2 : 1, st));
+      SPK_HIP(launch_colpass(p->n, p->tile, p->ept, (swbits & 2) ? 9 : 0, a, nact,
+                             any_dm[k] ? 2 : 1, st));
     }
     a.errp = nullptr;
     if ((rc = prof_arm(p, 1))) return rc;
-    SPK_HIP(launch_rowpass(p->n, p->tile, p->ept, any_dm[k] != 0, emits[k] != 0, a, nact, st));
+    SPK_HIP(launch_rowpass(p->n, p->tile, p->ept, any_dm[k] != 0, emits[k] != 0, swbits, a, nact,
+                           st));
     p->phib_fresh = emits[k] != 0;
-    if (any_sw[k] && (rc = shrinkwrap_after(k))) return rc;
+    if (sw_fuse[k]) p->sw_fused_count++;
+    if (any_sw[k] && !sw_fuse[k] && (rc = shrinkwrap_after(k))) return rc;
     ++k;
   }
   if (err_trace) {
@@ -1350,7 +1441,18 @@ int spk_graph_begin(spk_plan* p) {
   SPK_REQUIRE(p->ctx->stream != nullptr, SPK_EINVAL,
               "graph capture needs a context created on a non-null stream");
   SPK_REQUIRE(!p->profiling, SPK_ESTATE, "disable profiling before capturing");
+  SPK_REQUIRE(!p->capturing, SPK_ESTATE, "the plan is already capturing");
+  // table buffers of this graph, allocated before the capture starts
+  unsigned char *d = nullptr, *h = nullptr;
+  if ((rc = dalloc(&d, kCaptureStage))) return rc;
+  if (hipHostMalloc((void**)&h, kCaptureStage, hipHostMallocDefault) != hipSuccess) {
+    dfree(d);
+    return fail(SPK_EHIP, "hipHostMalloc of the captured step tables failed");
+  }
+  p->cap_stages.emplace_back(d, h);
+  p->cap_used = 0;
   SPK_HIP(hipStreamBeginCapture(p->ctx->stream, hipStreamCaptureModeThreadLocal));
+  p->capturing = true;
   return SPK_OK;
 }
 
@@ -1360,6 +1462,7 @@ int spk_graph_end(spk_plan* p, spk_graph** out) {
   int rc = set_device(p->ctx);
   if (rc) return rc;
   hipGraph_t g = nullptr;
+  p->capturing = false;
   SPK_HIP(hipStreamEndCapture(p->ctx->stream, &g));
   spk_graph* h = new spk_graph;
   h->ctx = p->ctx;
@@ -1439,7 +1542,7 @@ int spk_time_passes(spk_plan* p, int which, int reps, const spk_step* step, floa
   SPK_HIP(hipEventRecord(e0, st));
   for (int r = 0; r < reps; ++r) {
     if (which != 1) SPK_HIP(launch_colpass(p->n, p->tile, p->ept, 0, a, p->batch, 1, st));
-    if (which != 0) SPK_HIP(launch_rowpass(p->n, p->tile, p->ept, false, false, a, p->batch, st));
+    if (which != 0) SPK_HIP(launch_rowpass(p->n, p->tile, p->ept, false, false, 0, a, p->batch, st));
   }
   SPK_HIP(hipEventRecord(e1, st));
   SPK_HIP(hipEventSynchronize(e1));

@@ -19,6 +19,10 @@ enum : int {
   kActive = 1,    // frame takes part in this step
   kUseT = 2,      // step is DM: needs t (second column pass on phib)
   kEmitB = 4,     // next step is DM: row pass also writes phib = rowFFT(dm1(psi))
+  // shrinkwrap fused into the passes (a shrinkwrap after step k, radius <= 24):
+  kSwRow = 8,     // step k's row pass: row blur of |new psi| -> sw_tmp
+  kSwFin = 16,    // step k+1: column pass blurs sw_tmp's columns -> sw_blur and
+                  // band maxima; row pass applies s *= [blur > tau max] first
 };
 
 struct Coef {
@@ -27,6 +31,22 @@ struct Coef {
   float g_next;   // gamma_m = 1/beta of the next (DM) step, used with kEmitB
   int flags;
 };
+
+struct BlurW {
+  float w[48];  // w[d] for |d| <= radius, normalised over the circle
+  int radius;
+};
+
+struct SwParam {  // one frame's shrinkwrap: blur weights and threshold fraction
+  BlurW w;
+  float tau;
+};
+
+// Register window of the shrinkwrap blur for radii up to 24 px (sigma <= 4),
+// the largest radius whose blur is fused into the passes; the stand-alone
+// kernels also have a 48 px window.  Both forms sum the same terms in the
+// same order at radius <= 24.
+constexpr int kSwFuseRmax = 24;
 
 struct PassArgs {
   c32* psi;               // tiled iterate, frame stride n*n
@@ -52,6 +72,14 @@ struct PassArgs {
   // column-band-major like the modulus
   float* imap;
   float* dmap;
+  // fused shrinkwrap (kSwRow / kSwFin): work maps, per-frame band maxima and
+  // the blur/threshold parameters of the shrinkwrap (per frame, or one set)
+  float* sw_tmp;
+  float* sw_blur;
+  float* sw_maxp;
+  const SwParam* swr_tab;   // for kSwRow (this step's shrinkwrap)
+  const SwParam* swf_tab;   // for kSwFin (the previous step's)
+  int sw_pf;                // 1: the tables hold one entry per frame, 0: one entry
 };
 
 // One step of a persistent iteration block (k_persist).  Offsets are bytes
@@ -65,15 +93,7 @@ struct PStep {
   int pad_;
 };
 
-struct BlurW {
-  float w[48];  // w[d] for |d| <= radius, normalised over the circle
-  int radius;
-};
 
-struct SwParam {  // one frame's shrinkwrap: blur weights and threshold fraction
-  BlurW w;
-  float tau;
-};
 
 // Kernel timing: when set, the next column-pass or row-pass launch on this
 // thread is made with hipExtLaunchKernelGGL and records these start/stop
@@ -87,7 +107,9 @@ extern thread_local hipEvent_t g_launch_events[2];
 bool size_supported(int n, int tile, int ept);
 hipError_t launch_colpass(int n, int tile, int ept, int mode, const PassArgs& a,
                           int frames, int zdim, hipStream_t st);
-hipError_t launch_rowpass(int n, int tile, int ept, bool use_t, bool emit_b,
+// sw (no DM only): bit 0 builds the kSwRow work, bit 1 the kSwFin work; the
+// column pass of a kSwFin step is mode 9 (mode 0 plus the column blur)
+hipError_t launch_rowpass(int n, int tile, int ept, bool use_t, bool emit_b, int sw,
                           const PassArgs& a, int frames, hipStream_t st);
 // persistent iteration block: one cooperative launch of `grid` workgroups runs
 // nsteps (column pass, grid barrier, row pass) steps; bar = barrier counter

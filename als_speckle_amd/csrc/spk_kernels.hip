@@ -199,8 +199,67 @@ __device__ __forceinline__ int xcd_band(int bid) {
   else return bid;
 }
 
+// LDS slot of logical index i in a blur line: one pad word per 8 so that
+// threads starting 8 apart hit different banks
+__device__ __forceinline__ int blur_slot(int i) { return i + (i >> 3); }
+
+// Sliding-window convolution of one line held in LDS: outputs
+// out[c0 .. c0+K) = sum_{|d| <= R} w[|d|] line[(c0 + k + d) mod N], with the
+// window in registers (K + 2*RMAX loads) and weights w[d] = 0 for d > R.
+template <int N, int K, int RMAX>
+__device__ __forceinline__ void blur_window(const float* line, int c0, const float (&w)[RMAX + 1],
+                                            float (&acc)[K]) {
+  float win[K + 2 * RMAX];
+#pragma unroll
+  for (int j = 0; j < K + 2 * RMAX; ++j) win[j] = line[blur_slot((c0 - RMAX + j) & (N - 1))];
+#pragma unroll
+  for (int k = 0; k < K; ++k) acc[k] = w[0] * win[k + RMAX];
+#pragma unroll
+  for (int d = 1; d <= RMAX; ++d) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) acc[k] = fmaf(w[d], win[k + RMAX + d] + win[k + RMAX - d], acc[k]);
+  }
+}
+
+// Fused shrinkwrap (kSwRow / kSwFin steps): the blur of the passes' own
+// bands, in the LDS of the pass once its transforms are done.  A line of the
+// blur is TT x (N + N/8 + 8) floats from the start of the pass's LDS, which
+// the exchange buffers exceed.
+template <int N>
+constexpr int sw_line() { return N + N / 8 + 8; }
+
+__device__ __forceinline__ void sw_weights(const BlurW& bw, float (&w)[kSwFuseRmax + 1]) {
+#pragma unroll
+  for (int d = 0; d <= kSwFuseRmax; ++d) w[d] = d <= bw.radius ? bw.w[d] : 0.0f;
+}
+
+// Blur the TT lines of length N held in LDS (line l at l * sw_line<N>()) by
+// segments of 8 outputs; out(l, j) gets the blurred value j of line l and
+// the function returns this thread's largest output.
+template <int N, int TT, class OUT>
+__device__ __forceinline__ float sw_blur_lines(const float* line, const float (&w)[kSwFuseRmax + 1],
+                                               OUT out) {
+  constexpr int K = 8;
+  const int nt = blockDim.x;
+  float mx = -INFINITY;
+#pragma unroll 1
+  for (int seg = threadIdx.x; seg < N * TT / K; seg += nt) {
+    const int l = seg / (N / K), j0 = (seg % (N / K)) * K;
+    float acc[K];
+    blur_window<N, K, kSwFuseRmax>(line + l * sw_line<N>(), j0, w, acc);
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      out(l, j0 + k, acc[k]);
+      mx = fmaxf(mx, acc[k]);
+    }
+  }
+  return mx;
+}
+
 // ---------------------------------------------------------------------------
 // Column pass.  MODE 0: FFT -> projection (+ error) -> IFFT
+//               MODE 9: MODE 0, then the column blur of a fused shrinkwrap
+//               (sw_tmp -> sw_blur, band maxima) for frames with kSwFin
 //               MODE 2: forward FFT only; MODE 3: inverse FFT only
 //               MODE 4, 5: the two column passes of the partially coherent
 //               projection (colpass_ipsf_a, colpass_div_b)
@@ -214,8 +273,9 @@ __device__ __forceinline__ int xcd_band(int bid) {
 // ---------------------------------------------------------------------------
 template <int N, int TT, int E, int MODE, bool OWN_TW>
 __device__ __forceinline__ void colpass_item(const PassArgs& a, int f, int band, int z,
-                                             double* errp, c32* lds) {
+                                             double* errp, c32* lds, bool swf = false) {
   constexpr int NB = N / TT;
+  constexpr bool PROJ = MODE == 0 || MODE == 9;
   using L = LdsPlan<N, TT>;
   const size_t fo = (size_t)f * N * N;
   // dual layout: read Phi column-band-major (contiguous), write it tiled for
@@ -237,11 +297,12 @@ __device__ __forceinline__ void colpass_item(const PassArgs& a, int f, int band,
   // after the forward transform
   constexpr bool PREFETCH = E <= 16 && NT <= 512;
   const float* __restrict__ mod = a.mod + (size_t)f * a.mstride;
-  float m[MODE == 0 ? E : 1];
-  if constexpr (MODE == 0 && PREFETCH) {
+  float m[PROJ ? E : 1];
+  if constexpr (PROJ && PREFETCH) {
 #pragma unroll
     for (int i = 0; i < E; ++i) m[i] = gld(mod, mcol_pos<N, TT, E>(band, b, t, i));
   }
+  float rb[MODE == 9 ? E : 1];
   c32* twl = lds + L::NBUF * L::BUF;
   if constexpr (OWN_TW) tws.put(twl);
   const auto tw = twiddle_view<N, TT>(twl);
@@ -252,7 +313,7 @@ __device__ __forceinline__ void colpass_item(const PassArgs& a, int f, int band,
   } else {
     fft_regs<N, E, TT, false, true>(x, t, b, lds, tw);
   }
-  if constexpr (MODE == 0) {
+  if constexpr (PROJ) {
     if constexpr (!PREFETCH) {
 #pragma unroll
       for (int i = 0; i < E; ++i) m[i] = gld(mod, mcol_pos<N, TT, E>(band, b, t, i));
@@ -271,14 +332,43 @@ __device__ __forceinline__ void colpass_item(const PassArgs& a, int f, int band,
       err = fma(d, d, err);
       x[i] = scale(x[i], m[i] * rs);
     }
+    // fused shrinkwrap: this band's columns of the row-blurred |psi|
+    // (tiled), loaded under the inverse transform (the modulus is dead)
+    if constexpr (MODE == 9) {
+      if (swf) {
+#pragma unroll
+        for (int i = 0; i < E; ++i) rb[i] = gld(a.sw_tmp + fo, col_pos<N, TT, E>(band, b, t, i));
+      }
+    }
     fft_regs<N, E, TT, true, false>(x, t, b, lds, tw);
   }
 #pragma unroll
   for (int i = 0; i < E; ++i) gst(dst, col_pos<N, TT, E>(band, b, t, i), x[i]);
-  if constexpr (MODE == 0) {
+  if constexpr (PROJ) {
     if (z == 0 && errp) {
       const double s = block_sum(err, reinterpret_cast<double*>(lds));
       if (tid == 0) errp[(size_t)f * NB + band] = s;
+    }
+  }
+  if constexpr (MODE == 9) {
+    // the column half of the shrinkwrap blur (as k_blur_cols): lines are the
+    // TT columns of the band, outputs tiled into sw_blur, the band maximum
+    // into sw_maxp[f][band]
+    if (swf) {
+      constexpr int T = N / E;
+      float* line = reinterpret_cast<float*>(lds);
+      __syncthreads();  // every LDS reader above is done
+#pragma unroll
+      for (int i = 0; i < E; ++i) line[b * sw_line<N>() + blur_slot(t + i * T)] = rb[i];
+      float w[kSwFuseRmax + 1];
+      sw_weights(a.swf_tab[a.sw_pf ? f : 0].w, w);
+      __syncthreads();
+      float* blur = a.sw_blur + fo;
+      const float mx = sw_blur_lines<N, TT>(line, w, [&](int l, int row, float v) {
+        blur[((row / TT) * NB + band) * (TT * TT) + (row % TT) * TT + l] = v;
+      });
+      const float bm = block_max(mx, line);
+      if (tid == 0) a.sw_maxp[(size_t)f * NB + band] = bm;
     }
   }
 }
@@ -412,6 +502,8 @@ template <int N, int TT, int E, int MODE>
 __global__ void __launch_bounds__((N / E) * TT) k_colpass(PassArgs a) {
   constexpr int NB = N / TT;
   __shared__ c32 lds[LdsPlan<N, TT>::TOTAL];
+  static_assert(MODE != 9 || TT * sw_line<N>() * sizeof(float) <= sizeof(lds),
+                "fused blur lines exceed the pass's LDS");
   const int band = xcd_band<NB>(blockIdx.x), z = blockIdx.z;
   const int f = a.fmap ? a.fmap[blockIdx.y] : blockIdx.y;
   const Coef c = a.tab ? a.tab[f] : a.c;
@@ -426,7 +518,8 @@ __global__ void __launch_bounds__((N / E) * TT) k_colpass(PassArgs a) {
   } else if constexpr (MODE == 8) {
     colpass_div_b<N, TT, E, 1>(a, f, band, a.errp, lds);
   } else {
-    colpass_item<N, TT, E, MODE, true>(a, f, band, z, a.errp, lds);
+    colpass_item<N, TT, E, MODE, true>(a, f, band, z, a.errp, lds,
+                                       MODE == 9 && (c.flags & kSwFin));
   }
 }
 
@@ -434,21 +527,43 @@ __global__ void __launch_bounds__((N / E) * TT) k_colpass(PassArgs a) {
 // Row pass: row IFFT of phi (and phib for DM) -> update -> psi; row FFT of the
 // new psi -> phi (and of dm1(psi) -> phib when the next step is DM).
 // One item = one band of TT rows of frame f with coefficients c.
+// SW (fused shrinkwrap, no DM): bit 1, for a frame with kSwFin, first applies
+// the previous step's shrinkwrap to the support rows of the band (s = 0
+// where blur <= tau * max, the FP64 threshold of k_sw_threshold, with the
+// frame maximum reduced from the column pass's band maxima); bit 0, for a
+// frame with kSwRow, ends with the row half of this step's shrinkwrap blur of
+// |new psi| into sw_tmp (as k_blur_rows).
 // ---------------------------------------------------------------------------
-template <int N, int TT, int E, bool USE_T, bool EMIT_B, bool OWN_TW>
+template <int N, int TT, int E, bool USE_T, bool EMIT_B, bool OWN_TW, int SW = 0>
 __device__ __forceinline__ void rowpass_item(const PassArgs& a, int f, int band, const Coef& c,
                                              c32* lds) {
+  static_assert(SW == 0 || (!USE_T && !EMIT_B), "fused shrinkwrap is built without DM");
   using L = LdsPlan<N, TT>;
+  constexpr int NB = N / TT;
   const size_t fo = (size_t)f * N * N;
   const int tid = threadIdx.x, b = tid % TT, t = tid / TT;
   constexpr int NT = (N / E) * TT;
 #define SPK_RPOS(i) row_pos<N, TT, E>(band, b, t, (i))
   c32* phi = a.phi + fo;
   c32* psi = a.psi + fo;
+  const bool swf = (SW & 2) && (c.flags & kSwFin);
+  const bool swr = (SW & 1) && (c.flags & kSwRow);
 
-  // every global load up front: twiddles, phi (and phib), psi_in, support
+  // every global load up front: twiddles, the band maxima of a fused
+  // shrinkwrap, phi (and phib), psi_in, support (and the blur)
   TwStage<N, TT, NT> tws;
   if constexpr (OWN_TW) tws.fetch(a.tw);
+  constexpr int MPT = (SW & 2) ? (NB + NT - 1) / NT : 1;
+  float mpv[MPT];
+  if constexpr (SW & 2) {
+    if (swf) {
+#pragma unroll
+      for (int j = 0; j < MPT; ++j) {
+        const int q = tid + j * NT;
+        mpv[j] = q < NB ? a.sw_maxp[(size_t)f * NB + q] : -INFINITY;
+      }
+    }
+  }
   c32 x[E];
 #pragma unroll
   for (int i = 0; i < E; ++i) x[i] = gld(phi, SPK_RPOS(i));
@@ -463,11 +578,41 @@ __device__ __forceinline__ void rowpass_item(const PassArgs& a, int f, int band,
   const float* __restrict__ sup = a.sup + (size_t)f * a.sstride;
   c32 pin[E];
   float sv[E];
+  float bl[(SW & 2) ? E : 1];
   if constexpr (PREFETCH) {
 #pragma unroll
     for (int i = 0; i < E; ++i) {
       pin[i] = gld(psi, SPK_RPOS(i));
       sv[i] = gld(sup, SPK_RPOS(i));
+    }
+    if constexpr (SW & 2) {
+      if (swf) {
+#pragma unroll
+        for (int i = 0; i < E; ++i) bl[i] = gld(a.sw_blur + fo, SPK_RPOS(i));
+      }
+    }
+  }
+  double thr = 0.0;
+  // s = 0 where blur <= tau * max, in the support and in sv
+  auto apply_threshold = [&]() {
+    float* supw = const_cast<float*>(sup);  // the support rows of this band
+#pragma unroll
+    for (int i = 0; i < E; ++i) {
+      if (!((double)bl[i] > thr)) {
+        sv[i] = 0.0f;
+        gst(supw, SPK_RPOS(i), 0.0f);
+      }
+    }
+  };
+  if constexpr (SW & 2) {
+    if (swf) {  // uniform over the workgroup
+      float mx = mpv[0];
+#pragma unroll
+      for (int j = 1; j < MPT; ++j) mx = fmaxf(mx, mpv[j]);
+      thr = (double)a.swf_tab[a.sw_pf ? f : 0].tau *
+            (double)block_max(mx, reinterpret_cast<float*>(lds));
+      __syncthreads();  // the reduction's LDS is reused by the transforms
+      if constexpr (PREFETCH) apply_threshold();
     }
   }
   c32* twl = lds + L::NBUF * L::BUF;
@@ -482,9 +627,17 @@ __device__ __forceinline__ void rowpass_item(const PassArgs& a, int f, int band,
       pin[i] = gld(psi, SPK_RPOS(i));
       sv[i] = gld(sup, SPK_RPOS(i));
     }
+    if constexpr (SW & 2) {
+      if (swf) {
+#pragma unroll
+        for (int i = 0; i < E; ++i) bl[i] = gld(a.sw_blur + fo, SPK_RPOS(i));
+        apply_threshold();
+      }
+    }
   }
 
   c32 y[EMIT_B ? E : 1];
+  float av[(SW & 1) ? E : 1];
 #pragma unroll
   for (int i = 0; i < E; ++i) {
     const float s = sv[i];
@@ -500,6 +653,7 @@ __device__ __forceinline__ void rowpass_item(const PassArgs& a, int f, int band,
     const c32 nv = axpby(in_s, s, out_s, u);
     gst(psi, SPK_RPOS(i), nv);
     x[i] = nv;
+    if constexpr (SW & 1) av[i] = sqrtf(nv.x * nv.x + nv.y * nv.y);
     if constexpr (EMIT_B) y[i] = scale(nv, fmaf(-c.g_next, u, s));
   }
   // the new row transforms: column-band-major for the column pass (dual
@@ -519,17 +673,36 @@ __device__ __forceinline__ void rowpass_item(const PassArgs& a, int f, int band,
     if constexpr (DUAL) gst(a.phic + fo, cbrow_pos<N, TT, E>(band, b, t, i), x[i]);
     else gst(phi, SPK_RPOS(i), x[i]);
   }
+  if constexpr (SW & 1) {
+    // the row half of the shrinkwrap blur: lines are the TT rows of the band
+    if (swr) {
+      constexpr int T = N / E;
+      float* line = reinterpret_cast<float*>(lds);
+      __syncthreads();  // the transforms' LDS reads are done
+#pragma unroll
+      for (int i = 0; i < E; ++i) line[b * sw_line<N>() + blur_slot(t + i * T)] = av[i];
+      float w[kSwFuseRmax + 1];
+      sw_weights(a.swr_tab[a.sw_pf ? f : 0].w, w);
+      __syncthreads();
+      float* tmp = a.sw_tmp + fo + (size_t)band * N * TT;
+      sw_blur_lines<N, TT>(line, w, [&](int l, int col, float v) {
+        tmp[(col / TT) * (TT * TT) + l * TT + (col % TT)] = v;
+      });
+    }
+  }
 #undef SPK_RPOS
 }
 
 // grid (n/TT bands, frames)
-template <int N, int TT, int E, bool USE_T, bool EMIT_B>
+template <int N, int TT, int E, bool USE_T, bool EMIT_B, int SW>
 __global__ void __launch_bounds__((N / E) * TT) k_rowpass(PassArgs a) {
   __shared__ c32 lds[LdsPlan<N, TT>::TOTAL];
+  static_assert(SW == 0 || TT * sw_line<N>() * sizeof(float) <= sizeof(lds),
+                "fused blur lines exceed the pass's LDS");
   const int f = a.fmap ? a.fmap[blockIdx.y] : blockIdx.y;
   const Coef c = a.tab ? a.tab[f] : a.c;
   if (!(c.flags & kActive)) return;
-  rowpass_item<N, TT, E, USE_T, EMIT_B, true>(a, f, blockIdx.x, c, lds);
+  rowpass_item<N, TT, E, USE_T, EMIT_B, true, SW>(a, f, blockIdx.x, c, lds);
 }
 
 // ---------------------------------------------------------------------------
@@ -809,28 +982,6 @@ static __global__ void k_err_finalize(const double* errp, int nsteps, int frames
 // grid.y walks the frames to process: frame = fmap[y] (or y), parameters
 // tab[y] (or p0).
 // ---------------------------------------------------------------------------
-// LDS slot of logical index i in a blur line: one pad word per 8 so that
-// threads starting 8 apart hit different banks
-__device__ __forceinline__ int blur_slot(int i) { return i + (i >> 3); }
-
-// Sliding-window convolution of one line held in LDS: outputs
-// out[c0 .. c0+K) = sum_{|d| <= R} w[|d|] line[(c0 + k + d) mod N], with the
-// window in registers (K + 2*RMAX loads) and weights w[d] = 0 for d > R.
-template <int N, int K, int RMAX>
-__device__ __forceinline__ void blur_window(const float* line, int c0, const float (&w)[RMAX + 1],
-                                            float (&acc)[K]) {
-  float win[K + 2 * RMAX];
-#pragma unroll
-  for (int j = 0; j < K + 2 * RMAX; ++j) win[j] = line[blur_slot((c0 - RMAX + j) & (N - 1))];
-#pragma unroll
-  for (int k = 0; k < K; ++k) acc[k] = w[0] * win[k + RMAX];
-#pragma unroll
-  for (int d = 1; d <= RMAX; ++d) {
-#pragma unroll
-    for (int k = 0; k < K; ++k) acc[k] = fmaf(w[d], win[k + RMAX + d] + win[k + RMAX - d], acc[k]);
-  }
-}
-
 template <int RMAX>
 __device__ __forceinline__ void load_blur_weights(const SwParam* tab, const SwParam& p0,
                                                   float* wsh, float (&w)[RMAX + 1]) {
@@ -1487,23 +1638,31 @@ static hipError_t colpass_nt(int mode, const PassArgs& a, int frames, int zdim,
     case 6: launch_pass(k_colpass<N, TT, E, 6>, grid, block, st, a); break;
     case 7: launch_pass(k_colpass<N, TT, E, 7>, grid, block, st, a); break;
     case 8: launch_pass(k_colpass<N, TT, E, 8>, grid, block, st, a); break;
+    case 9: launch_pass(k_colpass<N, TT, E, 9>, grid, block, st, a); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
 }
 
 template <int N, int TT, int E>
-static hipError_t rowpass_nt(bool use_t, bool emit_b, const PassArgs& a, int frames,
+static hipError_t rowpass_nt(bool use_t, bool emit_b, int sw, const PassArgs& a, int frames,
                              hipStream_t st) {
   dim3 grid(N / TT, frames), block((N / E) * TT);
+  if (sw && (use_t || emit_b)) return hipErrorInvalidValue;
   if (use_t && emit_b)
-    launch_pass(k_rowpass<N, TT, E, true, true>, grid, block, st, a);
+    launch_pass(k_rowpass<N, TT, E, true, true, 0>, grid, block, st, a);
   else if (use_t)
-    launch_pass(k_rowpass<N, TT, E, true, false>, grid, block, st, a);
+    launch_pass(k_rowpass<N, TT, E, true, false, 0>, grid, block, st, a);
   else if (emit_b)
-    launch_pass(k_rowpass<N, TT, E, false, true>, grid, block, st, a);
+    launch_pass(k_rowpass<N, TT, E, false, true, 0>, grid, block, st, a);
+  else if (sw == 1)
+    launch_pass(k_rowpass<N, TT, E, false, false, 1>, grid, block, st, a);
+  else if (sw == 2)
+    launch_pass(k_rowpass<N, TT, E, false, false, 2>, grid, block, st, a);
+  else if (sw == 3)
+    launch_pass(k_rowpass<N, TT, E, false, false, 3>, grid, block, st, a);
   else
-    launch_pass(k_rowpass<N, TT, E, false, false>, grid, block, st, a);
+    launch_pass(k_rowpass<N, TT, E, false, false, 0>, grid, block, st, a);
   return hipGetLastError();
 }
 
@@ -1560,9 +1719,9 @@ static hipError_t shrinkwrap_nt(const c32* psi, float* sup, long long sstride, f
                                 const int* fmap, int frames, int rmax, hipStream_t st) {
   dim3 grid(N / TT, frames);
   // register window sized for the largest radius among the frames
-  if (rmax <= 24) {
-    hipLaunchKernelGGL((k_blur_rows<N, TT, 24>), grid, dim3(256), 0, st, psi, tmp, tab, p0, fmap);
-    hipLaunchKernelGGL((k_blur_cols<N, TT, 24>), grid, dim3(256), 0, st, tmp, blur, maxp, tab,
+  if (rmax <= kSwFuseRmax) {
+    hipLaunchKernelGGL((k_blur_rows<N, TT, kSwFuseRmax>), grid, dim3(256), 0, st, psi, tmp, tab, p0, fmap);
+    hipLaunchKernelGGL((k_blur_cols<N, TT, kSwFuseRmax>), grid, dim3(256), 0, st, tmp, blur, maxp, tab,
                        p0, fmap);
   } else {
     hipLaunchKernelGGL((k_blur_rows<N, TT, 48>), grid, dim3(256), 0, st, psi, tmp, tab, p0, fmap);
@@ -1596,8 +1755,8 @@ constexpr bool shape_ok() {
 #define SPK_SIZED_DECL(n)                                                                   \
   hipError_t colpass_##n(int tile, int ept, int mode, const PassArgs& a, int frames,       \
                          int zdim, hipStream_t st);                                         \
-  hipError_t rowpass_##n(int tile, int ept, bool use_t, bool emit_b, const PassArgs& a,    \
-                         int frames, hipStream_t st);                                       \
+  hipError_t rowpass_##n(int tile, int ept, bool use_t, bool emit_b, int sw,              \
+                         const PassArgs& a, int frames, hipStream_t st);                    \
   hipError_t rowfft_##n(int tile, int ept, int mode, const c32* in, c32* out,              \
                         const float* sup, long long sstride, float g, const c32* tw,       \
                         const int* fmap, int frames, hipStream_t st);                       \
@@ -1637,7 +1796,7 @@ constexpr bool shape_ok() {
 #define SPK_CALL_ROW(TT_, E_)                                                   \
   (shape_ok<SPK_N, TT_, E_>() ? rowpass_nt<SPK_N, (shape_ok<SPK_N, TT_, E_>() ? TT_ : 4), \
                                            (shape_ok<SPK_N, TT_, E_>() ? E_ : 16)>(     \
-                                    use_t, emit_b, a, frames, st)                       \
+                                    use_t, emit_b, sw, a, frames, st)                   \
                               : hipErrorInvalidValue)
 #define SPK_CALL_FFT(TT_, E_)                                                   \
   (shape_ok<SPK_N, TT_, E_>() ? rowfft_nt<SPK_N, (shape_ok<SPK_N, TT_, E_>() ? TT_ : 4),  \
@@ -1651,8 +1810,8 @@ constexpr bool shape_ok() {
     constexpr int SPK_N = n;                                                                \
     SPK_SHAPE_SWITCH(n, SPK_CALL_COL)                                                       \
   }                                                                                         \
-  hipError_t rowpass_##n(int tile, int ept, bool use_t, bool emit_b, const PassArgs& a,    \
-                         int frames, hipStream_t st) {                                      \
+  hipError_t rowpass_##n(int tile, int ept, bool use_t, bool emit_b, int sw,              \
+                         const PassArgs& a, int frames, hipStream_t st) {                   \
     constexpr int SPK_N = n;                                                                \
     SPK_SHAPE_SWITCH(n, SPK_CALL_ROW)                                                       \
   }                                                                                         \
@@ -1753,9 +1912,9 @@ hipError_t launch_colpass(int n, int tile, int ept, int mode, const PassArgs& a,
   SPK_SIZE_SWITCH(colpass, tile, ept, mode, a, frames, zdim, st)
 }
 
-hipError_t launch_rowpass(int n, int tile, int ept, bool use_t, bool emit_b, const PassArgs& a,
-                          int frames, hipStream_t st) {
-  SPK_SIZE_SWITCH(rowpass, tile, ept, use_t, emit_b, a, frames, st)
+hipError_t launch_rowpass(int n, int tile, int ept, bool use_t, bool emit_b, int sw,
+                          const PassArgs& a, int frames, hipStream_t st) {
+  SPK_SIZE_SWITCH(rowpass, tile, ept, use_t, emit_b, sw, a, frames, st)
 }
 
 hipError_t launch_rowfft(int n, int tile, int ept, int mode, const c32* in, c32* out,

@@ -133,7 +133,8 @@ class Plan:
     """``batch`` resident frames of n x n (see spk_plan_create)."""
 
     def __init__(self, ctx: Context, n: int, batch: int = 1, shared: bool = True,
-                 tile: int = 0, ept: int = 0, persist: bool = False):
+                 tile: int = 0, ept: int = 0, persist: bool = False,
+                 sw_separate: bool = False):
         self.ctx, self.lib = ctx, ctx.lib
         self.n, self.batch, self.shared = int(n), int(batch), bool(shared)
         self.persist = bool(persist)
@@ -141,6 +142,7 @@ class Plan:
         tune.tile = int(tile)
         tune.ept = int(ept)
         tune.persist = int(self.persist)
+        tune.sw_separate = int(bool(sw_separate))
         h = c_void_p()
         check(self.lib.spk_plan_create(ctx.handle, self.n, self.batch, int(self.shared),
                                        byref(tune), byref(h)))
@@ -202,6 +204,13 @@ class Plan:
                                    tab.ctypes.data_as(ctypes.POINTER(_capi.SpkStep)), nsteps,
                                    int(per_frame), _ptr(err) if err is not None else None))
         return err
+
+    def sw_counts(self) -> tuple[int, int]:
+        """(fused, separate): shrinkwraps run so far inside the passes and as
+        stand-alone kernels."""
+        fu, se = ctypes.c_longlong(), ctypes.c_longlong()
+        check(self.lib.spk_plan_sw_counts(self.handle, byref(fu), byref(se)))
+        return fu.value, se.value
 
     def shrinkwrap(self, sigma: float, tau: float, frame: int = -1) -> None:
         check(self.lib.spk_shrinkwrap(self.handle, frame, float(sigma), float(tau)))

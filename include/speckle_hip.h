@@ -65,7 +65,10 @@ typedef struct spk_tune {
   int32_t tile;       /* tile edge of the internal layout: 2, 4 or 8         */
   int32_t ept;        /* values per thread of one transform: 8 or 16         */
   int32_t persist;    /* 1: run iteration blocks as one persistent launch    */
-  int32_t reserved[5];
+  int32_t sw_separate;/* 1: run each shrinkwrap as its own kernels (default:
+                         fused into the passes of its step and the next
+                         where the blur radius is <= 24 px, i.e. sigma <= 4) */
+  int32_t reserved[4];
 } spk_tune;
 
 /* -------------------------------------------------------------------------
@@ -90,6 +93,9 @@ uint32_t spk_supported_sizes(void);
 /* The layout tile, values per thread and launch mode a plan was built with
  * (the defaults resolved).  No reference counterpart. */
 int spk_plan_info(const spk_plan* plan, int* tile, int* ept, int* persist);
+/* Shrinkwraps the plan's spk_iterate calls have run so far, fused into the
+ * passes and as stand-alone kernels.  No reference counterpart. */
+int spk_plan_sw_counts(const spk_plan* plan, long long* fused, long long* separate);
 
 /* Waits for the plan's queued work and reports whether a grid barrier of a
  * persistent iteration block gave up waiting (SPK_EHIP; the flag is then

@@ -358,5 +358,7 @@ def test_graph_replay_matches_eager():
             assert np.array_equal(err.cpu().numpy(), want_err), rep
             q = load_plan(ctx, f, psi)   # unrelated eager work in between
             q.iterate([step("raar", 0.8)] * 3)
+            # eager work on the captured plan itself, with other step tables
+            p.iterate([step("hio", 0.5, "numpy", 1.0 + rep, 0.3)] * 4 + [step("er")])
             ctx.sync()
     graph.close()
