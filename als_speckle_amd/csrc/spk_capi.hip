@@ -42,11 +42,14 @@ bool is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
 //    register spills (tile 4 needs 16 values per thread there, which spill);
 //    at 2048 it is 13 % faster per iteration for one frame and 17 % for four;
 //  - 8 values per thread (more threads, shorter dependency chains) wherever
-//    the workgroup fits, except 1024^2 with 8 or more frames, where 16 values
-//    per thread (radix-16 stages) are 8 % faster per batched iteration.
+//    the workgroup fits, except: 1024^2 with 4 or more frames, where 16
+//    values per thread (radix-16 stages) are 9 % faster per batched
+//    iteration, and 2048^2 (tile 2), where they are 10 % faster for one
+//    frame and 8 % for four.
 int default_tile(int n) { return n >= 2048 ? 2 : 4; }
 int default_ept(int n, int tile, int batch) {
-  if (n == 1024 && batch >= 8 && (n / 16) * tile <= 1024) return 16;
+  if (n == 1024 && batch >= 4 && (n / 16) * tile <= 1024) return 16;
+  if (n == 2048 && tile == 2) return 16;
   return (n / 8) * tile <= 1024 ? 8 : 16;
 }
 

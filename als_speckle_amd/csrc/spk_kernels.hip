@@ -256,6 +256,35 @@ __device__ __forceinline__ float sw_blur_lines(const float* line, const float (&
   return mx;
 }
 
+// Modulus replacement m * Psi / |Psi| (phasing_fourier.cl:12-15,
+// phasing.py:1217) through one reciprocal square root of |Psi|^2 (v_rsq_f32,
+// about 1 ulp); 0 where |Psi|^2 is below the smallest normal float, i.e.
+// |Psi| < 1.1e-19 (the reference gives NaN at 0).  err accumulates the
+// error term: exact difference and square in FP64 of the FP32 |Psi|.
+__device__ __forceinline__ c32 project_modulus(c32 v, float m, double& err) {
+  const float p2 = fmaf(v.x, v.x, v.y * v.y);
+  const float rs = p2 >= 1.17549435e-38f ? __builtin_amdgcn_rsqf(p2) : 0.0f;
+  const float am = p2 * rs;  // |Psi|
+  const double d = (double)am - (double)m;
+  err = fma(d, d, err);
+  return scale(v, m * rs);
+}
+
+// The real-space update of one value (the coefficient form of
+// phasing.py:1231-1328 and the hio/er/raar/sf/dm2 kernels):
+// s (a_o pout + a_i pin + a_t t) + (1 - s) (b_o pout + b_i pin + b_t t)
+template <bool USE_T>
+__device__ __forceinline__ c32 update_value(const Coef& c, c32 pout, c32 pin, float s, c32 tt) {
+  const float u = 1.0f - s;
+  c32 in_s = axpby(pout, c.ao, pin, c.ai);
+  c32 out_s = axpby(pout, c.bo, pin, c.bi);
+  if constexpr (USE_T) {
+    in_s = axpy(in_s, tt, c.at);
+    out_s = axpy(out_s, tt, c.bt);
+  }
+  return axpby(in_s, s, out_s, u);
+}
+
 // ---------------------------------------------------------------------------
 // Column pass.  MODE 0: FFT -> projection (+ error) -> IFFT
 //               MODE 9: MODE 0, then the column blur of a fused shrinkwrap
@@ -319,19 +348,7 @@ __device__ __forceinline__ void colpass_item(const PassArgs& a, int f, int band,
       for (int i = 0; i < E; ++i) m[i] = gld(mod, mcol_pos<N, TT, E>(band, b, t, i));
     }
 #pragma unroll
-    for (int i = 0; i < E; ++i) {
-      // modulus replacement m * Psi / |Psi| (phasing_fourier.cl:12-15,
-      // phasing.py:1217) through one reciprocal square root of |Psi|^2
-      // (v_rsq_f32, about 1 ulp); 0 where |Psi|^2 is below the smallest
-      // normal float, i.e. |Psi| < 1.1e-19 (the reference gives NaN at 0)
-      const float p2 = fmaf(x[i].x, x[i].x, x[i].y * x[i].y);
-      const float rs = p2 >= 1.17549435e-38f ? __builtin_amdgcn_rsqf(p2) : 0.0f;
-      const float am = p2 * rs;  // |Psi|
-      // error term: exact difference and square in FP64 of the FP32 |Psi|
-      const double d = (double)am - (double)m[i];
-      err = fma(d, d, err);
-      x[i] = scale(x[i], m[i] * rs);
-    }
+    for (int i = 0; i < E; ++i) x[i] = project_modulus(x[i], m[i], err);
     // fused shrinkwrap: this band's columns of the row-blurred |psi|
     // (tiled), loaded under the inverse transform (the modulus is dead)
     if constexpr (MODE == 9) {
@@ -642,15 +659,9 @@ __device__ __forceinline__ void rowpass_item(const PassArgs& a, int f, int band,
   for (int i = 0; i < E; ++i) {
     const float s = sv[i];
     const float u = 1.0f - s;
-    const c32 pout = scale(x[i], a.inv_nn);
-    c32 in_s = axpby(pout, c.ao, pin[i], c.ai);
-    c32 out_s = axpby(pout, c.bo, pin[i], c.bi);
-    if constexpr (USE_T) {
-      const c32 tt = scale(tv[i], a.inv_nn);
-      in_s = axpy(in_s, tt, c.at);
-      out_s = axpy(out_s, tt, c.bt);
-    }
-    const c32 nv = axpby(in_s, s, out_s, u);
+    c32 tt{0.0f, 0.0f};
+    if constexpr (USE_T) tt = scale(tv[i], a.inv_nn);
+    const c32 nv = update_value<USE_T>(c, scale(x[i], a.inv_nn), pin[i], s, tt);
     gst(psi, SPK_RPOS(i), nv);
     x[i] = nv;
     if constexpr (SW & 1) av[i] = sqrtf(nv.x * nv.x + nv.y * nv.y);

@@ -24,6 +24,9 @@ Reported beside it in the same JSON line:
                 1024: 20 %, 2048: 10 %}, 500 HIO/ER iterations with shrinkwrap
                 each, continuous batching under an HBM budget: p50/p99
                 latency and recons/s
+  large_4096    one 4096^2 reconstruction (polygon ~8 %, amplitude U[0.3, 1],
+                Poisson 1e9; 2000 iterations 90 HIO / 10 ER with shrinkwrap
+                every 20): iterations/s, per-pass times and HBM fractions
   cpu_baseline  the restated reference numpy path on the host cores
                 (bounded sample of the headline frame)
 """
@@ -55,6 +58,7 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--no-large", action="store_true")
     ap.add_argument("--requests", type=int, default=2000)
     ap.add_argument("--budget-gb", type=float, default=48.0)
     ap.add_argument("--kernel-probe", action="store_true",
@@ -280,6 +284,60 @@ def bench_batched_1024(ctx, frame, trials=16):
             "row_pass_hbm_frac": round(B_ROW * px / (row_us * 1e-6) / HBM_PEAK, 4)}
 
 
+def bench_large_4096(ctx):
+    """The large-frame workload: one 4096^2 reconstruction of 2000 iterations."""
+    import torch
+    from als_speckle_amd import synth
+    from als_speckle_amd.device import Plan
+    from als_speckle_amd.phasing import expand_order, shrinkwrap_schedule
+    from als_speckle_amd import _capi
+    c = synth.CONFIGS["large_4096"]
+    n, iters = c["n"], c["iterations"]
+    frame = synth.config_frame("large_4096", 0)
+    algs = expand_order(c["order"], iters)
+    sw = shrinkwrap_schedule(iters, **c["shrinkwrap"])
+    steps = [(_capi.ALG_CODES[x], _capi.SEM_NUMPY, c["beta"], *sw.get(k, (0.0, 0.0)))
+             for k, x in enumerate(algs)]
+    dev = ctx.device
+    plan = Plan(ctx, n, batch=1, shared=True)
+    s_d = torch.from_numpy(frame.support).to(dev)
+    np.random.seed(1)
+    g_d = torch.from_numpy((np.random.rand(n, n) + 1j * np.random.rand(n, n))
+                           .astype(np.complex64)).to(dev)
+    plan.set_modulus(torch.from_numpy(frame.modulus).to(dev))
+    err = torch.empty((iters, 1), dtype=torch.float64, device=dev)
+
+    def run():
+        plan.set_support(s_d)
+        plan.set_psi(g_d, mult_support=True)
+        plan.iterate(steps, err_out=err)
+
+    run()
+    ctx.sync()
+    t0 = time.perf_counter()
+    run()
+    ctx.sync()
+    dt = time.perf_counter() - t0
+    final_err = float(err[-1, 0].item())
+    plan.profile(True)
+    run()
+    ctx.sync()
+    nc, tc = plan.profile_read(0)
+    nr, tr = plan.profile_read(1)
+    plan.profile(False)
+    plan.close()
+    px = n * n
+    col_us, row_us = tc / nc * 1e3, tr / nr * 1e3
+    model = iters * (B_COL + B_ROW) * px + len(sw) * B_SHRINK * px + B_ONEOFF * px
+    return {"n": n, "iterations": iters, "iterations_per_s": round(iters / dt, 1),
+            "recons_per_s": round(1.0 / dt, 4), "final_modulus_error": final_err,
+            "tile": plan.tile, "values_per_thread": plan.ept,
+            "model_hbm_frac_whole_recon": round(model / dt / HBM_PEAK, 4),
+            "column_pass_us": round(col_us, 2), "row_pass_us": round(row_us, 2),
+            "column_pass_hbm_frac": round(B_COL * px / (col_us * 1e-6) / HBM_PEAK, 4),
+            "row_pass_hbm_frac": round(B_ROW * px / (row_us * 1e-6) / HBM_PEAK, 4)}
+
+
 def bench_serving(a):
     """Poisson arrivals of mixed frame sizes through the continuous-batching server."""
     import torch
@@ -418,6 +476,9 @@ def main():
         b16 = bench_batched_1024(ctx, s["frame"])
         b16["speedup_vs_single"] = round(b16["recons_per_s"] / plain_rps, 3)
         out["batched"]["trials_1024"] = b16
+    if not a.no_large:
+        log("large 4096^2 reconstruction")
+        out["large_4096"] = bench_large_4096(ctx)
     if not a.no_serving:
         log(f"serving {a.requests} requests")
         out["serving"] = bench_serving(a)
