@@ -478,18 +478,29 @@ int spk_plan_create(spk_ctx* ctx, int n, int batch, int shared, const spk_tune* 
   if ((rc = dalloc(&p->psi, nn * batch)) || (rc = dalloc(&p->phi, nn * batch)) ||
       (p->dual() && (rc = dalloc(&p->phic, nn * batch))) ||
       (rc = dalloc(&p->mod, nn * fms)) || (rc = dalloc(&p->sup, nn * fms)) ||
-      (rc = dalloc(&p->tw, (size_t)n)) || (rc = dalloc(&p->summ2, (size_t)batch)) ||
+      (rc = dalloc(&p->tw, (size_t)2 * n)) || (rc = dalloc(&p->summ2, (size_t)batch)) ||
       (rc = dalloc(&p->sumpart, (size_t)64 * batch)) || (rc = dalloc(&p->pbar, (size_t)2))) {
     spk_plan_destroy(p);
     return rc;
   }
-  // forward twiddles W_n^k = exp(-2 pi i k / n), evaluated in FP64
-  std::vector<c32> tw(n);
-  for (int k = 0; k < n; ++k) {
-    const double ang = -2.0 * M_PI * (double)k / (double)n;
-    tw[k] = c32{(float)std::cos(ang), (float)std::sin(ang)};
+  // forward twiddles W_n^m = exp(-2 pi i m / n), evaluated in FP64: the
+  // exponent-indexed table, then the per-stage table of the plan's radix
+  // schedule (the same values, k-major per stage; zero padded to n entries)
+  std::vector<c32> tw((size_t)2 * n, c32{0.0f, 0.0f});
+  auto w = [n](int m) {
+    const double ang = -2.0 * M_PI * (double)m / (double)n;
+    return c32{(float)std::cos(ang), (float)std::sin(ang)};
+  };
+  for (int m = 0; m < n; ++m) tw[m] = w(m);
+  for (int st = 1; st < plan_stages(n, ept); ++st) {
+    const int R = plan_radix(n, ept, st), off = staged_tw_offset(n, ept, st);
+    int ns = 1;
+    for (int i = 0; i < st; ++i) ns *= plan_radix(n, ept, i);
+    for (int k = 0; k < ns; ++k)
+      for (int r = 1; r < R; ++r)
+        tw[(size_t)n + off + k * (R - 1) + (r - 1)] = w(staged_tw_exponent(n, ept, st, k, r));
   }
-  hipError_t e = hipMemcpy(p->tw, tw.data(), n * sizeof(c32), hipMemcpyHostToDevice);
+  hipError_t e = hipMemcpy(p->tw, tw.data(), tw.size() * sizeof(c32), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemset(p->pbar, 0, 2 * sizeof(unsigned));
   if (e == hipSuccess) e = hipEventCreateWithFlags(&p->upload_done, hipEventDisableTiming);
   if (e != hipSuccess) {

@@ -197,6 +197,47 @@ struct RadixPlan {
   static_assert(N % E == 0, "E must divide N");
 };
 
+// The same plan at run time (host code that lays out twiddle tables).
+SPK_HD constexpr int plan_radix(int n, int e, int s) {
+  const int maxr = e < 16 ? e : 16;
+  const int full = ilog2(n) / ilog2(maxr);
+  return s < full ? maxr : (1 << (ilog2(n) - full * ilog2(maxr)));
+}
+SPK_HD constexpr int plan_stages(int n, int e) {
+  const int maxr = e < 16 ? e : 16;
+  const int full = ilog2(n) / ilog2(maxr);
+  return full + (ilog2(n) - full * ilog2(maxr) ? 1 : 0);
+}
+
+// ---------------------------------------------------------------------------
+// Per-stage twiddle table.  Stage s >= 1 (span NS, radix R) multiplies value
+// r of butterfly j by W_N^(r k N/(NS R)), k = j mod NS.  Its block holds
+// these as [k][r - 1] (k-major, R - 1 entries per k), blocks in stage order
+// from stage 1; the table has N - R_0 entries.  Lanes with consecutive k read
+// entries R - 1 apart: an odd number of 8-byte entries, so a wavefront's
+// reads spread over the LDS banks (in the exponent-indexed table the lanes of
+// one stage read entries a power of two apart, which share banks).
+// ---------------------------------------------------------------------------
+SPK_HD constexpr int staged_tw_offset(int n, int e, int s) {
+  int off = 0, ns = 1;
+  for (int i = 0; i < s; ++i) {
+    const int r = plan_radix(n, e, i);
+    if (i >= 1) off += ns * (r - 1);
+    ns *= r;
+  }
+  return off;
+}
+// exponent m of entry (stage s, k, r): the table entry is W_N^m
+SPK_HD constexpr int staged_tw_exponent(int n, int e, int s, int k, int r) {
+  int ns = 1;
+  for (int i = 0; i < s; ++i) ns *= plan_radix(n, e, i);
+  return r * k * (n / (ns * plan_radix(n, e, s)));
+}
+
+struct TwStaged {  // the per-stage table (LDS)
+  const c32* p;
+};
+
 // Output position of value r of butterfly j at a stage with span ns, radix R.
 SPK_HD int stockham_out(int j, int r, int ns, int R) {
   return (j / ns) * ns * R + (j % ns) + r * ns;
@@ -308,6 +349,18 @@ struct TwTwoLevel {  // W^m = hi[m >> 6] * lo[m & 63]; hi: N/64 entries, lo: 64
   SPK_HD c32 operator()(int m) const { return mul(p[m >> 6], p[(N >> 6) + (m & 63)]); }
 };
 
+// Twiddle r of butterfly index k at stage S (exponent m = r k N/(NS R)).
+template <int N, int E, int S, class TW>
+SPK_HD c32 tw_get(const TW& tw, int k, int r, int m) {
+  return tw(m);
+}
+template <int N, int E, int S>
+SPK_HD c32 tw_get(const TwStaged& tw, int k, int r, int m) {
+  constexpr int OFF = staged_tw_offset(N, E, S);
+  constexpr int R = plan_radix(N, E, S);
+  return tw.p[OFF + k * (R - 1) + (r - 1)];
+}
+
 template <int N, int E, int TT, bool INV, int S, class TW>
 SPK_HD void stage_butterflies(c32 (&x)[E], int t, int b, c32* lds, const TW& tw) {
   using P = RadixPlan<N, E>;
@@ -326,7 +379,7 @@ SPK_HD void stage_butterflies(c32 (&x)[E], int t, int b, c32* lds, const TW& tw)
     if constexpr (NS > 1) {
       const int k = j & (NS - 1);
       constexpr int STRIDE = N / (NS * R);
-      for (int r = 1; r < R; ++r) v[r] = twiddle<INV>(v[r], tw((r * k) * STRIDE));
+      for (int r = 1; r < R; ++r) v[r] = twiddle<INV>(v[r], tw_get<N, E, S>(tw, k, r, (r * k) * STRIDE));
     }
     dft<R, INV>(v);
     if constexpr (LAST) {

@@ -33,8 +33,11 @@ namespace spk {
 // the twiddle table fit in the 160 KB of a gfx950 CU and the second buffer
 // does not lower the number of workgroups the LDS admits per CU (counted up
 // to 4): measured at 1024^2 with 16 frames, 3 resident workgroups with one
-// buffer beat 2 with two by 17 % (column pass) and 4 % (row pass).  The full
-// twiddle table (N entries) is used where it fits, else the two-level one.
+// buffer beat 2 with two by 17 % (column pass) and 4 % (row pass).  The
+// per-stage twiddle table (TwStaged, N - R_0 entries, N reserved) is used
+// where it fits, else the two-level one.  The plan's global table holds the
+// exponent-indexed W_N^m (m < N, for the two-level form) followed by the
+// per-stage table of the plan's radix schedule.
 // ---------------------------------------------------------------------------
 constexpr int kLdsBytes = 160 * 1024;
 
@@ -73,7 +76,7 @@ struct TwStage {
     for (int k = 0; k < PER; ++k) {
       const int i = (int)threadIdx.x + k * NT;
       if (CNT % NT == 0 || i < CNT) {
-        if constexpr (L::FULL_TW) v[k] = tw[i];
+        if constexpr (L::FULL_TW) v[k] = tw[N + i];
         else v[k] = i < (N >> 6) ? tw[i * 64] : tw[i - (N >> 6)];
       }
     }
@@ -102,7 +105,7 @@ __device__ __forceinline__ void gst(T* base, unsigned i, const T& v) {
 
 template <int N, int TT>
 __device__ __forceinline__ auto twiddle_view(const c32* twl) {
-  if constexpr (LdsPlan<N, TT>::FULL_TW) return TwTable{twl};
+  if constexpr (LdsPlan<N, TT>::FULL_TW) return TwStaged{twl};
   else return TwTwoLevel<N>{twl};
 }
 

@@ -87,12 +87,25 @@ int fft2_tiled_tw(int inverse, const c32* in, c32* out, const TW& tw) {
   return 0;
 }
 
-// twiddles as the kernels use them: the full table (N <= 2048) or the
-// two-level table (hi = W^(64 i), lo = W^j) where LDS is short (N = 4096)
+// twiddles as the kernels use them: tw_mode 2, the per-stage table of the
+// radix schedule (where it fits in LDS), 1, the two-level table (hi =
+// W^(64 i), lo = W^j, where LDS is short), or 0, the exponent-indexed table
 template <int N, int TT, int E>
-int fft2_tiled(int inverse, const c32* in, c32* out, int two_level) {
+int fft2_tiled(int inverse, const c32* in, c32* out, int tw_mode) {
   const std::vector<c32> tw = twiddles(N);
-  if (!two_level) return fft2_tiled_tw<N, TT, E>(inverse, in, out, TwTable{tw.data()});
+  if (tw_mode == 0) return fft2_tiled_tw<N, TT, E>(inverse, in, out, TwTable{tw.data()});
+  if (tw_mode == 2) {
+    std::vector<c32> st((size_t)N, c32{0.0f, 0.0f});
+    for (int s = 1; s < plan_stages(N, E); ++s) {
+      const int R = plan_radix(N, E, s), off = staged_tw_offset(N, E, s);
+      int ns = 1;
+      for (int i = 0; i < s; ++i) ns *= plan_radix(N, E, i);
+      for (int k = 0; k < ns; ++k)
+        for (int r = 1; r < R; ++r)
+          st[off + k * (R - 1) + (r - 1)] = tw[staged_tw_exponent(N, E, s, k, r)];
+    }
+    return fft2_tiled_tw<N, TT, E>(inverse, in, out, TwStaged{st.data()});
+  }
   if constexpr (N >= 64) {
     std::vector<c32> hl((N >> 6) + 64);
     for (int i = 0; i < (N >> 6); ++i) hl[i] = tw[i * 64];
@@ -107,18 +120,18 @@ int fft2_tiled(int inverse, const c32* in, c32* out, int two_level) {
 extern "C" {
 
 // 2-D transform of an n x n complex64 array through the tiled layout
-int emu_fft2(int n, int tile, int inverse, const float* in, float* out, int two_level,
+int emu_fft2(int n, int tile, int inverse, const float* in, float* out, int tw_mode,
              int ept) {
   const c32* i = reinterpret_cast<const c32*>(in);
   c32* o = reinterpret_cast<c32*>(out);
 #define EMU_CASE(NN)                                            \
   if (n == NN) {                                                \
-    if (ept == 16 && tile == 8) return fft2_tiled<NN, 8, 16>(inverse, i, o, two_level); \
-    if (ept == 16 && tile == 4) return fft2_tiled<NN, 4, 16>(inverse, i, o, two_level); \
-    if (ept == 8 && tile == 8) return fft2_tiled<NN, 8, 8>(inverse, i, o, two_level);   \
-    if (ept == 8 && tile == 4) return fft2_tiled<NN, 4, 8>(inverse, i, o, two_level);   \
-    if (ept == 16 && tile == 2) return fft2_tiled<NN, 2, 16>(inverse, i, o, two_level); \
-    if (ept == 8 && tile == 2) return fft2_tiled<NN, 2, 8>(inverse, i, o, two_level);   \
+    if (ept == 16 && tile == 8) return fft2_tiled<NN, 8, 16>(inverse, i, o, tw_mode); \
+    if (ept == 16 && tile == 4) return fft2_tiled<NN, 4, 16>(inverse, i, o, tw_mode); \
+    if (ept == 8 && tile == 8) return fft2_tiled<NN, 8, 8>(inverse, i, o, tw_mode);   \
+    if (ept == 8 && tile == 4) return fft2_tiled<NN, 4, 8>(inverse, i, o, tw_mode);   \
+    if (ept == 16 && tile == 2) return fft2_tiled<NN, 2, 16>(inverse, i, o, tw_mode); \
+    if (ept == 8 && tile == 2) return fft2_tiled<NN, 2, 8>(inverse, i, o, tw_mode);   \
     return -1;                                                  \
   }
   EMU_CASE(16) EMU_CASE(32) EMU_CASE(64) EMU_CASE(128) EMU_CASE(256) EMU_CASE(512)
