@@ -225,11 +225,25 @@ __device__ __forceinline__ void blur_window(const float* line, int c0, const flo
 }
 
 // Fused shrinkwrap (kSwRow / kSwFin steps): the blur of the passes' own
-// bands, in the LDS of the pass once its transforms are done.  A line of the
-// blur is TT x (N + N/8 + 8) floats from the start of the pass's LDS, which
-// the exchange buffers exceed.
+// bands, in the LDS of the pass once its transforms are done.  The TT lines
+// of the blur take TT x (N + N/8 + 8) floats from the start of the pass's
+// LDS; the outputs are staged after them (N x TT floats, in the order of the
+// band's global layout) and stored as float4s, so that a wavefront writes
+// whole tiles instead of one float per tile.
 template <int N>
 constexpr int sw_line() { return N + N / 8 + 8; }
+template <int N, int TT>
+constexpr int sw_stage_off() { return (TT * sw_line<N>() + 3) & ~3; }
+template <int N, int TT>
+constexpr int sw_lds_floats() { return sw_stage_off<N, TT>() + N * TT; }
+
+// Store the staged band (N x TT floats) as float4s: group v of four staged
+// floats goes to dst + off(v).
+template <int N, int TT, class OFF>
+__device__ __forceinline__ void sw_store_staged(const float* stage, float* dst, OFF off) {
+  for (int v = threadIdx.x; v < N * TT / 4; v += blockDim.x)
+    *reinterpret_cast<float4*>(dst + off(v)) = *reinterpret_cast<const float4*>(stage + 4 * v);
+}
 
 __device__ __forceinline__ void sw_weights(const BlurW& bw, float (&w)[kSwFuseRmax + 1]) {
 #pragma unroll
@@ -383,9 +397,15 @@ __device__ __forceinline__ void colpass_item(const PassArgs& a, int f, int band,
       float w[kSwFuseRmax + 1];
       sw_weights(a.swf_tab[a.sw_pf ? f : 0].w, w);
       __syncthreads();
-      float* blur = a.sw_blur + fo;
+      // staged in tile order: tile q (rows qTT..qTT+TT-1 of the band) at q TT^2
+      float* stage = line + sw_stage_off<N, TT>();
       const float mx = sw_blur_lines<N, TT>(line, w, [&](int l, int row, float v) {
-        blur[((row / TT) * NB + band) * (TT * TT) + (row % TT) * TT + l] = v;
+        stage[(row / TT) * (TT * TT) + (row % TT) * TT + l] = v;
+      });
+      __syncthreads();
+      sw_store_staged<N, TT>(stage, a.sw_blur + fo, [&](int v) {
+        const int q = (4 * v) / (TT * TT), within = (4 * v) % (TT * TT);
+        return (q * NB + band) * (TT * TT) + within;
       });
       const float bm = block_max(mx, line);
       if (tid == 0) a.sw_maxp[(size_t)f * NB + band] = bm;
@@ -522,7 +542,7 @@ template <int N, int TT, int E, int MODE>
 __global__ void __launch_bounds__((N / E) * TT) k_colpass(PassArgs a) {
   constexpr int NB = N / TT;
   __shared__ c32 lds[LdsPlan<N, TT>::TOTAL];
-  static_assert(MODE != 9 || TT * sw_line<N>() * sizeof(float) <= sizeof(lds),
+  static_assert(MODE != 9 || sw_lds_floats<N, TT>() * sizeof(float) <= sizeof(lds),
                 "fused blur lines exceed the pass's LDS");
   const int band = xcd_band<NB>(blockIdx.x), z = blockIdx.z;
   const int f = a.fmap ? a.fmap[blockIdx.y] : blockIdx.y;
@@ -698,10 +718,14 @@ __device__ __forceinline__ void rowpass_item(const PassArgs& a, int f, int band,
       float w[kSwFuseRmax + 1];
       sw_weights(a.swr_tab[a.sw_pf ? f : 0].w, w);
       __syncthreads();
-      float* tmp = a.sw_tmp + fo + (size_t)band * N * TT;
+      // staged in the band's (contiguous) global order
+      float* stage = line + sw_stage_off<N, TT>();
       sw_blur_lines<N, TT>(line, w, [&](int l, int col, float v) {
-        tmp[(col / TT) * (TT * TT) + l * TT + (col % TT)] = v;
+        stage[(col / TT) * (TT * TT) + l * TT + (col % TT)] = v;
       });
+      __syncthreads();
+      sw_store_staged<N, TT>(stage, a.sw_tmp + fo + (size_t)band * N * TT,
+                             [](int v) { return 4 * v; });
     }
   }
 #undef SPK_RPOS
@@ -711,7 +735,7 @@ __device__ __forceinline__ void rowpass_item(const PassArgs& a, int f, int band,
 template <int N, int TT, int E, bool USE_T, bool EMIT_B, int SW>
 __global__ void __launch_bounds__((N / E) * TT) k_rowpass(PassArgs a) {
   __shared__ c32 lds[LdsPlan<N, TT>::TOTAL];
-  static_assert(SW == 0 || TT * sw_line<N>() * sizeof(float) <= sizeof(lds),
+  static_assert(SW == 0 || sw_lds_floats<N, TT>() * sizeof(float) <= sizeof(lds),
                 "fused blur lines exceed the pass's LDS");
   const int f = a.fmap ? a.fmap[blockIdx.y] : blockIdx.y;
   const Coef c = a.tab ? a.tab[f] : a.c;
