@@ -320,15 +320,21 @@ SPK_HD int cbrow_pos(int band, int b, int t, int i) {
 }
 
 // LDS slot of element p of transform b (TT transforms per workgroup,
-// interleaved, one pad slot per 16 to spread 8-byte accesses over banks).
-template <int TT>
+// interleaved, one pad slot per lds_pad<N>() slots to spread 8-byte accesses
+// over banks).  Measured on MI355X: one per 32 is 1-3 % faster per pass from
+// 1024 up, one per 16 is 2 % faster at 512^2 x 64 frames.
+template <int N>
+constexpr int lds_pad() {
+  return N >= 1024 ? 32 : 16;
+}
+template <int N, int TT>
 SPK_HD int lds_idx(int p, int b) {
   const int i = p * TT + b;
-  return i + (i >> 4);
+  return i + i / lds_pad<N>();
 }
 template <int N, int TT>
 constexpr int lds_elems() {
-  return N * TT + ((N * TT) >> 4) + 1;
+  return N * TT + (N * TT) / lds_pad<N>() + 1;
 }
 
 // ---------------------------------------------------------------------------
@@ -388,19 +394,20 @@ SPK_HD void stage_butterflies(c32 (&x)[E], int t, int b, c32* lds, const TW& tw)
     } else {
       const int base = (j / NS) * NS * R + (j & (NS - 1));
       // slots of positions base + r*NS.  The pad term of lds_idx is linear in
-      // r when the steps are whole 16-slot groups, so the R slots are one
+      // r when the steps are whole pad groups, so the R slots are one
       // base plus compile-time offsets (immediate LDS offsets, no per-value
       // index arithmetic).
-      if constexpr ((NS * TT) % 16 == 0) {
-        const int i0 = lds_idx<TT>(base, b);
-        for (int r = 0; r < R; ++r) lds[i0 + r * (NS * TT + NS * TT / 16)] = v[r];
-      } else if constexpr (NS == 1 && (R * TT) % 16 == 0) {
-        // base = j*R starts a 16-slot group; offsets r*TT stay inside the
-        // groups they start in (TT divides 16, b < TT)
-        const int i0 = j * (R * TT + R * TT / 16) + b;
-        for (int r = 0; r < R; ++r) lds[i0 + r * TT + ((r * TT) >> 4)] = v[r];
+      constexpr int PG = lds_pad<N>();
+      if constexpr ((NS * TT) % PG == 0) {
+        const int i0 = lds_idx<N, TT>(base, b);
+        for (int r = 0; r < R; ++r) lds[i0 + r * (NS * TT + NS * TT / PG)] = v[r];
+      } else if constexpr (NS == 1 && (R * TT) % PG == 0) {
+        // base = j*R starts a pad group; offsets r*TT stay inside the groups
+        // they start in (TT divides the group, b < TT)
+        const int i0 = j * (R * TT + R * TT / PG) + b;
+        for (int r = 0; r < R; ++r) lds[i0 + r * TT + (r * TT) / PG] = v[r];
       } else {
-        for (int r = 0; r < R; ++r) lds[lds_idx<TT>(base + r * NS, b)] = v[r];
+        for (int r = 0; r < R; ++r) lds[lds_idx<N, TT>(base + r * NS, b)] = v[r];
       }
     }
   }
@@ -409,11 +416,12 @@ SPK_HD void stage_butterflies(c32 (&x)[E], int t, int b, c32* lds, const TW& tw)
 template <int N, int E, int TT>
 SPK_HD void stage_gather(c32 (&x)[E], int t, int b, const c32* lds) {
   constexpr int T = N / E;
-  if constexpr ((T * TT) % 16 == 0) {  // linear slots, as in stage_butterflies
-    const int i0 = lds_idx<TT>(t, b);
-    for (int i = 0; i < E; ++i) x[i] = lds[i0 + i * (T * TT + T * TT / 16)];
+  constexpr int PG = lds_pad<N>();
+  if constexpr ((T * TT) % PG == 0) {  // linear slots, as in stage_butterflies
+    const int i0 = lds_idx<N, TT>(t, b);
+    for (int i = 0; i < E; ++i) x[i] = lds[i0 + i * (T * TT + T * TT / PG)];
   } else {
-    for (int i = 0; i < E; ++i) x[i] = lds[lds_idx<TT>(t + i * T, b)];
+    for (int i = 0; i < E; ++i) x[i] = lds[lds_idx<N, TT>(t + i * T, b)];
   }
 }
 

@@ -236,6 +236,12 @@ template <int N, int TT>
 constexpr int sw_stage_off() { return (TT * sw_line<N>() + 3) & ~3; }
 template <int N, int TT>
 constexpr int sw_lds_floats() { return sw_stage_off<N, TT>() + N * TT; }
+// the pass kernels' LDS holds the lines and the staged outputs (else the
+// outputs go straight to global memory)
+template <int N, int TT>
+constexpr bool sw_stage_fits() {
+  return sw_lds_floats<N, TT>() * 4 <= LdsPlan<N, TT>::TOTAL * 8;
+}
 
 // Store the staged band (N x TT floats) as float4s: group v of four staged
 // floats goes to dst + off(v).
@@ -397,16 +403,24 @@ __device__ __forceinline__ void colpass_item(const PassArgs& a, int f, int band,
       float w[kSwFuseRmax + 1];
       sw_weights(a.swf_tab[a.sw_pf ? f : 0].w, w);
       __syncthreads();
-      // staged in tile order: tile q (rows qTT..qTT+TT-1 of the band) at q TT^2
-      float* stage = line + sw_stage_off<N, TT>();
-      const float mx = sw_blur_lines<N, TT>(line, w, [&](int l, int row, float v) {
-        stage[(row / TT) * (TT * TT) + (row % TT) * TT + l] = v;
-      });
-      __syncthreads();
-      sw_store_staged<N, TT>(stage, a.sw_blur + fo, [&](int v) {
-        const int q = (4 * v) / (TT * TT), within = (4 * v) % (TT * TT);
-        return (q * NB + band) * (TT * TT) + within;
-      });
+      float mx;
+      if constexpr (sw_stage_fits<N, TT>()) {
+        // staged in tile order: tile q (rows qTT..qTT+TT-1 of the band) at q TT^2
+        float* stage = line + sw_stage_off<N, TT>();
+        mx = sw_blur_lines<N, TT>(line, w, [&](int l, int row, float v) {
+          stage[(row / TT) * (TT * TT) + (row % TT) * TT + l] = v;
+        });
+        __syncthreads();
+        sw_store_staged<N, TT>(stage, a.sw_blur + fo, [&](int v) {
+          const int q = (4 * v) / (TT * TT), within = (4 * v) % (TT * TT);
+          return (q * NB + band) * (TT * TT) + within;
+        });
+      } else {
+        float* blur = a.sw_blur + fo;
+        mx = sw_blur_lines<N, TT>(line, w, [&](int l, int row, float v) {
+          blur[((row / TT) * NB + band) * (TT * TT) + (row % TT) * TT + l] = v;
+        });
+      }
       const float bm = block_max(mx, line);
       if (tid == 0) a.sw_maxp[(size_t)f * NB + band] = bm;
     }
@@ -542,7 +556,7 @@ template <int N, int TT, int E, int MODE>
 __global__ void __launch_bounds__((N / E) * TT) k_colpass(PassArgs a) {
   constexpr int NB = N / TT;
   __shared__ c32 lds[LdsPlan<N, TT>::TOTAL];
-  static_assert(MODE != 9 || sw_lds_floats<N, TT>() * sizeof(float) <= sizeof(lds),
+  static_assert(MODE != 9 || TT * sw_line<N>() * sizeof(float) <= sizeof(lds),
                 "fused blur lines exceed the pass's LDS");
   const int band = xcd_band<NB>(blockIdx.x), z = blockIdx.z;
   const int f = a.fmap ? a.fmap[blockIdx.y] : blockIdx.y;
@@ -718,14 +732,20 @@ __device__ __forceinline__ void rowpass_item(const PassArgs& a, int f, int band,
       float w[kSwFuseRmax + 1];
       sw_weights(a.swr_tab[a.sw_pf ? f : 0].w, w);
       __syncthreads();
-      // staged in the band's (contiguous) global order
-      float* stage = line + sw_stage_off<N, TT>();
-      sw_blur_lines<N, TT>(line, w, [&](int l, int col, float v) {
-        stage[(col / TT) * (TT * TT) + l * TT + (col % TT)] = v;
-      });
-      __syncthreads();
-      sw_store_staged<N, TT>(stage, a.sw_tmp + fo + (size_t)band * N * TT,
-                             [](int v) { return 4 * v; });
+      float* tmp = a.sw_tmp + fo + (size_t)band * N * TT;
+      if constexpr (sw_stage_fits<N, TT>()) {
+        // staged in the band's (contiguous) global order
+        float* stage = line + sw_stage_off<N, TT>();
+        sw_blur_lines<N, TT>(line, w, [&](int l, int col, float v) {
+          stage[(col / TT) * (TT * TT) + l * TT + (col % TT)] = v;
+        });
+        __syncthreads();
+        sw_store_staged<N, TT>(stage, tmp, [](int v) { return 4 * v; });
+      } else {
+        sw_blur_lines<N, TT>(line, w, [&](int l, int col, float v) {
+          tmp[(col / TT) * (TT * TT) + l * TT + (col % TT)] = v;
+        });
+      }
     }
   }
 #undef SPK_RPOS
@@ -735,7 +755,7 @@ __device__ __forceinline__ void rowpass_item(const PassArgs& a, int f, int band,
 template <int N, int TT, int E, bool USE_T, bool EMIT_B, int SW>
 __global__ void __launch_bounds__((N / E) * TT) k_rowpass(PassArgs a) {
   __shared__ c32 lds[LdsPlan<N, TT>::TOTAL];
-  static_assert(SW == 0 || sw_lds_floats<N, TT>() * sizeof(float) <= sizeof(lds),
+  static_assert(SW == 0 || TT * sw_line<N>() * sizeof(float) <= sizeof(lds),
                 "fused blur lines exceed the pass's LDS");
   const int f = a.fmap ? a.fmap[blockIdx.y] : blockIdx.y;
   const Coef c = a.tab ? a.tab[f] : a.c;
