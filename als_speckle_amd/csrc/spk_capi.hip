@@ -120,11 +120,12 @@ struct spk_plan {
   unsigned char* stage_d = nullptr;
   unsigned char* stage_h = nullptr;
   unsigned char* stage_cur = nullptr;  // device tables of the latest spk_iterate
-  // Under graph capture the tables go to buffers of the capture's own (one
-  // pair per captured graph, kept until the plan is destroyed): a replay
+  // Under graph capture the tables go to buffers of the capture's own (a
+  // device + pinned pair that the resulting graph owns and frees): a replay
   // copies them from pinned memory that later calls do not overwrite.
   bool capturing = false;
-  std::vector<std::pair<unsigned char*, unsigned char*>> cap_stages;  // (device, pinned)
+  unsigned char* cap_d = nullptr;
+  unsigned char* cap_h = nullptr;
   size_t cap_used = 0;
   size_t stage_cap = 0;
   hipEvent_t upload_done = nullptr;
@@ -307,11 +308,11 @@ int upload_stage(spk_plan* p, const Stage& stg) {
   const size_t need = stg.buf.size();
   if (need == 0) return SPK_OK;
   if (p->capturing) {
-    SPK_REQUIRE(!p->cap_stages.empty() && p->cap_used + need <= kCaptureStage, SPK_EUNSUP,
+    SPK_REQUIRE(p->cap_d && p->cap_used + need <= kCaptureStage, SPK_EUNSUP,
                 "the step tables of a captured block exceed " +
                     std::to_string(kCaptureStage >> 20) + " MiB");
-    unsigned char* d = p->cap_stages.back().first + p->cap_used;
-    unsigned char* h = p->cap_stages.back().second + p->cap_used;
+    unsigned char* d = p->cap_d + p->cap_used;
+    unsigned char* h = p->cap_h + p->cap_used;
     std::memcpy(h, stg.buf.data(), need);
     SPK_HIP(hipMemcpyAsync(d, h, need, hipMemcpyHostToDevice, p->ctx->stream));
     p->cap_used += (need + 255) & ~(size_t)255;
@@ -557,10 +558,8 @@ int spk_plan_destroy(spk_plan* p) {
   dfree(p->coh_ac); dfree(p->coh_w); dfree(p->coh_cb); dfree(p->coh_m); dfree(p->coh_part);
   dfree(p->sw_tmp); dfree(p->sw_blur); dfree(p->sw_maxp); dfree(p->fftbuf);
   dfree(p->stage_d);
-  for (auto& cs : p->cap_stages) {
-    dfree(cs.first);
-    (void)hipHostFree(cs.second);
-  }
+  dfree(p->cap_d);  // a capture that was never ended
+  if (p->cap_h) (void)hipHostFree(p->cap_h);
   prof_clear(p, 0);
   prof_clear(p, 1);
   if (p->stage_h) (void)hipHostFree(p->stage_h);
@@ -1446,6 +1445,8 @@ struct spk_graph {
   spk_ctx* ctx = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  unsigned char* stage_d = nullptr;  // the captured step tables (device, pinned)
+  unsigned char* stage_h = nullptr;
 };
 
 int spk_graph_begin(spk_plan* p) {
@@ -1463,7 +1464,8 @@ int spk_graph_begin(spk_plan* p) {
     dfree(d);
     return fail(SPK_EHIP, "hipHostMalloc of the captured step tables failed");
   }
-  p->cap_stages.emplace_back(d, h);
+  p->cap_d = d;
+  p->cap_h = h;
   p->cap_used = 0;
   SPK_HIP(hipStreamBeginCapture(p->ctx->stream, hipStreamCaptureModeThreadLocal));
   p->capturing = true;
@@ -1477,16 +1479,31 @@ int spk_graph_end(spk_plan* p, spk_graph** out) {
   if (rc) return rc;
   hipGraph_t g = nullptr;
   p->capturing = false;
-  SPK_HIP(hipStreamEndCapture(p->ctx->stream, &g));
+  // the table buffers go with the graph (or are freed if there is none)
+  unsigned char* sd = p->cap_d;
+  unsigned char* sh = p->cap_h;
+  p->cap_d = p->cap_h = nullptr;
+  auto drop_stage = [&]() {
+    dfree(sd);
+    if (sh) (void)hipHostFree(sh);
+  };
+  hipError_t e = hipStreamEndCapture(p->ctx->stream, &g);
+  if (e != hipSuccess) {
+    drop_stage();
+    return fail(SPK_EHIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+  }
   spk_graph* h = new spk_graph;
   h->ctx = p->ctx;
   h->graph = g;
-  hipError_t e = hipGraphInstantiate(&h->exec, g, nullptr, nullptr, 0);
+  e = hipGraphInstantiate(&h->exec, g, nullptr, nullptr, 0);
   if (e != hipSuccess) {
     (void)hipGraphDestroy(g);
     delete h;
+    drop_stage();
     return fail(SPK_EHIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
   }
+  h->stage_d = sd;
+  h->stage_h = sh;
   p->ctx->refs++;
   *out = h;
   return SPK_OK;
@@ -1506,6 +1523,8 @@ int spk_graph_destroy(spk_graph* h) {
   (void)hipStreamSynchronize(h->ctx->stream);
   if (h->exec) (void)hipGraphExecDestroy(h->exec);
   if (h->graph) (void)hipGraphDestroy(h->graph);
+  dfree(h->stage_d);
+  if (h->stage_h) (void)hipHostFree(h->stage_h);
   ctx_release(h->ctx);
   delete h;
   return SPK_OK;

@@ -188,6 +188,9 @@ int spk_copy_roi(spk_plan* plan, int frame, float* dst_c64, int slots,
  * spk_graph_end instantiates them.  spk_graph_launch replays the recorded
  * work on the plan's stream with the same pointers and step coefficients.
  * Workspace must already exist: run the same calls once before capturing.
+ * The step tables of the recorded spk_iterate calls (at most 4 MiB per
+ * graph) live in buffers the graph owns, so eager calls between replays do
+ * not change what a replay runs; spk_graph_destroy frees them.
  * No counterpart in the reference (its launches block one by one,
  * gpu.py:349).
  * ---------------------------------------------------------------------- */

@@ -362,3 +362,44 @@ def test_graph_replay_matches_eager():
             p.iterate([step("hio", 0.5, "numpy", 1.0 + rep, 0.3)] * 4 + [step("er")])
             ctx.sync()
     graph.close()
+
+
+def test_two_graphs_of_one_plan_replay_their_own_tables():
+    """Two blocks with different step tables captured from one plan: each
+    graph replays its own block (the tables live in the graph), also after
+    the other graph has been destroyed."""
+    from als_speckle_amd.device import Context
+    n = 64
+    f = frame(n, seed=13)
+    psi = start_psi(n, f.support, seed=5)
+    stream = torch.cuda.Stream()
+    ctx = Context(0, stream=stream)
+    p = Plan(ctx, n)
+    m, s, g = dev(f.modulus, torch.float32), dev(f.support, torch.float32), dev(psi, torch.complex64)
+    torch.cuda.synchronize()
+    scheds = [[step("hio", 0.9, "numpy", 1.5 if k == 9 else 0.0, 0.2) for k in range(12)],
+              [step("raar", 0.8)] * 6 + [step("er", 0.9, "numpy", 2.0, 0.25)] + [step("er")] * 3]
+
+    def block(sched):
+        p.set_support(s)
+        p.set_psi(g, mult_support=False)
+        p.iterate(sched, error_trace=False)
+
+    with torch.cuda.stream(stream):
+        p.set_modulus(m)
+        want = []
+        for sc in scheds:
+            block(sc)
+            ctx.sync()
+            want.append(p.get_psi().cpu().numpy())
+        graphs = [p.capture(lambda sc=sc: block(sc)) for sc in scheds]
+        for rep in range(2):
+            for gr, w in zip(graphs, want):
+                gr.launch()
+                ctx.sync()
+                assert np.array_equal(p.get_psi().cpu().numpy(), w), rep
+        graphs[0].close()
+        graphs[1].launch()
+        ctx.sync()
+        assert np.array_equal(p.get_psi().cpu().numpy(), want[1])
+        graphs[1].close()
