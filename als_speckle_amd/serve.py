@@ -157,11 +157,15 @@ class Server:
     slots: {n: slot count}; or budget_bytes plus mix {n: request fraction},
     which sizes each pool in proportion to its expected work (fraction x n^2)
     within the budget.  block: iterations per launch round (also the
-    admission granularity).  autotune: pick each pool's FFT layout, radix
-    schedule and launch mode by measurement (tune.autotune).
+    admission granularity); the default 21 is prime to the shrinkwrap
+    interval of 20, so that a request's shrinkwraps mostly fall inside a
+    launch, where they run fused into the passes (a shrinkwrap on the last
+    step of a launch runs as stand-alone kernels).  autotune: pick each
+    pool's FFT layout, radix schedule and launch mode by measurement
+    (tune.autotune).
     """
 
-    def __init__(self, device=0, slots=None, budget_bytes=None, mix=None, block=20,
+    def __init__(self, device=0, slots=None, budget_bytes=None, mix=None, block=21,
                  autotune=False):
         require_gpu()
         self.device_index = int(device)

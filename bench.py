@@ -372,7 +372,7 @@ def bench_serving(a):
         return reqs
 
     budget = int(a.budget_gb * (1 << 30))
-    srv = Server(device=0, budget_bytes=budget, mix=mix, block=20)
+    srv = Server(device=0, budget_bytes=budget, mix=mix)
     slots = {p.n: p.slots for p in srv.pools.values()}
     # capacity: a saturated burst
     burst = make_requests(min(200, a.requests), np.zeros(a.requests))
@@ -385,7 +385,7 @@ def bench_serving(a):
     arrivals = np.cumsum(rng.exponential(1.0 / lam, a.requests))
     reqs = make_requests(a.requests, arrivals)
     torch.cuda.synchronize()
-    srv2 = Server(device=0, slots=slots, block=20)
+    srv2 = Server(device=0, slots=slots)
     t0 = time.perf_counter()
     res = srv2.run(reqs)
     span = time.perf_counter() - t0

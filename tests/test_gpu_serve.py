@@ -65,26 +65,31 @@ def _requests(specs):
     return reqs
 
 
-def test_server_result_same_alone_and_in_mix():
+@pytest.mark.parametrize("block", [20, 21])
+def test_server_result_same_alone_and_in_mix(block):
+    """block 20: every shrinkwrap ends a launch (stand-alone kernels); 21:
+    most run fused into the passes, and whether they do depends on the other
+    requests in the pool, which must not change any result."""
     specs = [(64, 1, 0.0), (128, 2, 0.0), (64, 3, 0.0), (64, 4, 0.0), (128, 5, 0.0),
              (64, 6, 0.0)]
-    mix = Server(slots={64: 2, 128: 1}, block=20)
+    mix = Server(slots={64: 2, 128: 1}, block=block)
     res = {r.rid: r for r in mix.run(_requests(specs))}
     assert sorted(res) == list(range(len(specs)))
     # queueing happened: more 64^2 requests than slots
     assert max(r.admitted for r in res.values()) > 0
     for rid in (2, 4):
-        alone = Server(slots={64: 2, 128: 1}, block=20)
+        alone = Server(slots={64: 2, 128: 1}, block=block)
         one = alone.run([_requests(specs)[rid]])[0]
         assert np.array_equal(one.psi.cpu().numpy(), res[rid].psi.cpu().numpy())
         assert np.array_equal(one.support.cpu().numpy(), res[rid].support.cpu().numpy())
         assert np.array_equal(one.trace, res[rid].trace)
 
 
-def test_server_matches_dropin_object():
+@pytest.mark.parametrize("block", [20, 21])
+def test_server_matches_dropin_object(block):
     specs = [(64, 7, 0.0)]
     req = _requests(specs)[0]
-    res = Server(slots={64: 3}, block=20).run([req])[0]
+    res = Server(slots={64: 3}, block=block).run([req])[0]
     r = phasing()
     r.load(modulus=req.modulus, support=req.support, numtrials=1)
     # the drop-in seeds from numpy; load the same start through the C ABI plan
