@@ -409,7 +409,7 @@ def main():
     a = parse()
     if a.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {a.gpus} is not supported "
-              "(the path runs on a single MI355X; DESIGN.md describes how it would shard)",
+              "(the path runs on a single MI355X)",
               file=sys.stderr)
         sys.exit(2)
     from als_speckle_amd.device import Context
