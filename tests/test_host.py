@@ -139,3 +139,27 @@ def test_autotune_candidates_are_built_shapes():
         assert (4, 16, False) in cands
     assert (2, 8, False) in candidates(4096) and (4, 16, False) in candidates(4096)
     assert all(t != 8 for t, _, _ in candidates(4096))
+
+
+def test_bench_refuses_more_than_one_gpu():
+    import subprocess
+    import sys
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr and not r.stdout.strip()
+
+
+def test_bench_headline_schedule():
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("bench_mod", os.path.join(ROOT, "bench.py"))
+    bench = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(bench)
+    steps, nsw = bench.headline_steps(1000)
+    assert len(steps) == 1000 and nsw == 50
+    algs = [s[0] for s in steps]
+    assert algs.count(_capi.HIO) == 900 and algs.count(_capi.ER) == 100
+    assert all(s[2] == 0.8 and s[1] == _capi.SEM_NUMPY for s in steps)
+    sw = [(k, s[3], s[4]) for k, s in enumerate(steps) if s[3] > 0]
+    assert [k for k, _, _ in sw] == list(range(19, 1000, 20))
+    assert sw[0][1] == 3.0 and sw[-1][1] == 1.5 and all(t == 0.2 for _, _, t in sw)
