@@ -1020,18 +1020,37 @@ static __global__ void k_sumsq_final(const double* partial, int nblk, double* ou
   if (f < frames) out[f] = kahan_sum(partial + (size_t)f * nblk, nblk);
 }
 
-// trace[k*frames + f] = sqrt(sum_bands errp[k][f][band] / summ2[f])
-static __global__ void k_err_finalize(const double* errp, int nsteps, int frames, int bands,
-                               const double* summ2, double* trace, const int* active) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nsteps * frames) return;
+__device__ __forceinline__ void neumaier_add(double& s, double& c, double x) {
+  const double t = s + x;
+  c += (fabs(s) >= fabs(x)) ? (s - t) + x : (x - t) + s;
+  s = t;
+}
+
+// trace[k*frames + f] = sqrt(sum_bands errp[k][f][band] / summ2[f]).  One
+// wavefront per (step, frame): lane l sums bands l, l+64, ... (Neumaier, in
+// band order), then the 64 lane sums are added in lane order (Neumaier), so
+// the result does not depend on the launch shape.
+static __global__ void __launch_bounds__(256) k_err_finalize(const double* errp, int nsteps,
+                                                             int frames, int bands,
+                                                             const double* summ2, double* trace,
+                                                             const int* active) {
+  const int i = (int)((blockIdx.x * 256u + threadIdx.x) >> 6), lane = threadIdx.x & 63;
+  if (i >= nsteps * frames) return;  // whole wavefronts
   const int f = i % frames;
   if (active && !active[i]) {
-    trace[i] = nan("");
+    if (lane == 0) trace[i] = nan("");
     return;
   }
-  const double s = kahan_sum(errp + (size_t)i * bands, bands);
-  trace[i] = sqrt(s / summ2[f]);
+  const double* v = errp + (size_t)i * bands;
+  double s = 0.0, c = 0.0;
+  for (int b = lane; b < bands; b += 64) neumaier_add(s, c, v[b]);
+  const double mine = s + c;
+  double S = 0.0, C = 0.0;
+  for (int l = 0; l < 64; ++l) {
+    const double x = __shfl(mine, l, 64);
+    neumaier_add(S, C, x);
+  }
+  if (lane == 0) trace[i] = sqrt((S + C) / summ2[f]);
 }
 
 // ---------------------------------------------------------------------------
@@ -2086,8 +2105,8 @@ hipError_t launch_sumsq(int n, const float* mod, long long mstride, double* part
 hipError_t launch_err_finalize(const double* errp, int nsteps, int frames, int bands,
                                const double* summ2, double* trace, const int* active,
                                hipStream_t st) {
-  const int total = nsteps * frames;
-  hipLaunchKernelGGL(k_err_finalize, dim3((total + 255) / 256), dim3(256), 0, st, errp,
+  const int total = nsteps * frames;  // one wavefront each, four per workgroup
+  hipLaunchKernelGGL(k_err_finalize, dim3((total + 3) / 4), dim3(256), 0, st, errp,
                      nsteps, frames, bands, summ2, trace, active);
   return hipGetLastError();
 }
