@@ -1088,6 +1088,8 @@ __global__ void __launch_bounds__(256) k_blur_rows(const c32* psi, float* tmp,
   }
   float w[RMAX + 1];
   load_blur_weights<RMAX>(tab, p0, wsh, w);   // includes the barrier
+  // outputs staged in the band's (contiguous) order, then stored as float4s
+  __shared__ __align__(16) float stg[N * TT];
   for (int seg = threadIdx.x; seg < N * TT / K; seg += 256) {
     const int b = seg / (N / K), c0 = (seg % (N / K)) * K;
     float acc[K];
@@ -1095,9 +1097,11 @@ __global__ void __launch_bounds__(256) k_blur_rows(const c32* psi, float* tmp,
 #pragma unroll
     for (int k = 0; k < K; ++k) {
       const int col = c0 + k;
-      tmp[off + (col / TT) * (TT * TT) + b * TT + (col % TT)] = acc[k];
+      stg[(col / TT) * (TT * TT) + b * TT + (col % TT)] = acc[k];
     }
   }
+  __syncthreads();
+  sw_store_staged<N, TT>(stg, tmp + off, [](int v) { return 4 * v; });
 }
 
 template <int N, int TT, int RMAX>
@@ -1119,6 +1123,9 @@ __global__ void __launch_bounds__(256) k_blur_cols(const float* tmp, float* blur
   }
   float w[RMAX + 1];
   load_blur_weights<RMAX>(tab, p0, wsh, w);
+  // outputs staged in tile order (tile q = rows qTT..qTT+TT-1 of the band),
+  // then stored as float4s
+  __shared__ __align__(16) float stg[N * TT];
   float mx = -INFINITY;
   for (int seg = threadIdx.x; seg < N * TT / K; seg += 256) {
     const int ci = seg / (N / K), r0 = (seg % (N / K)) * K;
@@ -1127,10 +1134,15 @@ __global__ void __launch_bounds__(256) k_blur_cols(const float* tmp, float* blur
 #pragma unroll
     for (int k = 0; k < K; ++k) {
       const int row = r0 + k;
-      blur[fo + (size_t)((row / TT) * NB + band) * (TT * TT) + (row % TT) * TT + ci] = acc[k];
+      stg[(row / TT) * (TT * TT) + (row % TT) * TT + ci] = acc[k];
       mx = fmaxf(mx, acc[k]);
     }
   }
+  __syncthreads();
+  sw_store_staged<N, TT>(stg, blur + fo, [&](int v) {
+    const int q = (4 * v) / (TT * TT), within = (4 * v) % (TT * TT);
+    return (q * NB + band) * (TT * TT) + within;
+  });
   const float m = block_max(mx, red);
   if (threadIdx.x == 0) maxp[(size_t)f * NB + band] = m;
 }
