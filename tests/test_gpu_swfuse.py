@@ -140,3 +140,23 @@ def test_fused_batched_shared_coefficients(gpu_ctx):
     sep = run(gpu_ctx, data, sched, separate=True)
     same(fused, sep)
     assert fused[3] == (2, 0)
+
+
+@pytest.mark.parametrize("mode", ["ipsf", "persist"])
+def test_no_fusion_with_ipsf_or_persistent_blocks(gpu_ctx, mode):
+    """The partially coherent projection and the persistent launch mode run
+    every shrinkwrap with the stand-alone kernels."""
+    n = 64
+    m, s, g = frames(n, 1, seed=31)[0]
+    p = Plan(gpu_ctx, n, persist=(mode == "persist"))
+    p.set_modulus(dev(m, torch.float32))
+    p.set_support(dev(s, torch.float32))
+    p.set_psi(dev(g, torch.complex64), mult_support=False)
+    if mode == "ipsf":
+        x = np.fft.fftfreq(n) * n
+        psf = np.exp(-(x[:, None] ** 2 + x[None, :] ** 2) / 8.0)
+        p.set_ipsf(dev(np.fft.fft2(psf / psf.sum()), torch.complex64))
+    tr = p.iterate(schedule(21, 10)).cpu().numpy()
+    assert np.all(np.isfinite(tr))
+    assert p.sw_counts() == (0, 2)
+    p.close()
