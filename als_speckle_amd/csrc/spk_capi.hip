@@ -215,6 +215,7 @@ int make_coef(const spk_step& s, Coef* c) {
   if (s.semantics != SPK_SEM_NUMPY && s.semantics != SPK_SEM_OPENCL)
     return fail(SPK_EINVAL, "unknown semantics " + std::to_string(s.semantics));
   c->flags |= kActive;
+  if (c->ao == 0.0f && c->bo == 0.0f && !(c->flags & kUseT)) c->flags |= kNoOut;
   return SPK_OK;
 }
 
@@ -870,6 +871,24 @@ int spk_iterate(spk_plan* p, const spk_step* steps, int nsteps, int per_frame,
     }
   }
   if (per_frame) coef_off = stg.add(coefs.data(), coefs.size());
+  // Steps whose updates all ignore psi_out (ER with numpy semantics): the
+  // column pass only forms the error partials (or is skipped without an
+  // error trace) and the row pass skips the row IFFT of Phi.  Not with DM
+  // work, a fused shrinkwrap or a partially coherent projection in the step.
+  std::vector<char> noout(nsteps, 0);
+  if (!p->ipsf_on && !p->spec_on) {
+    for (int k = 0; k < nsteps; ++k) {
+      if (any_dm[k] || emits[k] || sw_fuse[k] || (k > 0 && sw_fuse[k - 1])) continue;
+      bool all = true, any = false;
+      for (int f = 0; f < W; ++f) {
+        const Coef& c = coefs[(size_t)k * W + f];
+        if (!(c.flags & kActive)) continue;
+        any = true;
+        all &= (c.flags & kNoOut) != 0;
+      }
+      noout[k] = any && all;
+    }
+  }
   // Persistent blocks: runs of at least two steps with no DM step and no DM
   // successor, each run ending at a step with a shrinkwrap (which follows the
   // block).  block_end[k0] = k1 for a block of steps [k0, k1).
@@ -987,15 +1006,18 @@ int spk_iterate(spk_plan* p, const spk_step* steps, int nsteps, int per_frame,
       }
       if ((rc = prof_arm(p, 0))) return rc;
       SPK_HIP(launch_colpass(p->n, p->tile, p->ept, p->spec_on ? 8 : 5, a, nact, 1, st));
+    } else if (noout[k]) {
+      // not profiled: the pass timings cover the full passes
+      if (a.errp) SPK_HIP(launch_colpass(p->n, p->tile, p->ept, 10, a, nact, 1, st));
     } else {
       if ((rc = prof_arm(p, 0))) return rc;
       SPK_HIP(launch_colpass(p->n, p->tile, p->ept, (swbits & 2) ? 9 : 0, a, nact,
                              any_dm[k] ? 2 : 1, st));
     }
     a.errp = nullptr;
-    if ((rc = prof_arm(p, 1))) return rc;
-    SPK_HIP(launch_rowpass(p->n, p->tile, p->ept, any_dm[k] != 0, emits[k] != 0, swbits, a, nact,
-                           st));
+    if (!noout[k] && (rc = prof_arm(p, 1))) return rc;
+    SPK_HIP(launch_rowpass(p->n, p->tile, p->ept, any_dm[k] != 0, emits[k] != 0, swbits,
+                           noout[k] != 0, a, nact, st));
     p->phib_fresh = emits[k] != 0;
     if (sw_fuse[k]) p->sw_fused_count++;
     if (any_sw[k] && !sw_fuse[k] && (rc = shrinkwrap_after(k))) return rc;
@@ -1575,7 +1597,7 @@ int spk_time_passes(spk_plan* p, int which, int reps, const spk_step* step, floa
   SPK_HIP(hipEventRecord(e0, st));
   for (int r = 0; r < reps; ++r) {
     if (which != 1) SPK_HIP(launch_colpass(p->n, p->tile, p->ept, 0, a, p->batch, 1, st));
-    if (which != 0) SPK_HIP(launch_rowpass(p->n, p->tile, p->ept, false, false, 0, a, p->batch, st));
+    if (which != 0) SPK_HIP(launch_rowpass(p->n, p->tile, p->ept, false, false, 0, false, a, p->batch, st));
   }
   SPK_HIP(hipEventRecord(e1, st));
   SPK_HIP(hipEventSynchronize(e1));

@@ -23,6 +23,8 @@ enum : int {
   kSwRow = 8,     // step k's row pass: row blur of |new psi| -> sw_tmp
   kSwFin = 16,    // step k+1: column pass blurs sw_tmp's columns -> sw_blur and
                   // band maxima; row pass applies s *= [blur > tau max] first
+  kNoOut = 32,    // the update does not use psi_out (ER with numpy semantics,
+                  // s * psi_in): the projected Phi of the step is not needed
 };
 
 struct Coef {
@@ -108,9 +110,12 @@ bool size_supported(int n, int tile, int ept);
 hipError_t launch_colpass(int n, int tile, int ept, int mode, const PassArgs& a,
                           int frames, int zdim, hipStream_t st);
 // sw (no DM only): bit 0 builds the kSwRow work, bit 1 the kSwFin work; the
-// column pass of a kSwFin step is mode 9 (mode 0 plus the column blur)
+// column pass of a kSwFin step is mode 9 (mode 0 plus the column blur).
+// noout (no DM, no sw): the row pass of a step whose update ignores psi_out
+// (kNoOut) skips the row IFFT of Phi; its column pass is mode 10 (forward
+// transform and error partials only)
 hipError_t launch_rowpass(int n, int tile, int ept, bool use_t, bool emit_b, int sw,
-                          const PassArgs& a, int frames, hipStream_t st);
+                          bool noout, const PassArgs& a, int frames, hipStream_t st);
 // persistent iteration block: one cooperative launch of `grid` workgroups runs
 // nsteps (column pass, grid barrier, row pass) steps; bar = barrier counter
 // (zeroed by the launcher), flag set to 1 if a barrier wait gave up

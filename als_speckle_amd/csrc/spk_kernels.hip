@@ -312,6 +312,8 @@ __device__ __forceinline__ c32 update_value(const Coef& c, c32 pout, c32 pin, fl
 // Column pass.  MODE 0: FFT -> projection (+ error) -> IFFT
 //               MODE 9: MODE 0, then the column blur of a fused shrinkwrap
 //               (sw_tmp -> sw_blur, band maxima) for frames with kSwFin
+//               MODE 10: forward FFT and the error partials only (a step
+//               whose update ignores psi_out)
 //               MODE 2: forward FFT only; MODE 3: inverse FFT only
 //               MODE 4, 5: the two column passes of the partially coherent
 //               projection (colpass_ipsf_a, colpass_div_b)
@@ -327,7 +329,7 @@ template <int N, int TT, int E, int MODE, bool OWN_TW>
 __device__ __forceinline__ void colpass_item(const PassArgs& a, int f, int band, int z,
                                              double* errp, c32* lds, bool swf = false) {
   constexpr int NB = N / TT;
-  constexpr bool PROJ = MODE == 0 || MODE == 9;
+  constexpr bool PROJ = MODE == 0 || MODE == 9 || MODE == 10;
   using L = LdsPlan<N, TT>;
   const size_t fo = (size_t)f * N * N;
   // dual layout: read Phi column-band-major (contiguous), write it tiled for
@@ -372,6 +374,13 @@ __device__ __forceinline__ void colpass_item(const PassArgs& a, int f, int band,
     }
 #pragma unroll
     for (int i = 0; i < E; ++i) x[i] = project_modulus(x[i], m[i], err);
+    if constexpr (MODE == 10) {  // error only
+      if (z == 0 && errp) {
+        const double s = block_sum(err, reinterpret_cast<double*>(lds));
+        if (tid == 0) errp[(size_t)f * NB + band] = s;
+      }
+      return;
+    }
     // fused shrinkwrap: this band's columns of the row-blurred |psi|
     // (tiled), loaded under the inverse transform (the modulus is dead)
     if constexpr (MODE == 9) {
@@ -588,10 +597,12 @@ __global__ void __launch_bounds__((N / E) * TT) k_colpass(PassArgs a) {
 // frame with kSwRow, ends with the row half of this step's shrinkwrap blur of
 // |new psi| into sw_tmp (as k_blur_rows).
 // ---------------------------------------------------------------------------
-template <int N, int TT, int E, bool USE_T, bool EMIT_B, bool OWN_TW, int SW = 0>
+template <int N, int TT, int E, bool USE_T, bool EMIT_B, bool OWN_TW, int SW = 0,
+          bool NOOUT = false>
 __device__ __forceinline__ void rowpass_item(const PassArgs& a, int f, int band, const Coef& c,
                                              c32* lds) {
   static_assert(SW == 0 || (!USE_T && !EMIT_B), "fused shrinkwrap is built without DM");
+  static_assert(!NOOUT || (!USE_T && !EMIT_B && SW == 0), "kNoOut steps: no DM, no shrinkwrap");
   using L = LdsPlan<N, TT>;
   constexpr int NB = N / TT;
   const size_t fo = (size_t)f * N * N;
@@ -619,8 +630,10 @@ __device__ __forceinline__ void rowpass_item(const PassArgs& a, int f, int band,
     }
   }
   c32 x[E];
+  if constexpr (!NOOUT) {
 #pragma unroll
-  for (int i = 0; i < E; ++i) x[i] = gld(phi, SPK_RPOS(i));
+    for (int i = 0; i < E; ++i) x[i] = gld(phi, SPK_RPOS(i));
+  }
   c32 tv[USE_T ? E : 1];
   if constexpr (USE_T) {
 #pragma unroll
@@ -673,7 +686,7 @@ __device__ __forceinline__ void rowpass_item(const PassArgs& a, int f, int band,
   if constexpr (OWN_TW) tws.put(twl);
   const auto tw = twiddle_view<N, TT>(twl);
 
-  fft_regs<N, E, TT, true, true>(x, t, b, lds, tw);
+  if constexpr (!NOOUT) fft_regs<N, E, TT, true, true>(x, t, b, lds, tw);
   if constexpr (USE_T) fft_regs<N, E, TT, true, false>(tv, t, b, lds, tw);
   if constexpr (!PREFETCH) {
 #pragma unroll
@@ -698,7 +711,9 @@ __device__ __forceinline__ void rowpass_item(const PassArgs& a, int f, int band,
     const float u = 1.0f - s;
     c32 tt{0.0f, 0.0f};
     if constexpr (USE_T) tt = scale(tv[i], a.inv_nn);
-    const c32 nv = update_value<USE_T>(c, scale(x[i], a.inv_nn), pin[i], s, tt);
+    // kNoOut: the coefficients of psi_out are 0, and psi_out is not formed
+    const c32 pout = NOOUT ? c32{0.0f, 0.0f} : scale(x[i], a.inv_nn);
+    const c32 nv = update_value<USE_T>(c, pout, pin[i], s, tt);
     gst(psi, SPK_RPOS(i), nv);
     x[i] = nv;
     if constexpr (SW & 1) av[i] = sqrtf(nv.x * nv.x + nv.y * nv.y);
@@ -752,7 +767,7 @@ __device__ __forceinline__ void rowpass_item(const PassArgs& a, int f, int band,
 }
 
 // grid (n/TT bands, frames)
-template <int N, int TT, int E, bool USE_T, bool EMIT_B, int SW>
+template <int N, int TT, int E, bool USE_T, bool EMIT_B, int SW, bool NOOUT = false>
 __global__ void __launch_bounds__((N / E) * TT) k_rowpass(PassArgs a) {
   __shared__ c32 lds[LdsPlan<N, TT>::TOTAL];
   static_assert(SW == 0 || TT * sw_line<N>() * sizeof(float) <= sizeof(lds),
@@ -760,7 +775,7 @@ __global__ void __launch_bounds__((N / E) * TT) k_rowpass(PassArgs a) {
   const int f = a.fmap ? a.fmap[blockIdx.y] : blockIdx.y;
   const Coef c = a.tab ? a.tab[f] : a.c;
   if (!(c.flags & kActive)) return;
-  rowpass_item<N, TT, E, USE_T, EMIT_B, true, SW>(a, f, blockIdx.x, c, lds);
+  rowpass_item<N, TT, E, USE_T, EMIT_B, true, SW, NOOUT>(a, f, blockIdx.x, c, lds);
 }
 
 // ---------------------------------------------------------------------------
@@ -1728,16 +1743,22 @@ static hipError_t colpass_nt(int mode, const PassArgs& a, int frames, int zdim,
     case 7: launch_pass(k_colpass<N, TT, E, 7>, grid, block, st, a); break;
     case 8: launch_pass(k_colpass<N, TT, E, 8>, grid, block, st, a); break;
     case 9: launch_pass(k_colpass<N, TT, E, 9>, grid, block, st, a); break;
+    case 10: launch_pass(k_colpass<N, TT, E, 10>, grid, block, st, a); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
 }
 
 template <int N, int TT, int E>
-static hipError_t rowpass_nt(bool use_t, bool emit_b, int sw, const PassArgs& a, int frames,
-                             hipStream_t st) {
+static hipError_t rowpass_nt(bool use_t, bool emit_b, int sw, bool noout, const PassArgs& a,
+                             int frames, hipStream_t st) {
   dim3 grid(N / TT, frames), block((N / E) * TT);
   if (sw && (use_t || emit_b)) return hipErrorInvalidValue;
+  if (noout) {
+    if (use_t || emit_b || sw) return hipErrorInvalidValue;
+    launch_pass(k_rowpass<N, TT, E, false, false, 0, true>, grid, block, st, a);
+    return hipGetLastError();
+  }
   if (use_t && emit_b)
     launch_pass(k_rowpass<N, TT, E, true, true, 0>, grid, block, st, a);
   else if (use_t)
@@ -1844,7 +1865,7 @@ constexpr bool shape_ok() {
 #define SPK_SIZED_DECL(n)                                                                   \
   hipError_t colpass_##n(int tile, int ept, int mode, const PassArgs& a, int frames,       \
                          int zdim, hipStream_t st);                                         \
-  hipError_t rowpass_##n(int tile, int ept, bool use_t, bool emit_b, int sw,              \
+  hipError_t rowpass_##n(int tile, int ept, bool use_t, bool emit_b, int sw, bool noout,  \
                          const PassArgs& a, int frames, hipStream_t st);                    \
   hipError_t rowfft_##n(int tile, int ept, int mode, const c32* in, c32* out,              \
                         const float* sup, long long sstride, float g, const c32* tw,       \
@@ -1885,7 +1906,7 @@ constexpr bool shape_ok() {
 #define SPK_CALL_ROW(TT_, E_)                                                   \
   (shape_ok<SPK_N, TT_, E_>() ? rowpass_nt<SPK_N, (shape_ok<SPK_N, TT_, E_>() ? TT_ : 4), \
                                            (shape_ok<SPK_N, TT_, E_>() ? E_ : 16)>(     \
-                                    use_t, emit_b, sw, a, frames, st)                   \
+                                    use_t, emit_b, sw, noout, a, frames, st)            \
                               : hipErrorInvalidValue)
 #define SPK_CALL_FFT(TT_, E_)                                                   \
   (shape_ok<SPK_N, TT_, E_>() ? rowfft_nt<SPK_N, (shape_ok<SPK_N, TT_, E_>() ? TT_ : 4),  \
@@ -1899,7 +1920,7 @@ constexpr bool shape_ok() {
     constexpr int SPK_N = n;                                                                \
     SPK_SHAPE_SWITCH(n, SPK_CALL_COL)                                                       \
   }                                                                                         \
-  hipError_t rowpass_##n(int tile, int ept, bool use_t, bool emit_b, int sw,              \
+  hipError_t rowpass_##n(int tile, int ept, bool use_t, bool emit_b, int sw, bool noout,  \
                          const PassArgs& a, int frames, hipStream_t st) {                   \
     constexpr int SPK_N = n;                                                                \
     SPK_SHAPE_SWITCH(n, SPK_CALL_ROW)                                                       \
@@ -2002,8 +2023,8 @@ hipError_t launch_colpass(int n, int tile, int ept, int mode, const PassArgs& a,
 }
 
 hipError_t launch_rowpass(int n, int tile, int ept, bool use_t, bool emit_b, int sw,
-                          const PassArgs& a, int frames, hipStream_t st) {
-  SPK_SIZE_SWITCH(rowpass, tile, ept, use_t, emit_b, sw, a, frames, st)
+                          bool noout, const PassArgs& a, int frames, hipStream_t st) {
+  SPK_SIZE_SWITCH(rowpass, tile, ept, use_t, emit_b, sw, noout, a, frames, st)
 }
 
 hipError_t launch_rowfft(int n, int tile, int ept, int mode, const c32* in, c32* out,
