@@ -403,3 +403,25 @@ def test_two_graphs_of_one_plan_replay_their_own_tables():
         ctx.sync()
         assert np.array_equal(p.get_psi().cpu().numpy(), want[1])
         graphs[1].close()
+
+
+def test_er_numpy_steps_without_trace(gpu_ctx):
+    """ER with numpy semantics (s * psi_in) does not need the projected Phi:
+    without an error trace its column pass is skipped.  The iterates must
+    not depend on whether the trace is formed, and one ER step must match
+    the oracle."""
+    n = 128
+    f = frame(n, seed=21)
+    psi = start_psi(n, f.support, seed=8)
+    sched = [step("hio", 0.9)] * 4 + [step("er", 0.9)] * 3 + [step("hio", 0.9)] * 2 + \
+            [step("er", 0.9)]
+    a = load_plan(gpu_ctx, f, psi)
+    ta = a.iterate(sched).cpu().numpy()[:, 0]
+    b = load_plan(gpu_ctx, f, psi)
+    assert b.iterate(sched, error_trace=False) is None
+    assert np.array_equal(a.get_psi().cpu().numpy(), b.get_psi().cpu().numpy())
+    assert np.all(np.isfinite(ta))
+    c = load_plan(gpu_ctx, f, psi)
+    c.iterate([step("er", 0.9)], error_trace=False)
+    want, _, _ = O.iteration(psi.astype(np.complex128), f.modulus, f.support, "er", 0.9)
+    assert rel(c.get_psi().cpu().numpy(), want) < TOL_STEP
