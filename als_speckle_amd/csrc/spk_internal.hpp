@@ -82,6 +82,7 @@ struct PassArgs {
   const SwParam* swr_tab;   // for kSwRow (this step's shrinkwrap)
   const SwParam* swf_tab;   // for kSwFin (the previous step's)
   int sw_pf;                // 1: the tables hold one entry per frame, 0: one entry
+  int nt_store;             // 1: the row pass stores psi and Phi non-temporally
 };
 
 // One step of a persistent iteration block (k_persist).  Offsets are bytes

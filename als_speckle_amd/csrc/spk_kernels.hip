@@ -102,6 +102,18 @@ template <class T>
 __device__ __forceinline__ void gst(T* base, unsigned i, const T& v) {
   *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + i * (unsigned)sizeof(T)) = v;
 }
+// Non-temporal store of a complex value (streaming; not kept in L2).  Used
+// for the row pass's contiguous outputs of a single 1024^2 frame, where
+// less dirty data in the L2s at the end of the pass shortens the kernel
+// boundary (measured: -3 % per iteration); with narrow tiles, smaller frames
+// or many frames plain stores are as fast or faster.
+__device__ __forceinline__ void gst_nt(c32* base, unsigned i, const c32& v) {
+  using V2 = float __attribute__((ext_vector_type(2)));
+  V2 w;
+  w.x = v.x;
+  w.y = v.y;
+  __builtin_nontemporal_store(w, reinterpret_cast<V2*>(reinterpret_cast<char*>(base) + i * 8u));
+}
 
 template <int N, int TT>
 __device__ __forceinline__ auto twiddle_view(const c32* twl) {
@@ -714,7 +726,8 @@ __device__ __forceinline__ void rowpass_item(const PassArgs& a, int f, int band,
     // kNoOut: the coefficients of psi_out are 0, and psi_out is not formed
     const c32 pout = NOOUT ? c32{0.0f, 0.0f} : scale(x[i], a.inv_nn);
     const c32 nv = update_value<USE_T>(c, pout, pin[i], s, tt);
-    gst(psi, SPK_RPOS(i), nv);
+    if (a.nt_store) gst_nt(psi, SPK_RPOS(i), nv);
+    else gst(psi, SPK_RPOS(i), nv);
     x[i] = nv;
     if constexpr (SW & 1) av[i] = sqrtf(nv.x * nv.x + nv.y * nv.y);
     if constexpr (EMIT_B) y[i] = scale(nv, fmaf(-c.g_next, u, s));
@@ -734,6 +747,7 @@ __device__ __forceinline__ void rowpass_item(const PassArgs& a, int f, int band,
 #pragma unroll
   for (int i = 0; i < E; ++i) {
     if constexpr (DUAL) gst(a.phic + fo, cbrow_pos<N, TT, E>(band, b, t, i), x[i]);
+    else if (a.nt_store) gst_nt(phi, SPK_RPOS(i), x[i]);
     else gst(phi, SPK_RPOS(i), x[i]);
   }
   if constexpr (SW & 1) {
