@@ -230,6 +230,24 @@ def bench_trials_512(ctx):
                     "model_hbm_frac": round(model / dt / HBM_PEAK, 4),
                     "final_error_mean": float(np.mean(tr[-1])),
                     "final_error_min": float(np.min(tr[-1]))}
+    # the FFT + projection (column) pass and the update (row) pass of RAAR at
+    # this batched shape, timed per launch; m and s are shared by the trials,
+    # so the bytes are Phi r/w (16 B/px) + m once, and Phi, psi r/w (32 B/px)
+    # + s once
+    plan = b._plan
+    b.seed(guesses=g_d)
+    plan.profile(True)
+    b.iterate(20, order=(("raar", 1),), beta=0.87)
+    ctx.sync()
+    nc, tc = plan.profile_read(0)
+    nr, tr_ = plan.profile_read(1)
+    plan.profile(False)
+    col_us, row_us = tc / nc * 1e3, tr_ / nr * 1e3
+    col_b = (16 * T + 4) * n * n
+    row_b = (32 * T + 4) * n * n
+    out["raar"].update({"column_pass_us": round(col_us, 2), "row_pass_us": round(row_us, 2),
+                        "column_pass_hbm_frac": round(col_b / (col_us * 1e-6) / HBM_PEAK, 4),
+                        "row_pass_hbm_frac": round(row_b / (row_us * 1e-6) / HBM_PEAK, 4)})
     return out
 
 
