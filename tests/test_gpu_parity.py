@@ -425,3 +425,19 @@ def test_er_numpy_steps_without_trace(gpu_ctx):
     c.iterate([step("er", 0.9)], error_trace=False)
     want, _, _ = O.iteration(psi.astype(np.complex128), f.modulus, f.support, "er", 0.9)
     assert rel(c.get_psi().cpu().numpy(), want) < TOL_STEP
+
+
+def test_single_1024_frame_equals_batched_copy(gpu_ctx):
+    """One 1024^2 frame alone (row-pass stores non-temporal) and the same
+    frame in a batch of two (plain stores) give identical iterates and
+    traces: the store form changes no value."""
+    n = 1024
+    f = synth.config_frame("single_1024", seed=0)
+    psi = start_psi(n, f.support, seed=9)
+    sched = [step("hio", 0.8)] * 6 + [step("er", 0.8)] * 2 + [step("hio", 0.8)] * 2
+    one = load_plan(gpu_ctx, f, psi)
+    t1 = one.iterate(sched).cpu().numpy()[:, 0]
+    two = load_plan(gpu_ctx, f, psi, batch=2)
+    t2 = two.iterate(sched).cpu().numpy()
+    assert np.array_equal(t1, t2[:, 0]) and np.array_equal(t2[:, 0], t2[:, 1])
+    assert np.array_equal(one.get_psi().cpu().numpy(), two.get_psi().cpu().numpy()[0])
