@@ -25,7 +25,7 @@ from . import _capi
 from .device import Context, Plan, plan_bytes, require_gpu
 from .phasing import expand_order, shrinkwrap_schedule
 
-__all__ = ["Request", "Result", "Server"]
+__all__ = ["Request", "Result", "Server", "block_table", "admission_order"]
 
 
 @dataclasses.dataclass
@@ -76,6 +76,36 @@ def _steps_for(req: Request) -> np.ndarray:
                              for k, a in enumerate(expand_order(req.order, req.iterations))])
 
 
+def block_table(active, block: int, slots: int) -> tuple[np.ndarray, dict]:
+    """The (block, slots) step table of one launch round and the steps each
+    slot takes: active maps slot -> (schedule, position); each active slot's
+    column is the next stretch of its own schedule, idle entries are NOOP."""
+    table = np.zeros((block, slots), _capi.STEP_DTYPE)
+    table["alg"] = _capi.NOOP
+    takes = {}
+    for f, (sched, pos) in active.items():
+        take = min(block, len(sched) - pos)
+        table[:take, f] = sched[pos:pos + take]
+        takes[f] = take
+    return table, takes
+
+
+def admission_order(queue, free, now: float):
+    """Split the waiting requests (in submission order) into those admitted
+    now and those still waiting: a request is admitted when it has arrived and
+    its size class has a free slot; free maps size -> free slot count and is
+    consumed.  Classes never block each other; within a class the order is
+    first come, first served."""
+    admitted, waiting = [], collections.deque()
+    for r in queue:
+        if r.arrival <= now and free.get(r.n, 0) > 0:
+            free[r.n] -= 1
+            admitted.append(r)
+        else:
+            waiting.append(r)
+    return admitted, waiting
+
+
 class _Pool:
     """Resident slots of one frame size on their own stream."""
 
@@ -112,19 +142,13 @@ class _Pool:
         """Queue the next `block` iterations of every active slot."""
         if not self.active:
             return False
-        # (block, slots) table, idle entries NOOP; each active slot's column
-        # is the next stretch of its own schedule
-        table = np.zeros((block, self.slots), _capi.STEP_DTYPE)
-        table["alg"] = _capi.NOOP
-        for f, st in self.active.items():
-            take = min(block, len(st[1]) - st[2])
-            table[:take, f] = st[1][st[2]:st[2] + take]
+        table, takes = block_table({f: (st[1], st[2]) for f, st in self.active.items()},
+                                   block, self.slots)
         with torch.cuda.stream(self.stream):
             err = self.plan.iterate(table, per_frame=True)
         for f, st in self.active.items():
-            take = min(block, len(st[1]) - st[2])
-            st[3].append((err, f, take))
-            st[2] += take
+            st[3].append((err, f, takes[f]))
+            st[2] += takes[f]
         self.event = torch.cuda.Event()
         self.event.record(self.stream)
         return True
@@ -202,15 +226,10 @@ class Server:
         self.queue.append(req)
 
     def _admit(self, now):
-        waiting = collections.deque()
-        while self.queue:
-            r = self.queue.popleft()
-            pool = self.pools[r.n]
-            if r.arrival <= now and pool.free:
-                pool.admit(r, now)
-            else:
-                waiting.append(r)
-        self.queue = waiting
+        free = {n: len(p.free) for n, p in self.pools.items()}
+        admitted, self.queue = admission_order(self.queue, free, now)
+        for r in admitted:
+            self.pools[r.n].admit(r, now)
 
     def tick(self):
         """Admit, launch one block on every busy pool, wait, retire (lockstep)."""

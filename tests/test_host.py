@@ -163,3 +163,49 @@ def test_bench_headline_schedule():
     sw = [(k, s[3], s[4]) for k, s in enumerate(steps) if s[3] > 0]
     assert [k for k, _, _ in sw] == list(range(19, 1000, 20))
     assert sw[0][1] == 3.0 and sw[-1][1] == 1.5 and all(t == 0.2 for _, _, t in sw)
+
+
+def _req(rid, n, arrival, iterations=30, shrinkwrap=None):
+    from als_speckle_amd.serve import Request
+    z = np.zeros((n, n), np.float32)
+    return Request(rid, z, z, z.astype(np.complex64), iterations=iterations,
+                   shrinkwrap=shrinkwrap, arrival=arrival)
+
+
+def test_serving_schedule_of_a_request():
+    from als_speckle_amd.serve import _steps_for
+    r = _req(0, 64, 0.0, iterations=45, shrinkwrap=True)
+    r.order = (("hio", 4), ("er", 1))
+    t = _steps_for(r)
+    assert t.dtype == _capi.STEP_DTYPE and t.size == 45
+    assert list(t["alg"][:6]) == [_capi.HIO] * 4 + [_capi.ER, _capi.HIO]
+    sw = np.nonzero(t["sw_sigma"] > 0)[0]
+    assert list(sw) == [19, 39] and t["sw_sigma"][19] == 3.0 and t["sw_sigma"][39] == 1.5
+    assert np.all(t["beta"] == np.float32(0.8))
+
+
+def test_serving_block_table_advances_each_slot_on_its_own_schedule():
+    from als_speckle_amd.serve import _steps_for, block_table
+    a = _steps_for(_req(0, 64, 0.0, iterations=30))
+    b = _steps_for(_req(1, 64, 0.0, iterations=7, shrinkwrap=dict(every=3)))
+    # slot 0 is 25 steps into its schedule, slot 2 just admitted, slot 1 idle
+    table, takes = block_table({0: (a, 25), 2: (b, 0)}, 21, 3)
+    assert table.shape == (21, 3) and takes == {0: 5, 2: 7}
+    assert np.array_equal(table[:5, 0], a[25:30]) and np.all(table["alg"][5:, 0] == _capi.NOOP)
+    assert np.all(table["alg"][:, 1] == _capi.NOOP)
+    assert np.array_equal(table[:7, 2], b) and np.all(table["alg"][7:, 2] == _capi.NOOP)
+
+
+def test_serving_admission_order():
+    from als_speckle_amd.serve import admission_order
+    q = [_req(0, 512, 0.0), _req(1, 512, 0.1), _req(2, 256, 0.2), _req(3, 512, 0.3),
+         _req(4, 256, 5.0)]
+    free = {512: 1, 256: 2}
+    admitted, waiting = admission_order(q, free, now=1.0)
+    # first come first served within a size; a full class does not block another;
+    # requests that have not arrived wait
+    assert [r.rid for r in admitted] == [0, 2]
+    assert [r.rid for r in waiting] == [1, 3, 4]
+    assert free == {512: 0, 256: 1}
+    admitted, waiting = admission_order(waiting, {512: 2, 256: 1}, now=10.0)
+    assert [r.rid for r in admitted] == [1, 3, 4] and not waiting
