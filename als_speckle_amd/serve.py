@@ -225,6 +225,26 @@ class Server:
         assert req.n in self.pools, f"no slots for frame size {req.n}"
         self.queue.append(req)
 
+    def cancel(self, rid: int) -> bool:
+        """Drop a waiting or running request (no result is produced).  A
+        running request's slot is freed at once: its frame is not in any
+        later launch and the next admission overwrites it.  Returns whether
+        the request was found."""
+        for r in list(self.queue):
+            if r.rid == rid:
+                self.queue.remove(r)
+                return True
+        for p in self.pools.values():
+            for f, st in list(p.active.items()):
+                if st[0].rid == rid:
+                    if p.event is not None:   # its last launch may still run
+                        p.event.synchronize()
+                    del p.active[f]
+                    p.free.append(f)
+                    p.free.sort()
+                    return True
+        return False
+
     def _admit(self, now):
         free = {n: len(p.free) for n, p in self.pools.items()}
         admitted, self.queue = admission_order(self.queue, free, now)

@@ -124,3 +124,26 @@ def test_autotune_picks_a_built_shape_and_caches(tmp_path):
         b.iterate(5, order=(("hio", 1),), beta=0.9)
         runs.append(b.get_error_trace())
     assert np.allclose(runs[0], runs[1], rtol=1e-5)
+
+
+def test_server_cancel_and_a_bad_request_do_not_touch_the_others():
+    """Cancelling a waiting and a running request, and a request whose data is
+    NaN, leave the other results bitwise as they are alone."""
+    specs = [(64, 11, 0.0), (64, 12, 0.0), (64, 13, 0.0), (64, 14, 0.0)]
+    reqs = _requests(specs)
+    bad = _requests([(64, 15, 0.0)])[0]
+    bad.rid = 99
+    bad.modulus = np.full_like(bad.modulus, np.nan)
+    srv = Server(slots={64: 2}, block=21)
+    for r in reqs[:3] + [bad]:
+        srv.submit(r)
+    srv.tick()                       # requests 0 and 1 run, 2 and 99 wait
+    assert srv.cancel(1) and srv.cancel(2) and not srv.cancel(7)
+    while srv.busy():
+        srv.tick()
+    res = {r.rid: r for r in srv.results}
+    assert sorted(res) == [0, 99]
+    assert np.all(np.isnan(res[99].trace))
+    alone = Server(slots={64: 2}, block=21).run([_requests(specs)[0]])[0]
+    assert np.array_equal(alone.psi.cpu().numpy(), res[0].psi.cpu().numpy())
+    assert np.array_equal(alone.trace, res[0].trace)
