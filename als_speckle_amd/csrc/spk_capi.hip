@@ -462,7 +462,7 @@ int spk_plan_create(spk_ctx* ctx, int n, int batch, int shared, const spk_tune* 
   SPK_REQUIRE(size_supported(n, tile, ept), SPK_EUNSUP,
               "frame size " + std::to_string(n) + " with tile " + std::to_string(tile) +
                   " and " + std::to_string(ept) +
-                  " values per thread not built (sizes 16..4096, tile 4 or 8, 8 or 16 "
+                  " values per thread not built (sizes 16..4096, tile 2, 4 or 8, 8 or 16 "
                   "values per thread, at most 1024 threads per workgroup, no tile 8 at 4096)");
   spk_plan* p = new spk_plan;
   p->ctx = ctx;
