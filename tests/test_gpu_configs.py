@@ -12,6 +12,13 @@ the complex128 oracle:
 - the 4096^2 large frame (polygon ~8 %, amplitude U[0.3, 1], Poisson 1e9):
   HIO, HIO with the shrinkwrap, ER from the seeded start, one iteration each
   in lockstep (the dual Phi layout runs at this size).
+- the batched-trials frame (512^2, polygon ~10 %, Poisson 1e7) in a plan of
+  64 resident trials: RAAR beta 0.87 and DM beta 0.7 in lockstep on frames
+  0, 31 and 63, then the global-phase alignment and trial mean of all 64
+  final iterates against the oracle's;
+- the 256^2 disk frame of the reference's CPU case: HIO beta 0.9 in lockstep
+  for 20 iterations, then 20 ER iterations (s * psi_out form) free-running
+  (trace and final iterate).
 
 Shrinkwrap masks must agree except at pixels whose blurred value lies within
 1e-5 of the threshold (FP32 blur vs FP64 oracle).
@@ -54,16 +61,16 @@ def plan_for(ctx, f, psi):
     return p
 
 
-def lockstep(p, f, psi, steps):
+def lockstep(p, f, psi, steps, beta=0.8):
     """steps: (alg, sigma, tau) per iteration; returns the worst trace and
     iterate errors against the oracle restarted from the GPU state."""
     cur = psi.astype(np.complex128)
     s = f.support.astype(np.float64)
     worst_t = worst_s = 0.0
     for k, (alg, sig, tau) in enumerate(steps):
-        e_gpu = p.iterate([(_capi.ALG_CODES[alg], _capi.SEM_NUMPY, 0.8, sig, tau)]).cpu().numpy()[0, 0]
+        e_gpu = p.iterate([(_capi.ALG_CODES[alg], _capi.SEM_NUMPY, beta, sig, tau)]).cpu().numpy()[0, 0]
         nxt = p.get_psi().cpu().numpy()
-        want, _, e = O.iteration(cur, f.modulus, s, alg, 0.8, "numpy")
+        want, _, e = O.iteration(cur, f.modulus, s, alg, beta, "numpy")
         worst_t = max(worst_t, abs(e_gpu - e) / e)
         worst_s = max(worst_s, rel(nxt, want))
         cur = nxt.astype(np.complex128)
@@ -112,3 +119,51 @@ def test_large_4096_lockstep(gpu_ctx):
     wt, ws = lockstep(p, f, psi, [("hio", 0.0, 0.0), ("hio", 3.0, 0.2), ("er", 0.0, 0.0)])
     print(f"  worst trace rel {wt:.2e}, worst step rel {ws:.2e}")
     assert wt < TOL and ws < TOL
+
+
+def test_trials_512_batched_raar_dm_align_mean(gpu_ctx):
+    print("[trials_512 batch 64: raar/dm lockstep, align, mean]", flush=True)
+    f = synth.config_frame("trials_512", seed=0)
+    n, b = 512, 64
+    psis = np.stack([seeded(n, f.support, 10 + t) for t in range(b)])
+    p = Plan(gpu_ctx, n, batch=b, shared=True, tile=4, ept=8)
+    p.set_modulus(dev(f.modulus, torch.float32))
+    p.set_support(dev(f.support, torch.float32))
+    p.set_psi(dev(psis, torch.complex64), mult_support=False)
+    s = f.support.astype(np.float64)
+    cur = psis
+    for alg, beta in [("raar", 0.87), ("raar", 0.87), ("dm", 0.7), ("dm", 0.7)]:
+        e = p.iterate([(_capi.ALG_CODES[alg], _capi.SEM_NUMPY, beta, 0.0, 0.0)]).cpu().numpy()[0]
+        nxt = p.get_psi().cpu().numpy()
+        for t in (0, 31, 63):
+            want, _, eo = O.iteration(cur[t].astype(np.complex128), f.modulus, s, alg, beta,
+                                      "numpy")
+            assert abs(e[t] - eo) / eo < TOL, (alg, t)
+            assert rel(nxt[t], want) < TOL, (alg, t)
+        cur = nxt
+    gal = gpu_ctx.align_global_phase(dev(cur, torch.complex64))
+    mean = gpu_ctx.stack_mean(gal).cpu().numpy()
+    want_al = O.align_global_phase_closed(cur.astype(np.complex128))
+    assert rel(gal.cpu().numpy(), want_al) < TOL
+    assert rel(mean, want_al.mean(axis=0)) < TOL
+    p.close()
+
+
+def test_cpu_disk_256_hio_then_er(gpu_ctx):
+    print("[cpu_disk_256 hio lockstep 20, er free-running 20]", flush=True)
+    f = synth.config_frame("cpu_disk_256", seed=0)
+    psi = seeded(256, f.support, 4)
+    p = plan_for(gpu_ctx, f, psi)
+    wt, ws = lockstep(p, f, psi, [("hio", 0.0, 0.0)] * 20, beta=0.9)
+    print(f"  worst trace rel {wt:.2e}, worst step rel {ws:.2e}")
+    assert wt < TOL and ws < TOL
+    start = p.get_psi().cpu().numpy()
+    # ER in the reference's GPU-kernel form (s * psi_out, phasing_er.cl); the
+    # numpy form (s * psi_in) leaves a supported iterate unchanged
+    tr = p.iterate([(_capi.ER, _capi.SEM_OPENCL, 0.9, 0.0, 0.0)] * 20).cpu().numpy()[:, 0]
+    got = p.get_psi().cpu().numpy()
+    want, otr, _ = O.iterate(start, f.modulus, f.support, 20, (("er", 1),), 0.9, "opencl")
+    assert otr[-1] < otr[0]
+    print(f"  worst trace rel {np.max(np.abs(tr - otr) / otr):.2e}")
+    assert np.max(np.abs(tr - otr) / otr) < TOL
+    assert rel(O.align_global_phase_closed(got), O.align_global_phase_closed(want)) < 1e-3
