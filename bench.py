@@ -64,11 +64,14 @@ def parse():
     ap.add_argument("--kernel-probe", action="store_true",
                     help="only a 256 MiB device copy (HBM counter calibration) and 20 "
                          "iterations of the headline frame, for rocprofv3 --pmc runs")
+    ap.add_argument("--probe-frame", default="single_1024", choices=("single_1024", "large_4096"),
+                    help="the frame of --kernel-probe")
     return ap.parse_args()
 
 
-def kernel_probe(ctx):
-    """A copy of known size, then 20 headline iterations (single frame)."""
+def kernel_probe(ctx, which="single_1024"):
+    """A copy of known size, then 20 iterations of the headline schedule on
+    one frame (the 1024^2 headline frame or the 4096^2 large frame)."""
     import torch
     from als_speckle_amd.device import Plan
     from als_speckle_amd import synth
@@ -77,18 +80,19 @@ def kernel_probe(ctx):
     dst = torch.empty_like(src)
     for _ in range(3):
         ctx.copy_bytes(dst, src)
-    frame = synth.config_frame("single_1024", 0)
-    plan = Plan(ctx, 1024, batch=1, shared=True)
+    frame = synth.config_frame(which, 0)
+    n = frame.modulus.shape[0]
+    plan = Plan(ctx, n, batch=1, shared=True)
     plan.set_modulus(torch.from_numpy(frame.modulus).to(ctx.device))
     plan.set_support(torch.from_numpy(frame.support).to(ctx.device))
     np.random.seed(1)
-    g = (np.random.rand(1024, 1024) + 1j * np.random.rand(1024, 1024)).astype(np.complex64)
+    g = (np.random.rand(n, n) + 1j * np.random.rand(n, n)).astype(np.complex64)
     plan.set_psi(torch.from_numpy(g).to(ctx.device))
     steps, _ = headline_steps(20)
     plan.iterate(steps)
     ctx.sync()
     print(json.dumps({"kernel_probe": True, "copy_bytes": nbytes, "copies": 3,
-                      "iterations": 20, "n": 1024}), flush=True)
+                      "iterations": 20, "n": n}), flush=True)
 
 
 def log(msg):
@@ -433,7 +437,7 @@ def main():
     from als_speckle_amd.device import Context
     ctx = Context(0)
     if a.kernel_probe:
-        kernel_probe(ctx)
+        kernel_probe(ctx, a.probe_frame)
         return
     log("single 1024^2 reconstructions")
     s = bench_single(a)
