@@ -114,6 +114,17 @@ __device__ __forceinline__ void gst_nt(c32* base, unsigned i, const c32& v) {
   w.y = v.y;
   __builtin_nontemporal_store(w, reinterpret_cast<V2*>(reinterpret_cast<char*>(base) + i * 8u));
 }
+// Non-temporal loads (streamed inputs read once per pass)
+__device__ __forceinline__ c32 gld_nt(const c32* base, unsigned i) {
+  using V2 = float __attribute__((ext_vector_type(2)));
+  const V2 v = __builtin_nontemporal_load(
+      reinterpret_cast<const V2*>(reinterpret_cast<const char*>(base) + i * 8u));
+  return c32{v.x, v.y};
+}
+__device__ __forceinline__ float gld_nt(const float* base, unsigned i) {
+  return __builtin_nontemporal_load(
+      reinterpret_cast<const float*>(reinterpret_cast<const char*>(base) + i * 4u));
+}
 
 template <int N, int TT>
 __device__ __forceinline__ auto twiddle_view(const c32* twl) {
@@ -357,8 +368,10 @@ __device__ __forceinline__ void colpass_item(const PassArgs& a, int f, int band,
   if constexpr (OWN_TW) tws.fetch(a.tw);
   c32 x[E];
 #pragma unroll
-  for (int i = 0; i < E; ++i)
-    x[i] = gld(src, DUAL ? mcol_pos<N, TT, E>(band, b, t, i) : col_pos<N, TT, E>(band, b, t, i));
+  for (int i = 0; i < E; ++i) {
+    if constexpr (DUAL) x[i] = gld_nt(src, mcol_pos<N, TT, E>(band, b, t, i));
+    else x[i] = gld(src, col_pos<N, TT, E>(band, b, t, i));
+  }
   // the modulus is loaded with the data where registers allow, otherwise
   // after the forward transform
   constexpr bool PREFETCH = E <= 16 && NT <= 512;
@@ -366,7 +379,10 @@ __device__ __forceinline__ void colpass_item(const PassArgs& a, int f, int band,
   float m[PROJ ? E : 1];
   if constexpr (PROJ && PREFETCH) {
 #pragma unroll
-    for (int i = 0; i < E; ++i) m[i] = gld(mod, mcol_pos<N, TT, E>(band, b, t, i));
+    for (int i = 0; i < E; ++i) {
+      if constexpr (DUAL) m[i] = gld_nt(mod, mcol_pos<N, TT, E>(band, b, t, i));
+      else m[i] = gld(mod, mcol_pos<N, TT, E>(band, b, t, i));
+    }
   }
   float rb[MODE == 9 ? E : 1];
   c32* twl = lds + L::NBUF * L::BUF;
@@ -382,7 +398,10 @@ __device__ __forceinline__ void colpass_item(const PassArgs& a, int f, int band,
   if constexpr (PROJ) {
     if constexpr (!PREFETCH) {
 #pragma unroll
-      for (int i = 0; i < E; ++i) m[i] = gld(mod, mcol_pos<N, TT, E>(band, b, t, i));
+      for (int i = 0; i < E; ++i) {
+        if constexpr (DUAL) m[i] = gld_nt(mod, mcol_pos<N, TT, E>(band, b, t, i));
+        else m[i] = gld(mod, mcol_pos<N, TT, E>(band, b, t, i));
+      }
     }
 #pragma unroll
     for (int i = 0; i < E; ++i) x[i] = project_modulus(x[i], m[i], err);
@@ -644,7 +663,10 @@ __device__ __forceinline__ void rowpass_item(const PassArgs& a, int f, int band,
   c32 x[E];
   if constexpr (!NOOUT) {
 #pragma unroll
-    for (int i = 0; i < E; ++i) x[i] = gld(phi, SPK_RPOS(i));
+    for (int i = 0; i < E; ++i) {
+      if constexpr (dual_layout<N>()) x[i] = gld_nt(phi, SPK_RPOS(i));
+      else x[i] = gld(phi, SPK_RPOS(i));
+    }
   }
   c32 tv[USE_T ? E : 1];
   if constexpr (USE_T) {
@@ -661,8 +683,13 @@ __device__ __forceinline__ void rowpass_item(const PassArgs& a, int f, int band,
   if constexpr (PREFETCH) {
 #pragma unroll
     for (int i = 0; i < E; ++i) {
-      pin[i] = gld(psi, SPK_RPOS(i));
-      sv[i] = gld(sup, SPK_RPOS(i));
+      if constexpr (dual_layout<N>()) {
+        pin[i] = gld_nt(psi, SPK_RPOS(i));
+        sv[i] = gld_nt(sup, SPK_RPOS(i));
+      } else {
+        pin[i] = gld(psi, SPK_RPOS(i));
+        sv[i] = gld(sup, SPK_RPOS(i));
+      }
     }
     if constexpr (SW & 2) {
       if (swf) {
@@ -703,8 +730,13 @@ __device__ __forceinline__ void rowpass_item(const PassArgs& a, int f, int band,
   if constexpr (!PREFETCH) {
 #pragma unroll
     for (int i = 0; i < E; ++i) {
-      pin[i] = gld(psi, SPK_RPOS(i));
-      sv[i] = gld(sup, SPK_RPOS(i));
+      if constexpr (dual_layout<N>()) {
+        pin[i] = gld_nt(psi, SPK_RPOS(i));
+        sv[i] = gld_nt(sup, SPK_RPOS(i));
+      } else {
+        pin[i] = gld(psi, SPK_RPOS(i));
+        sv[i] = gld(sup, SPK_RPOS(i));
+      }
     }
     if constexpr (SW & 2) {
       if (swf) {
