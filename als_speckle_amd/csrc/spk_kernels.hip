@@ -114,7 +114,12 @@ __device__ __forceinline__ void gst_nt(c32* base, unsigned i, const c32& v) {
   w.y = v.y;
   __builtin_nontemporal_store(w, reinterpret_cast<V2*>(reinterpret_cast<char*>(base) + i * 8u));
 }
-// Non-temporal loads (streamed inputs read once per pass)
+// Non-temporal loads of inputs read once per pass.  Used by the 4096^2
+// passes (dual layout), where streamed reads otherwise evict the column
+// pass's partly written 128-byte lines of 32-byte tiles from the L2 before
+// the neighbouring column bands complete them (measured: 1.52x -> 1.31x
+// written bytes, both passes 13 % faster).  At 2048^2 and below they are
+// slower: m and s are shared across frames or the frame fits the MALL.
 __device__ __forceinline__ c32 gld_nt(const c32* base, unsigned i) {
   using V2 = float __attribute__((ext_vector_type(2)));
   const V2 v = __builtin_nontemporal_load(
