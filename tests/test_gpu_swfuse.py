@@ -80,7 +80,8 @@ def schedule(iters, every, sigmas=(3.0, 1.5), tau=0.2, alg="hio"):
 
 
 @pytest.mark.parametrize("n,tile,ept", [(64, 0, 0), (256, 2, 8), (256, 4, 16), (256, 8, 8),
-                                        (1024, 0, 0), (1024, 4, 16), (2048, 0, 0)])
+                                        (1024, 0, 0), (1024, 4, 16), (2048, 0, 0),
+                                        (4096, 0, 0), (4096, 4, 16)])
 def test_fused_equals_separate_single_frame(gpu_ctx, n, tile, ept):
     print(f"[swfuse n={n} tile={tile} ept={ept}]", flush=True)
     data = frames(n, 1, seed=n)
