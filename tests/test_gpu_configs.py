@@ -12,6 +12,10 @@ the complex128 oracle:
 - the 4096^2 large frame (polygon ~8 %, amplitude U[0.3, 1], Poisson 1e9):
   HIO, HIO with the shrinkwrap, ER from the seeded start, one iteration each
   in lockstep (the dual Phi layout runs at this size).
+- the same 4096^2 frame: one RAAR, SF (both semantics) and ER (s * psi_out)
+  step each against the oracle, and three DM steps in one call (the row
+  pass emits the next step's modified estimate in the dual layout) bitwise
+  equal to three single-step calls, each of those against the oracle;
 - the batched-trials frame (512^2, polygon ~10 %, Poisson 1e7) in a plan of
   64 resident trials: RAAR beta 0.87 and DM beta 0.7 in lockstep on frames
   0, 31 and 63, then the global-phase alignment and trial mean of all 64
@@ -167,3 +171,36 @@ def test_cpu_disk_256_hio_then_er(gpu_ctx):
     print(f"  worst trace rel {np.max(np.abs(tr - otr) / otr):.2e}")
     assert np.max(np.abs(tr - otr) / otr) < TOL
     assert rel(O.align_global_phase_closed(got), O.align_global_phase_closed(want)) < 1e-3
+
+
+def test_large_4096_every_update_and_dm_emission(gpu_ctx):
+    print("[large_4096 raar/sf/er one step each, dm 3 steps]", flush=True)
+    f = synth.config_frame("large_4096", seed=0)
+    psi = seeded(4096, f.support, 5)
+    s = f.support.astype(np.float64)
+    for alg, sem, beta in [("raar", "numpy", 0.87), ("sf", "numpy", 0.8), ("sf", "opencl", 0.8),
+                           ("er", "opencl", 0.8)]:
+        p = plan_for(gpu_ctx, f, psi)
+        e = p.iterate([(_capi.ALG_CODES[alg], _capi.SEM_CODES[sem], beta, 0.0, 0.0)]).cpu().numpy()[0, 0]
+        got = p.get_psi().cpu().numpy()
+        p.close()
+        want, _, eo = O.iteration(psi.astype(np.complex128), f.modulus, s, alg, beta, sem)
+        print(f"  {alg}/{sem}: trace rel {abs(e - eo) / eo:.2e}, step rel {rel(got, want):.2e}")
+        assert abs(e - eo) / eo < TOL and rel(got, want) < TOL, (alg, sem)
+    dm = (_capi.DM, _capi.SEM_NUMPY, 0.7, 0.0, 0.0)
+    p = plan_for(gpu_ctx, f, psi)
+    tr_all = p.iterate([dm] * 3).cpu().numpy()[:, 0]
+    end_all = p.get_psi().cpu().numpy()
+    p.close()
+    q = plan_for(gpu_ctx, f, psi)
+    cur = psi.astype(np.complex128)
+    for k in range(3):
+        e = q.iterate([dm]).cpu().numpy()[0, 0]
+        nxt = q.get_psi().cpu().numpy()
+        want, _, eo = O.iteration(cur, f.modulus, s, "dm", 0.7, "numpy")
+        print(f"  dm step {k}: trace rel {abs(e - eo) / eo:.2e}, step rel {rel(nxt, want):.2e}")
+        assert rel(nxt, want) < TOL and abs(e - eo) / eo < TOL
+        assert e == tr_all[k]
+        cur = nxt.astype(np.complex128)
+    assert np.array_equal(nxt, end_all)
+    q.close()
