@@ -12,10 +12,11 @@ the complex128 oracle:
 - the 4096^2 large frame (polygon ~8 %, amplitude U[0.3, 1], Poisson 1e9):
   HIO, HIO with the shrinkwrap, ER from the seeded start, one iteration each
   in lockstep (the dual Phi layout runs at this size).
-- the same 4096^2 frame: one RAAR, SF (both semantics) and ER (s * psi_out)
-  step each against the oracle, and three DM steps in one call (the row
-  pass emits the next step's modified estimate in the dual layout) bitwise
-  equal to three single-step calls, each of those against the oracle;
+- the same 4096^2 frame and a 2048^2 frame: one RAAR, SF (both semantics)
+  and ER (s * psi_out) step each against the oracle, and three DM steps in
+  one call (the row pass emits the next step's modified estimate; dual
+  layout at 4096) bitwise equal to three single-step calls, each of those
+  against the oracle;
 - the batched-trials frame (512^2, polygon ~10 %, Poisson 1e7) in a plan of
   64 resident trials: RAAR beta 0.87 and DM beta 0.7 in lockstep on frames
   0, 31 and 63, then the global-phase alignment and trial mean of all 64
@@ -173,10 +174,16 @@ def test_cpu_disk_256_hio_then_er(gpu_ctx):
     assert rel(O.align_global_phase_closed(got), O.align_global_phase_closed(want)) < 1e-3
 
 
-def test_large_4096_every_update_and_dm_emission(gpu_ctx):
-    print("[large_4096 raar/sf/er one step each, dm 3 steps]", flush=True)
-    f = synth.config_frame("large_4096", seed=0)
-    psi = seeded(4096, f.support, 5)
+@pytest.mark.parametrize("n", [2048, 4096])
+def test_large_every_update_and_dm_emission(gpu_ctx, n):
+    """4096: the large frame, dual layout; 2048: a frame of the same
+    generator (polygon ~10 %, Poisson 1e8), single tiled layout (TT 2, E 16)."""
+    print(f"[n={n} raar/sf/er one step each, dm 3 steps]", flush=True)
+    if n == 4096:
+        f = synth.config_frame("large_4096", seed=0)
+    else:
+        f = synth.make_frame(n, 3, shape="polygon", fill=0.10, photons=1e8)
+    psi = seeded(n, f.support, 5)
     s = f.support.astype(np.float64)
     for alg, sem, beta in [("raar", "numpy", 0.87), ("sf", "numpy", 0.8), ("sf", "opencl", 0.8),
                            ("er", "opencl", 0.8)]:
