@@ -59,6 +59,8 @@ def parse():
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-large", action="store_true")
+    ap.add_argument("--no-next", action="store_true",
+                    help="skip the partial-coherence / spectrum / PRTF / RFTF / RL timings")
     ap.add_argument("--requests", type=int, default=2000)
     ap.add_argument("--budget-gb", type=float, default=48.0)
     ap.add_argument("--kernel-probe", action="store_true",
@@ -106,6 +108,75 @@ def headline_steps(iters):
     sw = shrinkwrap_schedule(iters, every=20, sigma0=3.0, sigma1=1.5, tau=0.2)
     return [(_capi.ALG_CODES[x], _capi.SEM_NUMPY, 0.8, *sw.get(k, (0.0, 0.0)))
             for k, x in enumerate(algs)], len(sw)
+
+
+def bench_next_rows(ctx):
+    """Throughput of the paths around the loop, on the context's stream with
+    HIP events: HIO with a partial-coherence ipsf and with a spectrum on the
+    1024^2 headline frame, PRTF and stack RFTF of 64 estimates of 512^2, and
+    Richardson-Lucy coherence iterations on the 1024^2 frame."""
+    import torch
+    from als_speckle_amd import _capi, synth
+    from als_speckle_amd.device import Plan
+    dev = ctx.device
+
+    def timed(fn, reps):
+        fn()
+        ctx.sync()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(ctx.stream)
+        for _ in range(reps):
+            fn()
+        e1.record(ctx.stream)
+        e1.synchronize()
+        return e0.elapsed_time(e1) / 1e3 / reps          # seconds per call
+
+    out = {}
+    n = 1024
+    frame = synth.config_frame("single_1024", 0)
+    x = np.fft.fftfreq(n) * n
+    psf = np.exp(-(x[:, None] ** 2 + x[None, :] ** 2) / (2 * 2.0 ** 2))
+    ipsf = torch.from_numpy(np.fft.fft2(psf / psf.sum()).astype(np.complex64)).to(dev)
+    rng = np.random.default_rng(3)
+    g = (rng.random((n, n)) + 1j * rng.random((n, n))).astype(np.complex64)
+    hio = [(_capi.HIO, _capi.SEM_NUMPY, 0.8, 0.0, 0.0)] * 100
+    for name in ("ipsf", "spectrum"):
+        p = Plan(ctx, n)
+        p.set_modulus(torch.from_numpy(frame.modulus).to(dev))
+        p.set_support(torch.from_numpy(frame.support).to(dev))
+        p.set_psi(torch.from_numpy(g).to(dev))
+        if name == "ipsf":
+            p.set_ipsf(ipsf)
+            model = 104 * n * n               # bytes per iteration (column A, row conv, column B, row pass)
+        else:
+            e = np.linspace(0.98, 1.02, 5)
+            w = np.exp(-((e - 1.0) / 0.01) ** 2)
+            p.set_spectrum((e / 1.0).astype(np.float32), (w / w.sum()).astype(np.float32))
+            model = None
+        dt = timed(lambda: p.iterate(hio, error_trace=False), 3) / len(hio)
+        out[f"hio_{name}_1024"] = {"iterations_per_s": round(1.0 / dt, 1),
+                                   "us_per_iteration": round(dt * 1e6, 2)}
+        if model:
+            out[f"hio_{name}_1024"]["model_hbm_frac"] = round(model / dt / HBM_PEAK, 4)
+        p.close()
+    fr = synth.config_frame("trials_512", 0)
+    stack = torch.from_numpy((rng.random((64, 512, 512)) *
+                              np.exp(2j * np.pi * rng.random((64, 512, 512)))).astype(np.complex64)).to(dev)
+    goal = torch.from_numpy(fr.modulus.astype(np.float32)).to(dev)
+    q = Plan(ctx, 512)
+    dt = timed(lambda: q.prtf(stack), 5)
+    out["prtf_64x512"] = {"ms_per_call": round(dt * 1e3, 3), "frames_per_s": round(64 / dt, 1)}
+    dt = timed(lambda: q.rftf_stack(stack, goal), 5)
+    out["rftf_stack_64x512"] = {"ms_per_call": round(dt * 1e3, 3), "frames_per_s": round(64 / dt, 1)}
+    q.close()
+    r = Plan(ctx, n)
+    est = torch.from_numpy(g[:256, :256].copy()).to(dev)
+    mod = torch.from_numpy(frame.modulus).to(dev)
+    dt = timed(lambda: r.coherence_rl(est, mod, 20), 3) / 20
+    out["richardson_lucy_1024"] = {"iterations_per_s": round(1.0 / dt, 1),
+                                   "us_per_iteration": round(dt * 1e6, 2)}
+    r.close()
+    return out
 
 
 def cpu_baseline(frame, iters):
@@ -501,6 +572,9 @@ def main():
     if not a.no_large:
         log("large 4096^2 reconstruction")
         out["large_4096"] = bench_large_4096(ctx)
+    if not a.no_next:
+        log("partial coherence, spectrum, PRTF, RFTF, Richardson-Lucy")
+        out["next_rows"] = bench_next_rows(ctx)
     if not a.no_serving:
         log(f"serving {a.requests} requests")
         out["serving"] = bench_serving(a)
