@@ -18,6 +18,8 @@ HIO, ER, RAAR, SF, DM, NOOP = 0, 1, 2, 3, 4, 15
 ALG_CODES = {"hio": HIO, "er": ER, "raar": RAAR, "sf": SF, "dm": DM}
 SEM_NUMPY, SEM_OPENCL = 0, 1
 SEM_CODES = {"numpy": SEM_NUMPY, "opencl": SEM_OPENCL}
+# spk_plan_bytes option bits
+PB_DM, PB_SHRINKWRAP, PB_IPSF, PB_SPECTRUM, PB_TRACE = 1, 2, 4, 8, 16
 
 ERRORS = {-1: "EINVAL", -2: "EHIP", -3: "ENOMEM", -4: "ESTATE", -5: "EUNSUP"}
 

@@ -96,6 +96,8 @@ class phasing_batch:
         steps = [(_capi.ALG_CODES[a], sem, beta, 0.0, 0.0)
                  for a in expand_order(order, iterations)]
         self._trace = self._plan.iterate(steps, error_trace=error_trace) if steps else None
+        if self._plan.persist and steps:
+            self._plan.check()   # raises if a persistent block's grid barrier gave up
 
     def get_error_trace(self):
         """(iterations, trials) FP64 modulus errors of the last ``iterate``."""

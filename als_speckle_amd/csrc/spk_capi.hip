@@ -133,9 +133,13 @@ struct spk_plan {
   bool profiling = false;
   std::vector<hipEvent_t> prof_ev[2];  // start/stop pairs per pass
   bool mod_set = false, sup_set = false, psi_set = false;
-  bool phib_fresh = false;    // phib == rowFFT(dm1(psi)) for the next DM step
+  // per frame: g != 0 when phib == rowFFT((s - g (1 - s)) psi) for the
+  // current psi and support (dm1 of a DM step with beta = 1/g), 0 when stale
+  std::vector<float> phib_g;
   // persistent iteration blocks (k_persist): on/off, co-resident grid size
-  // (0 = not queried yet), barrier counter [0] and give-up flag [1]
+  // (0 = not queried yet), barrier counter [0], give-up flag of the running
+  // launch [1] (cleared with the counter before each launch) and the sticky
+  // give-up flag [2] that spk_plan_check reports and clears
   int persist = 0;
   int pgrid = 0;
   // shrinkwraps fused into the passes where possible (spk_tune.sw_separate = 0)
@@ -162,7 +166,6 @@ struct spk_plan {
   float* coh_m = nullptr;
   double* coh_part = nullptr;
   bool coh_ready = false;
-  float phib_g = 0.0f;
   long long nn() const { return (long long)n * n; }
   // Phi kept in two layouts (see spk_fft_core.hpp dual_layout)
   bool dual() const { return n >= kDualMin; }
@@ -282,6 +285,13 @@ int ensure_sw(spk_plan* p) {
   return SPK_OK;
 }
 
+// the DM estimate phib of frames [f0, f0 + nf) no longer matches psi / the
+// support (every frame when nf < 0)
+void phib_stale(spk_plan* p, int f0, int nf) {
+  if (nf < 0) { f0 = 0; nf = p->batch; }
+  for (int f = f0; f < f0 + nf && f < (int)p->phib_g.size(); ++f) p->phib_g[f] = 0.0f;
+}
+
 // phi = rowFFT(psi) for frames [f0, f0 + nf)
 int rowfft_frames(spk_plan* p, int f0, int nf) {
   const size_t fo = (size_t)f0 * p->nn();
@@ -380,7 +390,7 @@ int do_shrinkwrap(spk_plan* p, int frame0, int frames, float sigma, float tau) {
   SPK_HIP(launch_shrinkwrap(p->n, p->tile, p->psi, p->sup, p->mstride(), p->sw_tmp,
                             p->sw_blur, p->sw_maxp, nullptr, sp, nullptr, frame0, frames,
                             sp.w.radius, p->ctx->stream));
-  p->phib_fresh = false;
+  phib_stale(p, frame0, frames);
   return SPK_OK;
 }
 
@@ -435,16 +445,25 @@ int spk_sync(spk_ctx* ctx) {
   return SPK_OK;
 }
 
-int spk_plan_bytes(int n, int batch, int shared, int with_dm, size_t* bytes) {
+int spk_plan_bytes(int n, int batch, int shared, int options, size_t* bytes) {
   SPK_REQUIRE(bytes && is_pow2(n) && batch > 0, SPK_EINVAL, "bad plan size");
+  SPK_REQUIRE((options & ~31) == 0, SPK_EINVAL, "unknown SPK_PB_* option bits");
   const size_t nn = (size_t)n * n;
   const size_t frames_ms = shared ? 1 : (size_t)batch;
+  const size_t lay = n >= kDualMin ? 2 : 1;            // Phi layouts kept
+  const size_t bands = (size_t)n / 2;                  // most bands: tile 2
   size_t b = 0;
-  const size_t lay = n >= 4096 ? 2 : 1;          // phi layouts kept
-  b += (1 + lay) * nn * batch * sizeof(c32);     // psi, phi
-  if (with_dm) b += lay * nn * batch * sizeof(c32);  // phib
-  b += 2 * nn * frames_ms * sizeof(float);       // mod, sup
-  b += n * sizeof(c32) + 2 * 1024 * batch * sizeof(double);
+  b += (1 + lay) * nn * batch * sizeof(c32);           // psi, Phi
+  b += 2 * nn * frames_ms * sizeof(float);             // modulus, support
+  b += 2 * (size_t)n * sizeof(c32);                    // twiddles
+  b += (size_t)batch * (1 + 64) * sizeof(double) + 4 * sizeof(unsigned);
+  if (options & SPK_PB_DM) b += lay * nn * batch * sizeof(c32);        // phib
+  if (options & SPK_PB_SHRINKWRAP)                                     // sw maps, maxima
+    b += 2 * nn * batch * sizeof(float) + (size_t)batch * bands * sizeof(float);
+  if (options & SPK_PB_IPSF) b += nn * sizeof(c32) + lay * nn * batch * sizeof(c32);
+  if (options & SPK_PB_SPECTRUM) b += 2 * nn * batch * sizeof(float) + 2 * 4096 * sizeof(float);
+  if (options & SPK_PB_TRACE) b += 64 * (size_t)batch * bands * sizeof(double);
+  b += 64 * 1024;                                      // step-table staging
   *bytes = b;
   return SPK_OK;
 }
@@ -474,6 +493,7 @@ int spk_plan_create(spk_ctx* ctx, int n, int batch, int shared, const spk_tune* 
   p->ept = ept;
   p->bands = n / tile;
   p->persist = (tune && tune->persist > 0) ? 1 : 0;
+  p->phib_g.assign(batch, 0.0f);
   p->sw_fuse = !(tune && tune->sw_separate > 0);
   const size_t nn = (size_t)n * n;
   const size_t fms = shared ? 1 : (size_t)batch;
@@ -481,7 +501,7 @@ int spk_plan_create(spk_ctx* ctx, int n, int batch, int shared, const spk_tune* 
       (p->dual() && (rc = dalloc(&p->phic, nn * batch))) ||
       (rc = dalloc(&p->mod, nn * fms)) || (rc = dalloc(&p->sup, nn * fms)) ||
       (rc = dalloc(&p->tw, (size_t)2 * n)) || (rc = dalloc(&p->summ2, (size_t)batch)) ||
-      (rc = dalloc(&p->sumpart, (size_t)64 * batch)) || (rc = dalloc(&p->pbar, (size_t)2))) {
+      (rc = dalloc(&p->sumpart, (size_t)64 * batch)) || (rc = dalloc(&p->pbar, (size_t)4))) {
     spk_plan_destroy(p);
     return rc;
   }
@@ -502,8 +522,12 @@ int spk_plan_create(spk_ctx* ctx, int n, int batch, int shared, const spk_tune* 
       for (int r = 1; r < R; ++r)
         tw[(size_t)n + off + k * (R - 1) + (r - 1)] = w(staged_tw_exponent(n, ept, st, k, r));
   }
-  hipError_t e = hipMemcpy(p->tw, tw.data(), tw.size() * sizeof(c32), hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = hipMemset(p->pbar, 0, 2 * sizeof(unsigned));
+  // on the context's stream, so that creating a plan while other streams run
+  // (serving grows its slot chunks) waits for this stream only
+  hipStream_t cs = ctx->stream;
+  hipError_t e = hipMemcpyAsync(p->tw, tw.data(), tw.size() * sizeof(c32), hipMemcpyHostToDevice, cs);
+  if (e == hipSuccess) e = hipMemsetAsync(p->pbar, 0, 4 * sizeof(unsigned), cs);
+  if (e == hipSuccess) e = cs ? hipStreamSynchronize(cs) : hipDeviceSynchronize();
   if (e == hipSuccess) e = hipEventCreateWithFlags(&p->upload_done, hipEventDisableTiming);
   if (e != hipSuccess) {
     spk_plan_destroy(p);
@@ -534,10 +558,10 @@ int spk_plan_check(spk_plan* p) {
   if (rc) return rc;
   if (p->ctx->stream) SPK_HIP(hipStreamSynchronize(p->ctx->stream));
   else SPK_HIP(hipDeviceSynchronize());
-  unsigned h[2] = {0, 0};
+  unsigned h[3] = {0, 0, 0};
   SPK_HIP(hipMemcpy(h, p->pbar, sizeof(h), hipMemcpyDeviceToHost));
-  if (h[1]) {
-    SPK_HIP(hipMemset(p->pbar + 1, 0, sizeof(unsigned)));
+  if (h[2]) {
+    SPK_HIP(hipMemset(p->pbar + 2, 0, sizeof(unsigned)));
     return fail(SPK_EHIP, "a grid barrier of a persistent iteration block gave up waiting; "
                           "results of that block are invalid");
   }
@@ -615,7 +639,7 @@ int spk_set_support(spk_plan* p, int frame, const float* s) {
   if ((rc = frame_range(p, frame, !p->shared, &f0, &nf))) return rc;
   SPK_HIP(launch_pack_f32(p->n, p->tile, p->sup + (size_t)f0 * p->nn(), s, nf, p->ctx->stream));
   p->sup_set = true;
-  p->phib_fresh = false;
+  phib_stale(p, f0, p->shared ? -1 : nf);   // a shared support serves every frame
   return SPK_OK;
 }
 
@@ -643,7 +667,7 @@ int spk_set_psi(spk_plan* p, int frame, const float* guess, int mult_support) {
   // keep theirs)
   if ((rc = rowfft_frames(p, f0, nf))) return rc;
   p->psi_set = true;
-  p->phib_fresh = false;
+  phib_stale(p, f0, nf);
   return SPK_OK;
 }
 
@@ -766,17 +790,34 @@ int spk_iterate(spk_plan* p, const spk_step* steps, int nsteps, int per_frame,
   }
 
   // Per-frame schedules: per step the list of active frames (launch grids
-  // cover only those), of DM frames needing a fresh phib, and of frames with
-  // a shrinkwrap and their parameters.
+  // cover only those), of frames with a shrinkwrap and their parameters.
+  // Frame activity [step][frame] is staged in both modes: the error trace of
+  // an idle frame is NaN.
   Stage stg;
-  std::vector<size_t> map_off(nsteps), dm_off(nsteps), sw_off(nsteps), swp_off(nsteps);
-  std::vector<int> map_n(nsteps, B), dm_n(nsteps, 0), sw_n(nsteps, 0), sw_r(nsteps, 0);
+  std::vector<size_t> map_off(nsteps), sw_off(nsteps), swp_off(nsteps);
+  std::vector<int> map_n(nsteps, B), sw_n(nsteps, 0), sw_r(nsteps, 0);
   std::vector<float> dm_g(nsteps, 0.0f);
   std::vector<SwParam> sw_first(nsteps);
   std::vector<std::vector<SwParam>> sw_prm(per_frame ? nsteps : 0);
   size_t coef_off = 0, act_off = 0;
+  if (err_trace) {
+    std::vector<int> act_all((size_t)nsteps * B);
+    for (int k = 0; k < nsteps; ++k)
+      for (int f = 0; f < B; ++f) act_all[(size_t)k * B + f] = act[(size_t)k * W + (per_frame ? f : 0)];
+    act_off = stg.add(act_all.data(), act_all.size());
+  }
+  // DM steps: one beta per step (the dm1 map of the refresh kernel is shared)
+  for (int k = 0; k < nsteps; ++k) {
+    if (!any_dm[k]) continue;
+    for (int f = 0; f < W; ++f) {
+      const spk_step& sp = steps[(size_t)k * W + f];
+      if (sp.alg != SPK_DM) continue;
+      if (dm_g[k] == 0.0f) dm_g[k] = 1.0f / sp.beta;
+      SPK_REQUIRE(1.0f / sp.beta == dm_g[k], SPK_EUNSUP,
+                  "per-frame DM steps need one beta per step");
+    }
+  }
   if (per_frame) {
-    act_off = stg.add(act.data(), act.size());
     std::vector<int> lst;
     std::vector<SwParam> prm;
     for (int k = 0; k < nsteps; ++k) {
@@ -786,19 +827,6 @@ int spk_iterate(spk_plan* p, const spk_step* steps, int nsteps, int per_frame,
         if (act[row + f]) lst.push_back(f);
       map_n[k] = (int)lst.size();
       map_off[k] = stg.add(lst.data(), lst.size());
-      if (any_dm[k]) {
-        lst.clear();
-        for (int f = 0; f < B; ++f) {
-          const spk_step& sp = steps[row + f];
-          if (sp.alg != SPK_DM) continue;
-          if (dm_g[k] == 0.0f) dm_g[k] = 1.0f / sp.beta;
-          SPK_REQUIRE(1.0f / sp.beta == dm_g[k], SPK_EUNSUP,
-                      "per-frame DM steps need one beta per step");
-          lst.push_back(f);
-        }
-        dm_n[k] = (int)lst.size();
-        dm_off[k] = stg.add(lst.data(), lst.size());
-      }
       if (any_sw[k]) {
         lst.clear();
         prm.clear();
@@ -819,11 +847,42 @@ int spk_iterate(spk_plan* p, const spk_step* steps, int nsteps, int per_frame,
     }
   } else {
     for (int k = 0; k < nsteps; ++k) {
-      if (any_dm[k]) dm_g[k] = 1.0f / steps[k].beta;
+      if (!act[k]) map_n[k] = 0;   // the whole step is idle
       if (any_sw[k]) {
         if ((rc = make_swparam(p->n, steps[k].sw_sigma, steps[k].sw_tau, &sw_first[k])))
           return rc;
         sw_r[k] = sw_first[k].w.radius;
+      }
+    }
+  }
+  // The DM estimate phib of a frame must equal rowFFT(dm1(psi)) with the
+  // step's beta when the frame's DM step runs.  It is fresh when the
+  // frame's previous row pass emitted it for this beta (kEmitB) and nothing
+  // changed psi or the support since (an idle step changes neither);
+  // otherwise the refresh kernel recomputes it for exactly those frames
+  // before the step (frame list dm_off[k], dm_n[k] entries; dm_all: every
+  // frame, no list).  fr tracks the state through the call.
+  std::vector<size_t> dm_off(nsteps, 0);
+  std::vector<int> dm_n(nsteps, 0);
+  std::vector<char> dm_all(nsteps, 0);
+  std::vector<float> fr(p->phib_g);
+  {
+    std::vector<int> lst;
+    for (int k = 0; k < nsteps; ++k) {
+      const size_t row = (size_t)k * W;
+      if (any_dm[k]) {
+        lst.clear();
+        for (int f = 0; f < B; ++f) {
+          const spk_step& sp = steps[row + (per_frame ? f : 0)];
+          if (sp.alg == SPK_DM && fr[f] != dm_g[k]) lst.push_back(f);
+        }
+        dm_n[k] = (int)lst.size();
+        dm_all[k] = dm_n[k] == B;
+        if (dm_n[k] && !dm_all[k]) dm_off[k] = stg.add(lst.data(), lst.size());
+      }
+      for (int f = 0; f < B; ++f) {
+        const Coef& c = coefs[row + (per_frame ? f : 0)];
+        if (c.flags & kActive) fr[f] = (c.flags & kEmitB) ? c.g_next : 0.0f;
       }
     }
   }
@@ -935,6 +994,9 @@ int spk_iterate(spk_plan* p, const spk_step* steps, int nsteps, int per_frame,
     }
   }
 
+  // until every launch is queued the DM estimates count as stale (a failed
+  // call leaves them to be recomputed)
+  std::fill(p->phib_g.begin(), p->phib_g.end(), 0.0f);
   PassArgs a = base_args(p);
   auto shrinkwrap_after = [&](int k) -> int {
     if (per_frame) {
@@ -947,7 +1009,6 @@ int spk_iterate(spk_plan* p, const spk_step* steps, int nsteps, int per_frame,
                                 p->sw_blur, p->sw_maxp, nullptr, sw_first[k], nullptr, 0, B,
                                 sw_r[k], st));
     }
-    p->phib_fresh = false;
     p->sw_separate_count++;
     return SPK_OK;
   };
@@ -961,14 +1022,13 @@ int spk_iterate(spk_plan* p, const spk_step* steps, int nsteps, int per_frame,
       SPK_HIP(launch_persist(p->n, p->tile, p->ept, pa, dbase, (const PStep*)dptr(block_off[k]),
                              k1 - k, err_trace ? p->errp + (size_t)k * B * p->bands : nullptr,
                              (long long)B * p->bands, p->pbar, (int*)(p->pbar + 1), grid, st));
-      p->phib_fresh = false;
       if (any_sw[k1 - 1] && (rc = shrinkwrap_after(k1 - 1))) return rc;
       k = k1;
       continue;
     }
     const size_t row = (size_t)k * W;
+    if (map_n[k] == 0) { ++k; continue; }     // every frame idle this step
     if (per_frame) {
-      if (map_n[k] == 0) { ++k; continue; }   // every frame idle this step
       a.tab = (const Coef*)dptr(coef_off) + row;
       a.fmap = (const int*)dptr(map_off[k]);
     } else {
@@ -977,11 +1037,12 @@ int spk_iterate(spk_plan* p, const spk_step* steps, int nsteps, int per_frame,
       a.c = coefs[row];
     }
     const int nact = map_n[k];
-    if (any_dm[k] && !p->phib_fresh) {
-      // phib = rowFFT((s - gamma_m (1-s)) psi), dm1 (phasing_dm1.cl:16)
+    if (any_dm[k] && dm_n[k]) {
+      // phib = rowFFT((s - gamma_m (1-s)) psi), dm1 (phasing_dm1.cl:16), for
+      // the DM frames of the step whose phib is not fresh
       SPK_HIP(launch_rowfft(p->n, p->tile, p->ept, 2, p->psi, p->phib_in(), p->sup, p->mstride(),
-                            dm_g[k], p->tw, per_frame ? (const int*)dptr(dm_off[k]) : nullptr,
-                            per_frame ? dm_n[k] : B, st));
+                            dm_g[k], p->tw, dm_all[k] ? nullptr : (const int*)dptr(dm_off[k]),
+                            dm_n[k], st));
     }
     a.errp = err_trace ? p->errp + (size_t)k * B * p->bands : nullptr;
     const int swbits = (sw_fuse[k] ? 1 : 0) | (k > 0 && sw_fuse[k - 1] ? 2 : 0);
@@ -1022,15 +1083,15 @@ int spk_iterate(spk_plan* p, const spk_step* steps, int nsteps, int per_frame,
     if (!noout[k] && (rc = prof_arm(p, 1))) return rc;
     SPK_HIP(launch_rowpass(p->n, p->tile, p->ept, any_dm[k] != 0, emits[k] != 0, swbits,
                            noout[k] != 0, a, nact, st));
-    p->phib_fresh = emits[k] != 0;
     if (sw_fuse[k]) p->sw_fused_count++;
     if (any_sw[k] && !sw_fuse[k] && (rc = shrinkwrap_after(k))) return rc;
     ++k;
   }
   if (err_trace) {
     SPK_HIP(launch_err_finalize(p->errp, nsteps, B, p->bands, p->summ2, err_trace,
-                                per_frame ? (const int*)dptr(act_off) : nullptr, st));
+                                (const int*)dptr(act_off), st));
   }
+  p->phib_g = fr;
   return SPK_OK;
 }
 
@@ -1604,6 +1665,7 @@ int spk_time_passes(spk_plan* p, int which, int reps, const spk_step* step, floa
     if (which != 0) SPK_HIP(launch_rowpass(p->n, p->tile, p->ept, false, false, 0, false, a, p->batch, st));
   }
   SPK_HIP(hipEventRecord(e1, st));
+  if (which != 0) phib_stale(p, 0, -1);   // the row passes changed psi
   SPK_HIP(hipEventSynchronize(e1));
   SPK_HIP(hipEventElapsedTime(ms_out, e0, e1));
   (void)hipEventDestroy(e0);

@@ -673,10 +673,15 @@ __device__ __forceinline__ void rowpass_item(const PassArgs& a, int f, int band,
       else x[i] = gld(phi, SPK_RPOS(i));
     }
   }
+  // the DM term t only for frames whose step is DM (a mixed per-frame step
+  // runs its other frames through the same kernel; their phib is not read)
+  const bool use_t = USE_T && (c.flags & kUseT);
   c32 tv[USE_T ? E : 1];
   if constexpr (USE_T) {
+    if (use_t) {
 #pragma unroll
-    for (int i = 0; i < E; ++i) tv[i] = gld(a.phib + fo, SPK_RPOS(i));
+      for (int i = 0; i < E; ++i) tv[i] = gld(a.phib + fo, SPK_RPOS(i));
+    }
   }
   // psi_in and the support are loaded with phi where registers allow (no
   // DM), otherwise after the transforms
@@ -731,7 +736,9 @@ __device__ __forceinline__ void rowpass_item(const PassArgs& a, int f, int band,
   const auto tw = twiddle_view<N, TT>(twl);
 
   if constexpr (!NOOUT) fft_regs<N, E, TT, true, true>(x, t, b, lds, tw);
-  if constexpr (USE_T) fft_regs<N, E, TT, true, false>(tv, t, b, lds, tw);
+  if constexpr (USE_T) {
+    if (use_t) fft_regs<N, E, TT, true, false>(tv, t, b, lds, tw);  // uniform per workgroup
+  }
   if constexpr (!PREFETCH) {
 #pragma unroll
     for (int i = 0; i < E; ++i) {
@@ -759,7 +766,7 @@ __device__ __forceinline__ void rowpass_item(const PassArgs& a, int f, int band,
     const float s = sv[i];
     const float u = 1.0f - s;
     c32 tt{0.0f, 0.0f};
-    if constexpr (USE_T) tt = scale(tv[i], a.inv_nn);
+    if constexpr (USE_T) tt = use_t ? scale(tv[i], a.inv_nn) : c32{0.0f, 0.0f};
     // kNoOut: the coefficients of psi_out are 0, and psi_out is not formed
     const c32 pout = NOOUT ? c32{0.0f, 0.0f} : scale(x[i], a.inv_nn);
     const c32 nv = update_value<USE_T>(c, pout, pin[i], s, tt);
@@ -880,9 +887,11 @@ __global__ void __launch_bounds__((N / E) * TT)
 // (acquire: stale lines of other XCDs' data are dropped), then the workgroup
 // barrier releases the other waves.  The wait is bounded: after 2^18
 // polls (a quarter second or more, far above any legitimate wait; the launch
-// is cooperative, so every workgroup is resident) the flag is set and the
-// workgroup goes on, later barriers of the launch pass at once, and every
-// wave ends; the host reports the flag as an error (spk_plan_check).
+// is cooperative, so every workgroup is resident) flag[0] (this launch's,
+// cleared with the counter before each launch) and flag[1] (sticky) are set
+// and the workgroup goes on, later barriers of the launch pass at once, and
+// every wave ends; the host reports the sticky flag as an error
+// (spk_plan_check).
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ void grid_sync(unsigned* bar, unsigned target, int* flag) {
   // every wave waits for its own stores to reach L2 (s_waitcnt vmcnt(0):
@@ -901,6 +910,7 @@ __device__ __forceinline__ void grid_sync(unsigned* bar, unsigned target, int* f
         __builtin_amdgcn_s_sleep(1);
         if (++polls > (1u << 18)) {
           __hip_atomic_store(flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          __hip_atomic_store(flag + 1, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
           break;
         }
       }
@@ -981,8 +991,9 @@ __global__ void __launch_bounds__((N / E) * TT)
     const int p = row_pos<N, TT, E>(band, b, t, i);
     x[i] = gld(src, p);
     if constexpr (MODE == 2) {
+      // the rounding of the row pass's own emission (rowpass_item, kEmitB)
       const float s = gld(sp, p);
-      x[i] = scale(x[i], s - g * (1.0f - s));
+      x[i] = scale(x[i], fmaf(-g, 1.0f - s, s));
     }
   }
   c32* twl = lds + L::NBUF * L::BUF;
@@ -1831,7 +1842,8 @@ template <int N, int TT, int E>
 static hipError_t persist_nt(const PassArgs& a, const unsigned char* stage, const PStep* steps,
                              int nsteps, double* errp0, long long errp_step, unsigned* bar,
                              int* flag, int grid, hipStream_t st) {
-  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, bar, 1);
+  // the barrier counter and this launch's give-up flag (bar[1] == flag[0])
+  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, bar, 2);
   void* args[] = {(void*)&a, (void*)&stage, (void*)&steps, (void*)&nsteps, (void*)&errp0,
                   (void*)&errp_step, (void*)&bar, (void*)&flag};
   return hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&k_persist<N, TT, E>),

@@ -123,9 +123,11 @@ class Context:
             pass
 
 
-def plan_bytes(n: int, batch: int, shared: bool, with_dm: bool = False) -> int:
+def plan_bytes(n: int, batch: int, shared: bool, options: int = 0) -> int:
+    """Device bytes of a plan once the buffers of ``options`` (``_capi.PB_*``
+    bits: DM, SHRINKWRAP, IPSF, SPECTRUM, TRACE) are allocated."""
     out = c_size_t()
-    check(_capi.load().spk_plan_bytes(n, batch, int(shared), int(with_dm), byref(out)))
+    check(_capi.load().spk_plan_bytes(n, batch, int(shared), int(options), byref(out)))
     return out.value
 
 

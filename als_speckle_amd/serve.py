@@ -1,16 +1,26 @@
 """Continuous-batching reconstruction server on one GPU.
 
 Requests of different frame sizes are admitted into resident slots as slots
-free up, their iterates stay on the device, and every slot of a size class
-advances in the same launches (``spk_iterate`` with per-frame schedules: a
-request admitted mid-way runs its own position in its own schedule, idle
-slots are skipped).  Each size class has its own stream so classes overlap on
-the GPU.  Admission is bounded by an HBM budget: the slot pools are sized
-from it up front and a request waits for a free slot of its size.
+and memory free up, their iterates stay on the device, and every active slot
+of a chunk advances in the same launches (``spk_iterate`` with per-frame
+schedules: a request admitted mid-way runs its own position in its own
+schedule, idle slots are skipped).  Each size class has its own stream, so
+classes overlap on the GPU.
+
+Memory: slots live in *chunks* (one plan of ``chunk_slots[n]`` frames of n x n
+each).  With a byte budget, the size classes share it: a class allocates a
+chunk when a request finds no free slot and the budget has room; when it has
+none, chunks of other classes that hold no running request are released
+(oldest idle first) to make room.  With fixed ``slots`` (the static form)
+each class gets one chunk of that many slots up front and never grows.
 
 A request's result does not depend on what else is resident: frames never
-share a reduction, and a slot's kernels are the same code with the same
+share a reduction, every chunk of a class has the same shape (slot count,
+hence the same kernels), and a slot's kernels are the same code with the same
 inputs whether its neighbours are busy or idle (tests/test_gpu_serve.py).
+
+The reference has no server: it runs trials one after another from Python
+(``examples/basic_phasing_example.py:27-30``).
 """
 from __future__ import annotations
 
@@ -25,7 +35,11 @@ from . import _capi
 from .device import Context, Plan, plan_bytes, require_gpu
 from .phasing import expand_order, shrinkwrap_schedule
 
-__all__ = ["Request", "Result", "Server", "block_table", "admission_order"]
+__all__ = ["Request", "Result", "Server", "block_table", "plan_admission", "chunk_shapes"]
+
+# plan options a serving chunk is sized for: in-loop shrinkwrap work maps and
+# the error partials of one launch round (DM adds its modified estimate)
+CHUNK_OPTIONS = _capi.PB_SHRINKWRAP | _capi.PB_TRACE
 
 
 @dataclasses.dataclass
@@ -90,52 +104,111 @@ def block_table(active, block: int, slots: int) -> tuple[np.ndarray, dict]:
     return table, takes
 
 
-def admission_order(queue, free, now: float):
-    """Split the waiting requests (in submission order) into those admitted
-    now and those still waiting: a request is admitted when it has arrived and
-    its size class has a free slot; free maps size -> free slot count and is
-    consumed.  Classes never block each other; within a class the order is
-    first come, first served."""
+def chunk_shapes(sizes, budget_bytes, target_bytes=1 << 30, max_slots=64, options=CHUNK_OPTIONS):
+    """Slots per chunk and bytes per chunk for each frame size: about
+    ``target_bytes`` of slots (at most ``max_slots``, at least one, and no
+    more than an eighth of the budget where one slot allows)."""
+    target = max(1, min(int(target_bytes), int(budget_bytes) // 8))
+    slots, byts = {}, {}
+    for n in sizes:
+        one = plan_bytes(n, 1, False, options)
+        k = int(max(1, min(max_slots, target // one)))
+        slots[n] = k
+        byts[n] = plan_bytes(n, k, False, options)
+    return slots, byts
+
+
+@dataclasses.dataclass
+class Admission:
+    """Decisions of one admission round (plan_admission)."""
+    admitted: list            # requests to place now, in order
+    waiting: collections.deque
+    allocate: dict            # size -> chunks to allocate
+    release: list             # chunk keys to release first
+
+
+def plan_admission(queue, now, free, idle, chunk_slots, chunk_bytes, budget, in_use,
+                   growable=True) -> Admission:
+    """Which waiting requests are admitted now, and the chunk changes that needs.
+
+    queue: requests in submission order.  free: size -> free slots in the
+    allocated chunks.  idle: keys (size, id) of allocated chunks that hold no
+    running request, oldest idle first.  A request is admitted when it has
+    arrived and (1) its size class has a free slot, or (2) a new chunk of its
+    size fits in the budget, or (3) releasing idle chunks of *other* sizes,
+    oldest first, makes it fit.  Classes never block each other: a request
+    that cannot be placed waits while later requests of other sizes go
+    ahead; within a size the order is first come, first served.  Pure
+    function of its arguments (free is copied).
+    """
+    free = dict(free)
+    idle = list(idle)
     admitted, waiting = [], collections.deque()
+    allocate = collections.Counter()
+    release = []
+    blocked = set()                  # sizes that already failed this round
     for r in queue:
-        if r.arrival <= now and free.get(r.n, 0) > 0:
-            free[r.n] -= 1
-            admitted.append(r)
-        else:
+        n = r.n
+        if r.arrival > now or n in blocked:
             waiting.append(r)
-    return admitted, waiting
+            continue
+        if free.get(n, 0) > 0:
+            free[n] -= 1
+            admitted.append(r)
+            continue
+        need = chunk_bytes[n]
+        if not growable or need > budget:
+            blocked.add(n)
+            waiting.append(r)
+            continue
+        if in_use + need > budget:
+            picks, freed = [], 0
+            for key in idle:
+                if key[0] == n:
+                    continue
+                picks.append(key)
+                freed += chunk_bytes[key[0]]
+                if in_use - freed + need <= budget:
+                    break
+            if in_use - freed + need > budget:
+                blocked.add(n)
+                waiting.append(r)
+                continue
+            for key in picks:
+                idle.remove(key)
+                release.append(key)
+                free[key[0]] = free.get(key[0], 0) - chunk_slots[key[0]]
+            in_use -= freed
+        allocate[n] += 1
+        in_use += need
+        free[n] = free.get(n, 0) + chunk_slots[n] - 1
+        admitted.append(r)
+    return Admission(admitted, waiting, dict(allocate), release)
 
 
-class _Pool:
-    """Resident slots of one frame size on their own stream."""
+class _Chunk:
+    """Resident slots of one frame size (one plan) on its class's stream."""
 
-    def __init__(self, device_index, n, slots, autotune=False):
-        self.n, self.slots = n, slots
-        self.stream = torch.cuda.Stream(device=device_index)
-        self.ctx = Context(device_index, stream=self.stream)
-        tile = ept = 0
-        persist = False
-        if autotune:
-            from .tune import autotune as tune_fn
-            with torch.cuda.stream(self.stream):
-                r = tune_fn(self.ctx, n, slots, shared=False)
-            tile, ept, persist = r.tile, r.ept, r.persist
-        self.plan = Plan(self.ctx, n, batch=slots, shared=False, tile=tile, ept=ept,
-                         persist=persist)
+    def __init__(self, cls, cid, slots):
+        self.cls, self.cid, self.slots = cls, cid, slots
+        self.n = cls.n
+        self.plan = Plan(cls.ctx, cls.n, batch=slots, shared=False, tile=cls.tile, ept=cls.ept)
         self.free = list(range(slots))
         self.active = {}          # slot -> [request, steps, pos, trace chunks, admitted]
         self.event = None
+        self.idle_since = 0.0
+
+    @property
+    def key(self):
+        return (self.n, self.cid)
 
     def admit(self, req: Request, now: float):
         slot = self.free.pop(0)
-        dev = self.ctx.device
-        with torch.cuda.stream(self.stream):
-            m = _as_dev(req.modulus, dev, torch.float32)
-            s = _as_dev(req.support, dev, torch.float32)
-            g = _as_dev(req.guess, dev, torch.complex64)
-            self.plan.set_modulus(m, slot)
-            self.plan.set_support(s, slot)
-            self.plan.set_psi(g, slot, mult_support=True)
+        dev = self.cls.ctx.device
+        with torch.cuda.stream(self.cls.stream):
+            self.plan.set_modulus(_as_dev(req.modulus, dev, torch.float32), slot)
+            self.plan.set_support(_as_dev(req.support, dev, torch.float32), slot)
+            self.plan.set_psi(_as_dev(req.guess, dev, torch.complex64), slot, mult_support=True)
         self.active[slot] = [req, _steps_for(req), 0, [], now]
 
     def launch(self, block: int):
@@ -144,13 +217,13 @@ class _Pool:
             return False
         table, takes = block_table({f: (st[1], st[2]) for f, st in self.active.items()},
                                    block, self.slots)
-        with torch.cuda.stream(self.stream):
+        with torch.cuda.stream(self.cls.stream):
             err = self.plan.iterate(table, per_frame=True)
         for f, st in self.active.items():
             st[3].append((err, f, takes[f]))
             st[2] += takes[f]
         self.event = torch.cuda.Event()
-        self.event.record(self.stream)
+        self.event.record(self.cls.stream)
         return True
 
     def retire(self, now: float):
@@ -158,15 +231,48 @@ class _Pool:
         done = []
         for f in [f for f, st in self.active.items() if st[2] >= len(st[1])]:
             req, _, _, chunks, admitted = self.active.pop(f)
-            with torch.cuda.stream(self.stream):
+            with torch.cuda.stream(self.cls.stream):
                 psi = self.plan.get_psi(f)
                 sup = self.plan.get_support(f)
             trace = torch.cat([e[:t, fr] for e, fr, t in chunks]).cpu().numpy() \
                 if chunks else np.zeros(0)
             done.append(Result(req.rid, req.n, psi, sup, trace, req.arrival, admitted, now))
-            self.free.append(f)
-            self.free.sort()
+            self._free_slot(f, now)
         return done
+
+    def _free_slot(self, f, now):
+        self.free.append(f)
+        self.free.sort()
+        if not self.active:
+            self.idle_since = now
+
+    def close(self):
+        if self.event is not None:
+            self.event.synchronize()
+        self.plan.close()
+
+
+class _Class:
+    """One frame size: its stream, context, kernel shape and chunks."""
+
+    def __init__(self, device_index, n, chunk_slots, autotune=False):
+        self.n, self.chunk_slots = n, chunk_slots
+        self.stream = torch.cuda.Stream(device=device_index)
+        self.ctx = Context(device_index, stream=self.stream)
+        self.tile = self.ept = 0
+        if autotune:
+            from .tune import autotune as tune_fn
+            with torch.cuda.stream(self.stream):
+                r = tune_fn(self.ctx, n, chunk_slots, shared=False)
+            self.tile, self.ept = r.tile, r.ept
+        self.chunks = []
+        self.next_id = 0
+
+    def new_chunk(self):
+        c = _Chunk(self, self.next_id, self.chunk_slots)
+        self.next_id += 1
+        self.chunks.append(c)
+        return c
 
 
 def _as_dev(x, dev, dtype):
@@ -178,51 +284,80 @@ def _as_dev(x, dev, dtype):
 class Server:
     """HBM-budgeted continuous batching over size classes.
 
-    slots: {n: slot count}; or budget_bytes plus mix {n: request fraction},
-    which sizes each pool in proportion to its expected work (fraction x n^2)
-    within the budget.  block: iterations per launch round (also the
-    admission granularity); the default 21 is prime to the shrinkwrap
-    interval of 20, so that a request's shrinkwraps mostly fall inside a
-    launch, where they run fused into the passes (a shrinkwrap on the last
-    step of a launch runs as stand-alone kernels).  autotune: pick each
-    pool's FFT layout, radix schedule and launch mode by measurement
-    (tune.autotune).
+    Two forms:
+      - budget: ``budget_bytes`` and ``sizes`` (or ``mix``, whose keys are the
+        sizes): slot chunks of each size are allocated on demand from the one
+        budget and idle chunks are released for other sizes (plan_admission);
+        ``chunk_slots`` overrides the slots per chunk (chunk_shapes);
+      - static: ``slots`` {n: count}: one chunk per size, allocated up front,
+        never grown or released.
+    block: iterations per launch round (also the admission granularity); the
+    default 21 is prime to the shrinkwrap interval of 20, so that a request's
+    shrinkwraps mostly fall inside a launch, where they run fused into the
+    passes (a shrinkwrap on the last step of a launch runs as stand-alone
+    kernels).  autotune: pick each size's FFT layout and radix schedule by
+    measurement (tune.autotune).
     """
 
-    def __init__(self, device=0, slots=None, budget_bytes=None, mix=None, block=21,
-                 autotune=False):
+    def __init__(self, device=0, slots=None, budget_bytes=None, mix=None, sizes=None,
+                 block=21, autotune=False, chunk_slots=None):
         require_gpu()
         self.device_index = int(device)
         self.block = int(block)
-        if slots is None:
-            assert budget_bytes and mix, "give slots or budget_bytes and mix"
-            slots = self.size_slots(budget_bytes, mix)
-        self.pools = {int(n): _Pool(self.device_index, int(n), int(k), autotune)
-                      for n, k in sorted(slots.items()) if k > 0}
+        if slots is not None:
+            slots = {int(n): int(k) for n, k in slots.items() if k > 0}
+            self.chunk_slots = dict(slots)
+            self.chunk_bytes = {n: plan_bytes(n, k, False, CHUNK_OPTIONS) for n, k in slots.items()}
+            self.budget = sum(self.chunk_bytes.values())
+            self.growable = False
+        else:
+            assert budget_bytes, "give slots, or budget_bytes and sizes (or mix)"
+            sz = sorted(int(n) for n in (sizes if sizes is not None else mix))
+            self.budget = int(budget_bytes)
+            cs, cb = chunk_shapes(sz, self.budget)
+            if chunk_slots:
+                cs.update({int(n): int(k) for n, k in chunk_slots.items()})
+                cb = {n: plan_bytes(n, cs[n], False, CHUNK_OPTIONS) for n in cs}
+            self.chunk_slots, self.chunk_bytes = cs, cb
+            self.growable = True
+        self.classes = {n: _Class(self.device_index, n, k, autotune)
+                        for n, k in sorted(self.chunk_slots.items())}
+        if not self.growable:
+            for c in self.classes.values():
+                c.new_chunk()
         self.queue = collections.deque()
         self.results = []
+        self.peak_bytes = self.bytes_reserved()
+        self.chunks_allocated = self.chunks_released = 0
         self.t0 = time.perf_counter()
 
+    # -- accounting ------------------------------------------------------
     @staticmethod
     def size_slots(budget_bytes, mix, max_slots=256):
-        """Slots per size within the HBM budget, proportional to expected work."""
+        """Static slots per size within the HBM budget, proportional to
+        expected work (fraction x n^2), for the static form."""
         work = {n: f * n * n for n, f in mix.items()}
         tot = sum(work.values())
         out = {}
         for n, w in work.items():
             share = budget_bytes * w / tot
-            per = plan_bytes(n, 1, False)
+            per = plan_bytes(n, 1, False, CHUNK_OPTIONS)
             out[n] = int(max(1, min(max_slots, share // per)))
         return out
 
+    def chunks(self):
+        return [ch for c in self.classes.values() for ch in c.chunks]
+
     def bytes_reserved(self):
-        return sum(plan_bytes(p.n, p.slots, False) for p in self.pools.values())
+        return sum(self.chunk_bytes[ch.n] for ch in self.chunks())
 
     def now(self):
         return time.perf_counter() - self.t0
 
+    # -- requests ----------------------------------------------------------
     def submit(self, req: Request):
-        assert req.n in self.pools, f"no slots for frame size {req.n}"
+        assert req.n in self.classes, f"no slots for frame size {req.n}"
+        assert self.chunk_bytes[req.n] <= self.budget, "one chunk exceeds the HBM budget"
         self.queue.append(req)
 
     def cancel(self, rid: int) -> bool:
@@ -234,58 +369,90 @@ class Server:
             if r.rid == rid:
                 self.queue.remove(r)
                 return True
-        for p in self.pools.values():
-            for f, st in list(p.active.items()):
+        for ch in self.chunks():
+            for f, st in list(ch.active.items()):
                 if st[0].rid == rid:
-                    if p.event is not None:   # its last launch may still run
-                        p.event.synchronize()
-                    del p.active[f]
-                    p.free.append(f)
-                    p.free.sort()
+                    if ch.event is not None:   # its last launch may still run
+                        ch.event.synchronize()
+                    del ch.active[f]
+                    ch._free_slot(f, self.now())
                     return True
         return False
 
-    def _admit(self, now):
-        free = {n: len(p.free) for n, p in self.pools.items()}
-        admitted, self.queue = admission_order(self.queue, free, now)
-        for r in admitted:
-            self.pools[r.n].admit(r, now)
+    def peek(self, rid: int):
+        """(iterations done, device copy of the current iterate) of a running
+        request, after its queued launches; None if it is not running."""
+        for ch in self.chunks():
+            for f, st in ch.active.items():
+                if st[0].rid == rid:
+                    if ch.event is not None:
+                        ch.event.synchronize()
+                    with torch.cuda.stream(ch.cls.stream):
+                        psi = ch.plan.get_psi(f)
+                    ch.cls.stream.synchronize()
+                    return st[2], psi
+        return None
 
+    def _admit(self, now):
+        chunks = self.chunks()
+        free = collections.Counter()
+        for ch in chunks:
+            free[ch.n] += len(ch.free)
+        idle = [ch.key for ch in sorted((c for c in chunks if not c.active and c.event is None),
+                                        key=lambda c: c.idle_since)]
+        adm = plan_admission(self.queue, now, free, idle, self.chunk_slots, self.chunk_bytes,
+                             self.budget, self.bytes_reserved(), self.growable)
+        self.queue = adm.waiting
+        for key in adm.release:
+            cls = self.classes[key[0]]
+            ch = next(c for c in cls.chunks if c.key == key)
+            cls.chunks.remove(ch)
+            ch.close()
+            self.chunks_released += 1
+        for n, cnt in adm.allocate.items():
+            for _ in range(cnt):
+                self.classes[n].new_chunk()
+                self.chunks_allocated += 1
+        self.peak_bytes = max(self.peak_bytes, self.bytes_reserved())
+        for r in adm.admitted:
+            ch = next(c for c in self.classes[r.n].chunks if c.free)
+            ch.admit(r, now)
+
+    # -- the loop ------------------------------------------------------------
     def tick(self):
-        """Admit, launch one block on every busy pool, wait, retire (lockstep)."""
+        """Admit, launch one block on every busy chunk, wait, retire (lockstep)."""
         now = self.now()
         self._admit(now)
-        launched = [p for p in self.pools.values() if p.launch(self.block)]
-        for p in launched:
-            p.event.synchronize()
+        launched = [ch for ch in self.chunks() if ch.launch(self.block)]
+        for ch in launched:
+            ch.event.synchronize()
         now = self.now()
         out = []
-        for p in launched:
-            out.extend(p.retire(now))
-            p.event = None
+        for ch in launched:
+            out.extend(ch.retire(now))
+            ch.event = None
         self.results.extend(out)
         return out
 
     def busy(self):
-        return bool(self.queue) or any(p.active for p in self.pools.values())
+        return bool(self.queue) or any(ch.active for ch in self.chunks())
 
     def poll(self):
-        """One pass of the asynchronous loop: each pool whose block finished is
+        """One pass of the asynchronous loop: each chunk whose block finished is
         retired and relaunched on its own, so a size class never waits for
         another.  Returns True if anything progressed."""
         progressed = False
-        now = self.now()
-        self._admit(now)
-        for p in self.pools.values():
-            if p.event is not None:
-                if not p.event.query():
+        self._admit(self.now())
+        for ch in self.chunks():
+            if ch.event is not None:
+                if not ch.event.query():
                     continue
                 now = self.now()
-                self.results.extend(p.retire(now))
-                p.event = None
+                self.results.extend(ch.retire(now))
+                ch.event = None
                 self._admit(now)
                 progressed = True
-            if p.active and p.launch(self.block):
+            if ch.active and ch.launch(self.block):
                 progressed = True
         return progressed
 
@@ -294,16 +461,16 @@ class Server:
         self.t0 = time.perf_counter() if clock_start is None else clock_start
         for r in sorted(requests, key=lambda r: r.arrival):
             self.submit(r)
-        while self.busy() or any(p.event is not None for p in self.pools.values()):
+        while self.busy() or any(ch.event is not None for ch in self.chunks()):
             if lockstep:
-                if not any(p.active for p in self.pools.values()) and self.queue:
+                if not any(ch.active for ch in self.chunks()) and self.queue:
                     wait = min(r.arrival for r in self.queue) - self.now()
                     if wait > 0:
                         time.sleep(min(wait, 0.01))
                 self.tick()
                 continue
             if not self.poll():
-                idle = all(p.event is None and not p.active for p in self.pools.values())
+                idle = all(ch.event is None and not ch.active for ch in self.chunks())
                 if idle and self.queue:
                     wait = min(r.arrival for r in self.queue) - self.now()
                     if wait > 0:
@@ -311,3 +478,9 @@ class Server:
                 else:
                     time.sleep(20e-6)
         return self.results
+
+    def close(self):
+        for ch in self.chunks():
+            ch.close()
+        for c in self.classes.values():
+            c.chunks = []

@@ -8,7 +8,9 @@ A plan's pass kernels are parameterised by
     schedule: radix-8 stages for 8 (1024 = 8*8*8*2), radix-16 (4x4) stages
     for 16 (1024 = 16*16*4), with one LDS exchange between stages,
   - the launch mode (one launch per pass, or a persistent launch per block of
-    iterations).
+    iterations; the persistent candidates are timed only when asked for:
+    on MI355X they were 1.7-4x slower than the per-pass launches at every
+    measured size and trial count, so the default search leaves them out).
 The best choice depends on the frame size, the number of resident frames and
 the device, so it is measured: every built candidate runs the same
 HIO iterations on a synthetic frame and the fastest per iteration wins.
@@ -41,7 +43,7 @@ class TuneResult(dict):
     __getattr__ = dict.__getitem__
 
 
-def candidates(n: int, persist: bool = True):
+def candidates(n: int, persist: bool = False):
     """Built (tile, ept, persist) shapes for frame size n (see size_supported)."""
     out = []
     for tile in TILES:
@@ -85,13 +87,14 @@ def _time_candidate(ctx, n, batch, shared, tile, ept, persist, iters, m, s, g):
 
 
 def autotune(ctx: Context, n: int, batch: int = 1, shared: bool = True, iters: int = 20,
-             persist: bool = True, cache_path: str | None = None) -> TuneResult:
-    """Fastest (tile, ept, persist) for `batch` resident n x n frames.
+             persist: bool = False, cache_path: str | None = None) -> TuneResult:
+    """Fastest (tile, ept, persist) for `batch` resident n x n frames; the
+    persistent launch mode is a candidate only with persist=True.
 
     Times `iters` HIO iterations per candidate (best of 3) on a synthetic
     frame; allocates one plan at a time.  The result is cached.
     """
-    key = _key(ctx, n, batch, shared)
+    key = _key(ctx, n, batch, shared) + ("|persist" if persist else "")
     if key in _cache:
         return _cache[key]
     if cache_path and os.path.exists(cache_path):

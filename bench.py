@@ -97,6 +97,43 @@ def kernel_probe(ctx, which="single_1024"):
                       "iterations": 20, "n": n}), flush=True)
 
 
+def traffic_file(dom):
+    """The committed PMC summary of the dominant pass ("column" or "row") of
+    the 1024^2 headline frame: HBM bytes per dispatch from FETCH_SIZE and
+    WRITE_SIZE, scaled by the copy-kernel calibration."""
+    kname = "colpass" if dom == "column" else "rowpass"
+    return os.path.join(ROOT, "profiles", f"traffic_{kname}_1024.json")
+
+
+def measured_traffic(dom):
+    try:
+        with open(traffic_file(dom)) as fh:
+            return json.load(fh).get("bytes_per_launch")
+    except (OSError, ValueError):
+        return None
+
+
+def oracle_final_errors(workload, run, key="final_error"):
+    """The complex128 oracle's final modulus errors over its 8 random starts
+    (tests/golden/final_error_<workload>.json, written offline by
+    tests/golden/make_fixtures.py; read here as data)."""
+    with open(os.path.join(ROOT, "tests", "golden", f"final_error_{workload}.json")) as fh:
+        d = json.load(fh)["runs"][run]
+    return np.array([r[key] for r in d["runs"]]), d["seeds"]
+
+
+def compare_final(gpu, ref, label):
+    """Distribution of GPU final errors against the oracle's (chaotic
+    iterations: single trajectories part, error levels must agree)."""
+    gpu = np.asarray(gpu, np.float64)
+    med, omed = float(np.median(gpu)), float(np.median(ref))
+    return {"what": label, "gpu_runs": int(gpu.size), "oracle_runs": int(ref.size),
+            "gpu_min": float(gpu.min()), "gpu_median": med, "gpu_max": float(gpu.max()),
+            "oracle_min": float(ref.min()), "oracle_median": omed, "oracle_max": float(ref.max()),
+            "median_rel_diff": round(abs(med - omed) / omed, 5),
+            "gpu_median_within_oracle_range": bool(ref.min() <= med <= ref.max())}
+
+
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -203,7 +240,7 @@ def bench_single(a):
     import torch
     from als_speckle_amd.device import Context, Plan
     from als_speckle_amd.masking import bounding_box
-    from als_speckle_amd import synth
+    from als_speckle_amd import _capi, synth
 
     n, iters = 1024, 1000
     frame = synth.config_frame("single_1024", 0)
@@ -257,6 +294,23 @@ def bench_single(a):
         ncol, tcol = plan.profile_read(0)
         nrow, trow = plan.profile_read(1)
         plan.profile(False)
+
+        # equal final error: the same schedule from the oracle fixture's
+        # random starts; the error of each final iterate is the error trace
+        # entry of one more (ER, s * psi_in) step
+        ref, seeds = oracle_final_errors("single_1024", "schedule")
+        fin = []
+        probe = [(_capi.ER, _capi.SEM_NUMPY, 0.8, 0.0, 0.0)]
+        for sd in seeds:
+            np.random.seed(sd)
+            g = torch.from_numpy((np.random.rand(n, n) + 1j * np.random.rand(n, n))
+                                 .astype(np.complex64)).to(dev)
+            plan.set_support(s_d)
+            plan.set_psi(g, mult_support=True)
+            plan.iterate(steps, error_trace=False)
+            fin.append(float(plan.iterate(probe)[0, 0].item()))
+        vs_oracle = compare_final(fin, ref, "final modulus error over the oracle fixture's "
+                                            "8 random starts, 1000-iteration headline schedule")
     avg = {"column": tcol / ncol * 1e-3, "row": trow / nrow * 1e-3}
     byts = {"column": B_COL * n * n, "row": B_ROW * n * n}
     dom = max(avg, key=lambda k: avg[k])
@@ -265,7 +319,8 @@ def bench_single(a):
     dt = min(dt_eager, dt_graph)
     return dict(frame=frame, n=n, iters=iters, dt=dt, dt_eager=dt_eager, dt_graph=dt_graph,
                 launch="hipGraph" if dt_graph < dt_eager else "eager",
-                final_err=final_err, avg=avg, byts=byts, dom=dom, model=model)
+                final_err=final_err, avg=avg, byts=byts, dom=dom, model=model,
+                vs_oracle=vs_oracle)
 
 
 def bench_trials_512(ctx):
@@ -286,25 +341,46 @@ def bench_trials_512(ctx):
     g_d = torch.from_numpy(g).to(ctx.device)
     out = {"schedule": {"tile": b.tuning.tile, "values_per_thread": b.tuning.ept,
                         "persistent": b.tuning.persist, "autotune_table": b.tuning.table}}
-    for alg, beta, bpx in (("raar", 0.87, B_COL + B_ROW), ("dm", 0.7, 92)):
+    # DM ends with 20 ER iterations in the s * psi_out form: the DM iterate
+    # grows at a solution (the reference's DM update doubles it inside the
+    # support), so a projecting tail turns its trials into reconstructions
+    # whose error can be compared (ER in the numpy form, s * psi_in, would
+    # leave the iterate as it is)
+    tail = 20
+    for alg, beta, bpx, run in (("raar", 0.87, B_COL + B_ROW, "raar"),
+                                ("dm", 0.7, 92, "dm_er_tail")):
         b.seed(guesses=g_d)
         b.iterate(10, order=((alg, 1),), beta=beta)           # warm-up
         ctx.sync()
         t0 = time.perf_counter()
         b.seed(guesses=g_d)
         b.iterate(iters, order=((alg, 1),), beta=beta)
+        if alg == "dm":
+            b.iterate(tail, order=(("er", 1),), beta=beta, semantics="opencl")
         aligned, avg = b.aligned_average()
         ctx.sync()
         dt = time.perf_counter() - t0
         tr = b.get_error_trace()
-        # m and s are shared by the trials: charged once per batched iteration
+        # m and s are shared by the trials: charged once per batched
+        # iteration; the ER tail (s * psi_out) moves 48 B/px
         shared_px = 8 if alg == "raar" else 12
         model = iters * n * n * ((bpx - shared_px) * T + shared_px)
-        out[alg] = {"recons_per_s": round(T / dt, 2), "ms_per_batch": round(dt * 1e3, 2),
-                    "us_per_batched_iteration": round(dt / iters * 1e6, 2),
+        its = iters
+        if alg == "dm":
+            model += tail * n * n * ((48 - 8) * T + 8)
+            its += tail
+        ref, _ = oracle_final_errors("trials_512", run, key="trace_last")
+        out[alg] = {"iterations": its, "recons_per_s": round(T / dt, 2),
+                    "ms_per_batch": round(dt * 1e3, 2),
+                    "us_per_batched_iteration": round(dt / its * 1e6, 2),
                     "model_hbm_frac": round(model / dt / HBM_PEAK, 4),
                     "final_error_mean": float(np.mean(tr[-1])),
-                    "final_error_min": float(np.min(tr[-1]))}
+                    "final_error_min": float(np.min(tr[-1])),
+                    "final_error_vs_oracle": compare_final(
+                        tr[-1], ref, f"modulus error entering the last of {its} iterations, "
+                                     f"{T} GPU trials vs the oracle fixture's 8")}
+        if alg == "dm":
+            out[alg]["schedule"] = f"{iters} DM beta {beta} + {tail} ER (s * psi_out)"
     # the FFT + projection (column) pass and the update (row) pass of RAAR at
     # this batched shape, timed per launch; m and s are shared by the trials,
     # so the bytes are Phi r/w (16 B/px) + m once, and Phi, psi r/w (32 B/px)
@@ -465,37 +541,59 @@ def bench_serving(a):
         return reqs
 
     budget = int(a.budget_gb * (1 << 30))
+
+    def latencies(res, span):
+        lat = np.array([r.latency for r in res]) * 1e3
+        by = {}
+        for n in sizes:
+            ln = np.array([r.latency for r in res if r.n == n]) * 1e3
+            if ln.size:
+                by[str(n)] = {"count": int(ln.size),
+                              "p50_ms": round(float(np.percentile(ln, 50)), 2),
+                              "p99_ms": round(float(np.percentile(ln, 99)), 2)}
+        fin = np.array([r.trace[-1] for r in res])
+        return {"requests": len(res), "recons_per_s": round(len(res) / span, 2),
+                "p50_ms": round(float(np.percentile(lat, 50)), 2),
+                "p99_ms": round(float(np.percentile(lat, 99)), 2), "by_size": by,
+                "final_error_median": float(np.median(fin))}
+
+    # capacity of the budget-shared server: a saturated burst
     srv = Server(device=0, budget_bytes=budget, mix=mix)
-    slots = {p.n: p.slots for p in srv.pools.values()}
-    # capacity: a saturated burst
     burst = make_requests(min(200, a.requests), np.zeros(a.requests))
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     srv.run(burst)
     cap = len(burst) / (time.perf_counter() - t0)
-    log(f"serving capacity {cap:.1f} req/s with slots {slots}")
+    srv.close()
+    log(f"serving capacity {cap:.1f} req/s (budget {a.budget_gb} GB shared by the sizes)")
     lam = 0.8 * cap
     arrivals = np.cumsum(rng.exponential(1.0 / lam, a.requests))
     reqs = make_requests(a.requests, arrivals)
-    torch.cuda.synchronize()
-    srv2 = Server(device=0, slots=slots)
-    t0 = time.perf_counter()
-    res = srv2.run(reqs)
-    span = time.perf_counter() - t0
-    lat = np.array([r.latency for r in res]) * 1e3
-    by = {}
-    for n in sizes:
-        ln = np.array([r.latency for r in res if r.n == n]) * 1e3
-        if ln.size:
-            by[str(n)] = {"count": int(ln.size), "p50_ms": round(float(np.percentile(ln, 50)), 2),
-                          "p99_ms": round(float(np.percentile(ln, 99)), 2)}
-    fin = np.array([r.trace[-1] for r in res])
-    return {"requests": len(res), "arrival_rate_per_s": round(lam, 2),
-            "capacity_req_per_s": round(cap, 2), "recons_per_s": round(len(res) / span, 2),
-            "p50_ms": round(float(np.percentile(lat, 50)), 2),
-            "p99_ms": round(float(np.percentile(lat, 99)), 2), "by_size": by,
-            "slots": {str(k): v for k, v in slots.items()}, "hbm_budget_gb": a.budget_gb,
-            "final_error_median": float(np.median(fin))}
+    out = {"arrival_rate_per_s": round(lam, 2), "capacity_req_per_s": round(cap, 2),
+           "hbm_budget_gb": a.budget_gb}
+    # the same arrivals through the budget-shared form and the static form
+    # (one fixed pool per size, sized from the budget by expected work)
+    for form in ("shared_budget", "static_pools"):
+        if form == "shared_budget":
+            srv = Server(device=0, budget_bytes=budget, mix=mix)
+        else:
+            srv = Server(device=0, slots=Server.size_slots(budget, mix))
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        res = srv.run(reqs)
+        span = time.perf_counter() - t0
+        d = latencies(res, span)
+        d["chunk_slots"] = {str(k): v for k, v in srv.chunk_slots.items()}
+        d["peak_reserved_gb"] = round(srv.peak_bytes / (1 << 30), 2)
+        d["chunks_allocated"], d["chunks_released"] = srv.chunks_allocated, srv.chunks_released
+        srv.close()
+        out[form] = d
+        log(f"serving {form}: p50 {d['p50_ms']} ms, p99 {d['p99_ms']} ms, "
+            f"{d['recons_per_s']} recons/s, peak {d['peak_reserved_gb']} GB")
+    # headline fields: the budget-shared form
+    for k in ("requests", "recons_per_s", "p50_ms", "p99_ms", "by_size", "final_error_median"):
+        out[k] = out["shared_budget"][k]
+    return out
 
 
 def main():
@@ -519,13 +617,8 @@ def main():
     d_avg, d_bytes = s["avg"][dom], s["byts"][dom]
     achieved = d_bytes / d_avg
 
-    traffic = None
-    tfile = os.path.join(ROOT, "profiles", f"traffic_{dom}pass_1024.json")
-    if os.path.exists(tfile):
-        try:
-            traffic = json.load(open(tfile)).get("bytes_per_launch")
-        except Exception:
-            traffic = None
+    kname = "colpass" if dom == "column" else "rowpass"
+    traffic = measured_traffic(dom)
 
     out = {
         "metric": "HIO iters/s at 1024^2 c64",
@@ -549,13 +642,14 @@ def main():
             "eager_iterations_per_s": round(iters * a.steps / s["dt_eager"], 1),
             "graph_iterations_per_s": round(iters * a.steps / s["dt_graph"], 1),
             "final_modulus_error": s["final_err"],
+            "final_error_vs_oracle": s["vs_oracle"],
             "model_bytes_per_recon": s["model"],
             "model_hbm_frac_whole_recon": round(s["model"] * a.steps / dt / HBM_PEAK, 4),
             "column_pass_us": round(s["avg"]["column"] * 1e6, 3),
             "row_pass_us": round(s["avg"]["row"] * 1e6, 3),
         },
         "roofline": {
-            "bound": "hbm", "kernel": f"k_{'colpass' if dom == 'column' else 'rowpass'}<1024>",
+            "bound": "hbm", "kernel": f"k_{kname}<1024>",
             "achieved": round(achieved / 1e9, 1), "peak": HBM_PEAK / 1e9, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK, 4), "traffic": traffic,
             "algorithmic_bytes_per_launch": d_bytes,

@@ -115,8 +115,18 @@ int spk_plan_check(spk_plan* plan);
 int spk_plan_create(spk_ctx* ctx, int n, int batch, int shared,
                     const spk_tune* tune, spk_plan** out);
 int spk_plan_destroy(spk_plan* plan);
-/* bytes of device memory the plan holds (for HBM-budget admission) */
-int spk_plan_bytes(int n, int batch, int shared, int with_dm, size_t* bytes);
+/* Bytes of device memory a plan holds once the options it will use have
+ * allocated their buffers (for HBM-budget admission).  options: bitwise OR of
+ * SPK_PB_* (1 keeps the meaning of the former with_dm argument). */
+enum {
+  SPK_PB_DM = 1,         /* DM steps: the modified estimate phib            */
+  SPK_PB_SHRINKWRAP = 2, /* in-loop shrinkwrap: blur work maps, band maxima */
+  SPK_PB_IPSF = 4,       /* spk_set_ipsf: the ipsf and per-frame work frames */
+  SPK_PB_SPECTRUM = 8,   /* spk_set_spectrum: per-frame intensity maps       */
+  SPK_PB_TRACE = 16      /* error trace of spk_iterate calls of up to 64
+                            steps: the per-band FP64 partials              */
+};
+int spk_plan_bytes(int n, int batch, int shared, int options, size_t* bytes);
 
 /* -------------------------------------------------------------------------
  * Data in and out of a plan.  All array arguments are device pointers to

@@ -147,3 +147,80 @@ def test_server_cancel_and_a_bad_request_do_not_touch_the_others():
     alone = Server(slots={64: 2}, block=21).run([_requests(specs)[0]])[0]
     assert np.array_equal(alone.psi.cpu().numpy(), res[0].psi.cpu().numpy())
     assert np.array_equal(alone.trace, res[0].trace)
+
+
+# --- the serving workload's sizes: 256^2, 512^2, 1024^2 and 2048^2 ---------
+
+MIX_SIZES = (256, 512, 1024, 2048)
+MIX_CHUNKS = {256: 2, 512: 2, 1024: 1, 2048: 1}
+
+
+def _mix_requests():
+    """Ten requests of the serving mix's sizes (polygons 10-15 % fill,
+    Poisson 1e7), 25 iterations of 90 HIO / 10 ER with a shrinkwrap after
+    iteration 20, all arriving at once."""
+    specs = [(256, 1), (256, 2), (256, 3), (512, 4), (512, 5), (512, 6), (1024, 7), (1024, 8),
+             (2048, 9), (2048, 10)]
+    reqs = []
+    for rid, (n, seed) in enumerate(specs):
+        rng = np.random.default_rng(200 + rid)
+        f = synth.make_frame(n, 300 + seed, shape="polygon", fill=float(rng.uniform(0.10, 0.15)),
+                             photons=1e7)
+        g = (rng.random((n, n)) + 1j * rng.random((n, n))).astype(np.complex64)
+        reqs.append(Request(rid, f.modulus, f.support, g, iterations=25,
+                            shrinkwrap=dict(every=20), arrival=0.0))
+    return reqs
+
+
+def _chunk_bytes(n, k):
+    from als_speckle_amd.device import plan_bytes
+    from als_speckle_amd.serve import CHUNK_OPTIONS
+    return plan_bytes(n, k, False, CHUNK_OPTIONS)
+
+
+def test_serving_mix_sizes_budget_sharing_and_alone_identity():
+    """A budget of one chunk per size: the 2048^2 requests wait until the
+    chunks of the smaller sizes go idle and are released to them; every
+    result is bitwise the same as the request served alone."""
+    print("[serving mix 256..2048, shared budget]", flush=True)
+    budget = sum(_chunk_bytes(n, k) for n, k in MIX_CHUNKS.items())
+    srv = Server(budget_bytes=budget, sizes=MIX_SIZES, chunk_slots=MIX_CHUNKS)
+    res = {r.rid: r for r in srv.run(_mix_requests(), lockstep=True)}
+    assert sorted(res) == list(range(10))
+    assert srv.peak_bytes <= budget
+    assert srv.chunks_released >= 1, "the budget never had to move between sizes"
+    assert max(r.admitted for r in res.values() if r.n == 2048) > 0     # they queued
+    srv.close()
+    for rid in range(10):
+        alone = Server(budget_bytes=64 << 30, sizes=MIX_SIZES, chunk_slots=MIX_CHUNKS)
+        one = alone.run([_mix_requests()[rid]])[0]
+        assert np.array_equal(one.psi.cpu().numpy(), res[rid].psi.cpu().numpy()), rid
+        assert np.array_equal(one.support.cpu().numpy(), res[rid].support.cpu().numpy()), rid
+        assert np.array_equal(one.trace, res[rid].trace), rid
+        assert len(one.trace) == 25 and np.all(np.isfinite(one.trace))
+        alone.close()
+
+
+def test_serving_mix_sizes_lockstep_against_the_oracle():
+    """The first iterations of one request per size, read back from the
+    server after each one-iteration launch round, against the oracle's
+    iteration from the same state (1e-5 relative)."""
+    reqs = [r for r in _mix_requests() if r.rid in (0, 3, 6, 8)]
+    srv = Server(budget_bytes=64 << 30, sizes=MIX_SIZES, chunk_slots=MIX_CHUNKS, block=1)
+    for r in reqs:
+        srv.submit(r)
+    state = {r.rid: (r.guess.astype(np.complex128) * r.support) for r in reqs}
+    for k in range(3):
+        srv.tick()
+        for r in reqs:
+            print(f"[serving lockstep n={r.n} step {k}]", flush=True)
+            done, psi = srv.peek(r.rid)
+            assert done == k + 1
+            got = psi.cpu().numpy()
+            want, _, _ = O.iteration(state[r.rid], r.modulus, r.support, "hio", r.beta, "numpy")
+            rel = np.linalg.norm(got - want) / np.linalg.norm(want)
+            assert rel < 1e-5, (r.n, k, rel)
+            state[r.rid] = got.astype(np.complex128)
+    for r in reqs:
+        assert srv.cancel(r.rid)
+    srv.close()

@@ -105,6 +105,31 @@ def test_library_rejects_bad_arguments_without_gpu():
     assert "bad plan size" in _capi.last_error()
     assert lib.spk_plan_bytes(1024, 1, 1, 0, ctypes.byref(out)) == 0
     assert out.value >= 1024 * 1024 * (16 + 8)
+    assert lib.spk_plan_bytes(1024, 1, 1, 64, ctypes.byref(out)) == -1   # unknown option bit
+
+
+def test_plan_bytes_counts_the_buffers_of_each_option():
+    """spk_plan_bytes grows by each option's buffers (per frame: phib 8 B/px,
+    shrinkwrap maps 8 B/px, ipsf work 8 B/px, spectrum maps 8 B/px; two Phi
+    layouts at 4096)."""
+    from als_speckle_amd.device import plan_bytes
+    n, B = 1024, 3
+    nn = n * n
+    base = plan_bytes(n, B, False)
+    assert base >= B * nn * (8 + 8 + 4 + 4)
+    assert plan_bytes(n, B, False, _capi.PB_DM) - base == B * nn * 8
+    assert plan_bytes(n, B, False, _capi.PB_SHRINKWRAP) - base >= B * nn * 8
+    assert plan_bytes(n, B, False, _capi.PB_IPSF) - base >= (B + 1) * nn * 8
+    assert plan_bytes(n, B, False, _capi.PB_SPECTRUM) - base >= B * nn * 8
+    assert plan_bytes(n, B, False, _capi.PB_TRACE) > base
+    every = sum((_capi.PB_DM, _capi.PB_SHRINKWRAP, _capi.PB_IPSF, _capi.PB_SPECTRUM,
+                 _capi.PB_TRACE))
+    assert plan_bytes(n, B, False, every) - base == sum(
+        plan_bytes(n, B, False, o) - base for o in (1, 2, 4, 8, 16))
+    # the dual Phi layout at 4096: psi + two Phi layouts, and two phib layouts
+    big = 4096 * 4096
+    assert plan_bytes(4096, 1, True) >= big * (8 * 3 + 8)
+    assert plan_bytes(4096, 1, True, _capi.PB_DM) - plan_bytes(4096, 1, True) == big * 16
 
 
 def test_phasing_has_no_cpu_path():
@@ -118,10 +143,10 @@ def test_phasing_has_no_cpu_path():
 
 def test_server_budget_sizing():
     from als_speckle_amd.device import plan_bytes
-    from als_speckle_amd.serve import Server
+    from als_speckle_amd.serve import CHUNK_OPTIONS, Server
     mix = {256: 0.4, 512: 0.3, 1024: 0.2, 2048: 0.1}
     slots = Server.size_slots(8 << 30, mix)
-    used = sum(plan_bytes(n, k, False) for n, k in slots.items())
+    used = sum(plan_bytes(n, k, False, CHUNK_OPTIONS) for n, k in slots.items())
     assert used <= (8 << 30) * 1.01
     assert slots[2048] >= 1 and slots[256] >= 1
     # larger frames get more of the budget, smaller frames more slots
@@ -137,6 +162,10 @@ def test_autotune_candidates_are_built_shapes():
             # every candidate is a shape the library builds
             assert (n // ept) * tile <= 1024 and not (n == 4096 and tile == 8)
         assert (4, 16, False) in cands
+        # the persistent launch mode is searched only on request
+        assert not any(p for _, _, p in cands)
+        assert {c[:2] for c in candidates(n, persist=True)} == {c[:2] for c in cands}
+        assert len(candidates(n, persist=True)) == 2 * len(cands)
     assert (2, 8, False) in candidates(4096) and (4, 16, False) in candidates(4096)
     assert all(t != 8 for t, _, _ in candidates(4096))
 
@@ -196,16 +225,76 @@ def test_serving_block_table_advances_each_slot_on_its_own_schedule():
     assert np.array_equal(table[:7, 2], b) and np.all(table["alg"][7:, 2] == _capi.NOOP)
 
 
-def test_serving_admission_order():
-    from als_speckle_amd.serve import admission_order
+def _admit(q, now, free, idle=(), budget=10**9, in_use=0, growable=True):
+    from als_speckle_amd.serve import plan_admission
+    slots = {256: 4, 512: 2, 2048: 1}
+    byts = {256: 100, 512: 200, 2048: 600}
+    return plan_admission(q, now, free, list(idle), slots, byts, budget, in_use, growable)
+
+
+def test_serving_admission_static_slots():
+    """Fixed slots (no growth): first come first served within a size, a
+    full size does not block another, requests that have not arrived wait."""
     q = [_req(0, 512, 0.0), _req(1, 512, 0.1), _req(2, 256, 0.2), _req(3, 512, 0.3),
          _req(4, 256, 5.0)]
-    free = {512: 1, 256: 2}
-    admitted, waiting = admission_order(q, free, now=1.0)
-    # first come first served within a size; a full class does not block another;
-    # requests that have not arrived wait
-    assert [r.rid for r in admitted] == [0, 2]
-    assert [r.rid for r in waiting] == [1, 3, 4]
-    assert free == {512: 0, 256: 1}
-    admitted, waiting = admission_order(waiting, {512: 2, 256: 1}, now=10.0)
-    assert [r.rid for r in admitted] == [1, 3, 4] and not waiting
+    a = _admit(q, 1.0, {512: 1, 256: 2}, growable=False)
+    assert [r.rid for r in a.admitted] == [0, 2]
+    assert [r.rid for r in a.waiting] == [1, 3, 4]
+    assert not a.allocate and not a.release
+    a = _admit(a.waiting, 10.0, {512: 2, 256: 1}, growable=False)
+    assert [r.rid for r in a.admitted] == [1, 3, 4] and not a.waiting
+
+
+def test_serving_admission_grows_chunks_within_the_budget():
+    # budget 1000, nothing allocated: 5 x 256 need two 4-slot chunks (200),
+    # 3 x 512 two 2-slot chunks (400); the 2048 chunk (600) no longer fits
+    q = [_req(i, 256, 0.0) for i in range(5)] + [_req(5 + i, 512, 0.0) for i in range(3)] + \
+        [_req(8, 2048, 0.0), _req(9, 256, 0.0)]
+    a = _admit(q, 0.0, {}, budget=1000)
+    assert a.allocate == {256: 2, 512: 2}
+    assert [r.rid for r in a.admitted] == [0, 1, 2, 3, 4, 5, 6, 7, 9]
+    assert [r.rid for r in a.waiting] == [8] and not a.release
+
+
+def test_serving_admission_shares_the_budget_across_sizes():
+    """A size with no free slot takes memory from idle chunks of other sizes
+    (oldest idle first), never from chunks with running requests."""
+    # 800 of 1000 in use: two idle 256 chunks (oldest first), one 512 chunk
+    # with a running request, one idle 512 chunk
+    idle = [(256, 0), (512, 1), (256, 1)]
+    a = _admit([_req(0, 2048, 0.0)], 0.0, {256: 8, 512: 3}, idle, budget=1000, in_use=800)
+    # needs 600: 200 free, + (256, 0) 100 + (512, 1) 200 + (256, 1) 100 = 600
+    assert a.release == [(256, 0), (512, 1), (256, 1)]
+    assert a.allocate == {2048: 1} and [r.rid for r in a.admitted] == [0]
+    # not enough reclaimable memory: the request waits and nothing is released,
+    # while a request of another size that fits still goes ahead
+    a = _admit([_req(0, 2048, 0.0), _req(1, 256, 0.0)], 0.0, {256: 1}, [(256, 0)],
+               budget=1000, in_use=900)
+    assert not a.release and not a.allocate
+    assert [r.rid for r in a.admitted] == [1] and [r.rid for r in a.waiting] == [0]
+    # a later request of the same size waits behind the first (FCFS within a size)
+    a = _admit([_req(0, 2048, 0.0), _req(1, 2048, 0.0)], 0.0, {}, [], budget=500)
+    assert [r.rid for r in a.waiting] == [0, 1] and not a.allocate
+
+
+def test_serving_chunk_shapes_fit_the_budget():
+    from als_speckle_amd.device import plan_bytes
+    from als_speckle_amd.serve import CHUNK_OPTIONS, chunk_shapes
+    slots, byts = chunk_shapes((256, 512, 1024, 2048), 48 << 30)
+    for n in slots:
+        assert byts[n] == plan_bytes(n, slots[n], False, CHUNK_OPTIONS)
+        assert 1 <= slots[n] <= 64 and (byts[n] <= (1 << 30) or slots[n] == 1)
+    assert slots[256] >= slots[1024] >= slots[2048]
+    small, _ = chunk_shapes((2048,), 1 << 30)       # an eighth of a small budget
+    assert small[2048] == 1
+
+
+def test_bench_finds_the_traffic_summary_of_either_dominant_pass():
+    """roofline.traffic comes from the committed PMC summary of whichever
+    pass dominates the headline (column or row pass)."""
+    import bench
+    for dom in ("column", "row"):
+        path = bench.traffic_file(dom)
+        assert os.path.exists(path), path
+        t = bench.measured_traffic(dom)
+        assert isinstance(t, int) and t > 0
