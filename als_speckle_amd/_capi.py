@@ -70,6 +70,7 @@ SIGNATURES = {
     "spk_copy_roi": (c_int, [c_void_p, c_int, c_void_p, c_int, c_int, c_int, c_int, c_int, c_int]),
     "spk_fft2": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_int]),
     "spk_align_global_phase": (c_int, [c_void_p, c_void_p, c_int, c_int, c_int]),
+    "spk_align_global_phase_noalloc": (c_int, [c_void_p, c_void_p, c_int, c_int, c_int]),
     "spk_copy_bytes": (c_int, [c_void_p, c_void_p, c_void_p, c_size_t]),
     "spk_stack_mean": (c_int, [c_void_p, c_void_p, c_int, c_int, c_int, c_void_p]),
     "spk_com_average": (c_int, [c_void_p, c_void_p, c_int, c_int, c_int, c_void_p]),

@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -396,6 +397,41 @@ int do_shrinkwrap(spk_plan* p, int frame0, int frames, float sigma, float tau) {
 
 }  // namespace
 
+// ---------------------------------------------------------------------------
+// Workspace of spk_align_global_phase_noalloc: the per-block FP64 partial
+// sums (frames x 5 sums x blocks) of the alignment, in a file-scope device
+// array instead of an allocation.  Every GPU the library runs on holds its
+// own copy until the process ends; its address on each device is looked up
+// once, with that device current, when the first context on the device is
+// created, and only that address is handed to kernels.  The partial-sum
+// kernel writes every entry the apply kernel reads, so the array needs no
+// clearing; calls that overlap on one GPU share it and race.
+// ---------------------------------------------------------------------------
+constexpr size_t kScratchDoubles = (size_t)1 << 18;   // 2 MiB
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+
+// the device's copy of `scratch`; device must be current
+int static_scratch(int device, double** out) {
+  SPK_REQUIRE(device >= 0 && device < kMaxDevices, SPK_EUNSUP,
+              "device index above " + std::to_string(kMaxDevices - 1));
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    void* ptr = nullptr;
+    hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+      return fail(SPK_EHIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_ptr[device] = static_cast<double*>(ptr);
+  }
+  *out = g_scratch_ptr[device];
+  return SPK_OK;
+}
+}  // namespace
+
 // ===========================================================================
 extern "C" {
 
@@ -418,6 +454,9 @@ int spk_ctx_create(int device, void* stream, spk_ctx** out) {
               "device " + std::to_string(device) + " not present (" + std::to_string(count) +
                   " devices)");
   SPK_HIP(hipSetDevice(device));
+  double* ws = nullptr;   // the allocation-free entry point's workspace on this device
+  int rc = static_scratch(device, &ws);
+  if (rc) return rc;
   spk_ctx* c = new spk_ctx;
   c->device = device;
   c->stream = (hipStream_t)stream;
@@ -1168,6 +1207,25 @@ int spk_align_global_phase(spk_ctx* ctx, float* data, int frames, int rows, int 
     ctx->scratch_cap = need;
   }
   SPK_HIP(launch_align((c32*)data, frames, per, ctx->scratch, nblk, ctx->stream));
+  return SPK_OK;
+}
+
+int spk_align_global_phase_noalloc(spk_ctx* ctx, float* data, int frames, int rows, int cols) {
+  SPK_REQUIRE(data && frames >= 1 && rows >= 1 && cols >= 1, SPK_EINVAL, "bad stack");
+  int rc = set_device(ctx);
+  if (rc) return rc;
+  // the same partition as spk_align_global_phase (same kernels, same sums)
+  const long long per = (long long)rows * cols;
+  const int nblk = (int)std::min<long long>(64, (per + 255) / 256);
+  const size_t need = (size_t)frames * 5 * nblk;
+  SPK_REQUIRE(need <= kScratchDoubles, SPK_EINVAL,
+              "stack of " + std::to_string(frames) + " frames of " + std::to_string(rows) + "x" +
+                  std::to_string(cols) + " needs " + std::to_string(need) +
+                  " partial sums; the static workspace holds " + std::to_string(kScratchDoubles) +
+                  " (frames x 5 x min(64, ceil(rows*cols/256)))");
+  double* ws = nullptr;
+  if ((rc = static_scratch(ctx->device, &ws))) return rc;
+  SPK_HIP(launch_align((c32*)data, frames, per, ws, nblk, ctx->stream));
   return SPK_OK;
 }
 

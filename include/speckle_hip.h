@@ -227,6 +227,17 @@ int spk_fft2(spk_plan* plan, const float* in_c64, float* out_c64, int count,
 int spk_align_global_phase(spk_ctx* ctx, float* data_c64, int frames,
                            int rows, int cols);
 
+/* spk_align_global_phase without any device allocation.  No counterpart in
+ * the reference: for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The same
+ * kernels and results as spk_align_global_phase; the partial sums live in
+ * a static per-device workspace of 2 MiB instead of the context's grown
+ * buffer, so frames x 5 x min(64, ceil(rows*cols/256)) must not exceed
+ * 262144 (e.g. 819 frames of 128 x 128 or more; SPK_EINVAL beyond).  Calls
+ * that overlap on one GPU share the workspace and race. */
+int spk_align_global_phase_noalloc(spk_ctx* ctx, float* data_c64, int frames,
+                                   int rows, int cols);
+
 /* refine_support's blur (phasing.py:1787-1790): out = Re ifft2(fft2(K) *
  * fft2(in)) for K = fftshift(shape.gaussian((rows, cols), (sigma, sigma)))
  * normalised to sum 1 (shape.py:410-530), computed as the equivalent
