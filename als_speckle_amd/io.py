@@ -1,7 +1,8 @@
-"""FITS input and output for moduli, supports and reconstructions, without
+"""Input and output of moduli, supports and reconstructions without
 pyfits/astropy (``speckle/io.py``: ``open`` 35-91, ``save`` 93-175,
-``openfits`` 384-436, ``writefits`` 502-551, ``save_fits`` 714-745,
-``_labview_to_andor`` 626-645).
+``openfits`` 384-436, ``openimage`` 480-500, ``writefits`` 502-551,
+``save_fits`` 714-745, ``_labview_to_andor`` 626-645, ``open_ds9_mask``
+1309-1474): FITS, greyscale images through PIL, text arrays, ds9 regions.
 
 Implements the FITS standard's primary HDU and IMAGE extensions: 2880-byte
 blocks of 80-character header cards, big-endian data with
@@ -19,7 +20,7 @@ import os
 import numpy as np
 
 __all__ = ["open", "save", "openfits", "writefits", "save_fits", "read_header",
-           "set_overwrite"]
+           "set_overwrite", "openimage", "open_ds9_mask", "get_fits_dimensions"]
 
 BLOCK = 2880
 CARD = 80
@@ -255,19 +256,196 @@ def save_fits(filename, img, header=None, components=None, overwrite=None,
         writefits(name, _save_maps[c](np.asarray(img)), dict(header), overwrite)
 
 
+def get_fits_dimensions(filename, card=0):
+    """Dimensions of a FITS image from its header, slowest axis first
+    (``io.py:863-880``)."""
+    hdr = read_header(filename, card)
+    return tuple(int(hdr["NAXIS%d" % k]) for k in range(int(hdr["NAXIS"]), 0, -1))
+
+
+def openimage(filename):
+    """A greyscale image as an integer array (``io.py:480-500``: PIL
+    ``convert("L")``; colour images are converted to luminance)."""
+    from PIL import Image
+    with Image.open(filename) as im:
+        return np.asarray(im.convert("L")).copy()
+
+
+# -- ds9 region masks (io.py:1309-1474 with shape.py:138-420) ----------------
+# The shape rules restate the reference's helpers with AA=False, including
+# how it reads each region's numbers: ds9 positions are 1-based and become
+# 0-based; a box's width and height are passed as (rows, columns) lengths;
+# polygon vertices are shifted by one pixel more than the other shapes; an
+# annulus uses its first two radii.
+
+def _radial_mask(size, center, radius):
+    """``shape.circle(size, radius, center, AA=False)``: r < radius."""
+    rows, cols = np.indices(size, float)
+    return np.where(np.hypot(rows - center[0], cols - center[1]) < radius, 1, 0)
+
+
+def _rect(size, lengths, center):
+    """``shape.rect`` (``shape.py:138-215``): integer lengths and centre,
+    half lengths by floor division; where the box leaves the array the
+    reference's clipping is kept as written (a lower edge below 0 is clipped
+    to 0; an upper edge past the end replaces the other bound)."""
+    size = (int(size[0]), int(size[1]))
+    lengths = (int(lengths[0]), int(lengths[1]))
+    center = (int(center[0]), int(center[1]))
+    assert lengths[0] > 0 and lengths[1] > 0, "lengths must be > 0"
+    assert center[0] > 0 and center[1] > 0, "center must be > 0"
+    r_min, r_max = center[0] - lengths[0] // 2, center[0] + lengths[0] // 2
+    c_min, c_max = center[1] - lengths[1] // 2, center[1] + lengths[1] // 2
+    if r_min < 0:
+        r_min = 0
+    if c_min < 0:
+        c_min = 0
+    if r_max > size[0]:
+        r_min = size[0]
+    if c_max > size[1]:
+        r_max = size[1]
+    temp = np.zeros(size, int)
+    temp[r_min:r_max, c_min:c_max] = 1
+    return temp
+
+
+def _ellipse(size, axes, center, angle):
+    """``shape.ellipse(..., AA=False)`` (``shape.py:330-400``) with the
+    rotated indices of ``shape._make_indices`` (``shape.py:532-556``)."""
+    rows, cols = np.indices(size).astype(float)
+    rows -= center[0]
+    cols -= center[1]
+    if angle != 0:
+        a = angle * np.pi / 180.0
+        rows, cols = rows * np.cos(a) + cols * np.sin(a), cols * np.cos(a) - rows * np.sin(a)
+    ratio = float(axes[1]) / float(axes[0])
+    if ratio >= 1:
+        rows = rows * (float(axes[1]) / axes[0])
+        radius = axes[1]
+    else:
+        cols = cols * (1.0 / ratio)
+        radius = axes[0]
+    return np.where(np.hypot(rows, cols) < radius, 1, 0)
+
+
+def _polygon(desc, dim):
+    """``_draw_polygon`` (``io.py:1309-1326``): PIL fill and outline of the
+    (x, y) vertices."""
+    from PIL import Image, ImageDraw
+    ys, xs = dim
+    verts = [(desc[2 * v], desc[2 * v + 1]) for v in range(len(desc) // 2)]
+    img = Image.new("L", (xs, ys), 0)
+    ImageDraw.Draw(img).polygon(verts, outline=1, fill=1)
+    return np.array(img).astype(int)
+
+
+_REGION_SHAPES = ("box", "circle", "polygon", "ellipse", "annulus")
+
+
+def _parse_region_file(filename, force_reg_size):
+    import re
+    file_exp = re.compile(r"# Filename: ((?:\/[\w\.\-\ ]+)+)(?:(\[\w\])?)")
+    shapes, dim = [], None
+    with builtins.open(filename, "r") as fh:
+        for line in fh:
+            if file_exp.match(line):
+                spl = file_exp.split(line)
+                afile = spl[1]
+                card = spl[2][1:-1] if len(spl) >= 3 and spl[2] else 0
+                try:
+                    dim = get_fits_dimensions(afile, card)
+                except (IOError, OSError, KeyError, ValueError):
+                    dim = None
+                continue
+            if line.split("(")[0] in _REGION_SHAPES:
+                shapes.append(line.rstrip("\n"))
+    if dim is None:
+        # the FITS file named in the region file is unreadable (or no file is
+        # named: the reference then fails) -- force_reg_size, else 2048^2
+        if force_reg_size is None:
+            dim = (2048, 2048)
+        elif isinstance(force_reg_size, (tuple, list)):
+            dim = (int(force_reg_size[0]), int(force_reg_size[1]))
+        else:
+            dim = (int(force_reg_size), int(force_reg_size))
+    if len(dim) == 3:
+        dim = dim[1:]
+    return shapes, tuple(int(d) for d in dim)
+
+
+def open_ds9_mask(filename, individual_regions=False, remove_intersections=False,
+                  force_reg_size=None):
+    """A mask from a ds9 region file (``io.py:1328-1474``): box, circle,
+    polygon, ellipse and annulus regions, on an array the size of the FITS
+    file the region file names (else ``force_reg_size`` or 2048 x 2048).
+
+    Returns a 0/1 mask of the union of the regions (with
+    ``remove_intersections``, of the pixels covered an odd number of times);
+    with ``individual_regions`` the mask times the sum of the region numbers
+    (1, 2, ... in file order) covering each pixel."""
+    assert isinstance(individual_regions, (bool, int)), \
+        "individual_regions must be boolean-evaluable"
+    assert isinstance(remove_intersections, (bool, int)), \
+        "remove_intersections must be boolean-evaluable"
+    assert isinstance(force_reg_size, (type(None), int, tuple, list)), \
+        "force_size must be None, int, or iterable"
+    if isinstance(force_reg_size, (tuple, list)):
+        assert len(force_reg_size) == 2, "force_reg_size must be of length 2"
+    shapes, dim = _parse_region_file(filename, force_reg_size)
+    data = np.zeros(dim)
+    binarydata = np.zeros(dim)
+    for count, sdesc in enumerate(shapes, start=1):
+        kind = sdesc.split("(")[0]
+        desc = [float(v) for v in sdesc[sdesc.find("(") + 1:sdesc.find(")")].split(",")]
+        desc[0] -= 1
+        desc[1] -= 1
+        if kind == "circle":
+            obj = _radial_mask(dim, (desc[1], desc[0]), desc[2])
+        elif kind == "box":
+            center = (desc[1], desc[0])
+            obj = _rect(dim, (desc[2], desc[3]), center)
+            if len(desc) == 7:
+                obj = _rect(dim, (desc[4], desc[5]), center) - obj
+        elif kind == "polygon":
+            obj = _polygon([v - 1 for v in desc], dim)
+        elif kind == "ellipse":
+            angle = -1 * desc[-1]
+            center = (desc[1], desc[0])
+            obj = _ellipse(dim, (desc[2], desc[3]), center, angle)
+            if len(desc) == 7:
+                obj = _ellipse(dim, (desc[4], desc[5]), center, angle) - obj
+        else:  # annulus
+            center = (desc[1], desc[0])
+            assert center[0] > 0 and center[1] > 0, "center must be > 0"
+            r = (desc[2], desc[3])
+            obj = _radial_mask(dim, center, max(r)) - _radial_mask(dim, center, min(r))
+        data += count * obj
+        binarydata += obj
+    if remove_intersections:
+        mask = np.where((binarydata % 2) == 0, 0, 1)
+    else:
+        mask = np.where(binarydata >= 1, 1, 0)
+    return mask * data if individual_regions else mask
+
+
 def open(filename, quiet=True, orientImageHDU=True, convert_to="float", delimiter="\t",
          force_reg_size=None):
     """Open a data file by extension (``io.py:35-91``): FITS (.fits, .fit and
-    gzip .gz/.zip) and text arrays (.txt, .csv).  Image, pickle and ds9
-    region files are not provided."""
+    gzip .gz/.zip), images (.jpg, .jpeg, .gif, .png, .bmp: greyscale),
+    text arrays (.txt, .csv) and ds9 regions (.reg: a mask).  Pickles
+    (.pck) are not opened: loading one runs code from the file."""
     assert len(filename.split(".")) >= 2, "filename appears to have no extension"
     ext = filename.split(".")[-1].lower()
     if ext in ("fits", "fit", "gz", "zip"):
         return openfits(filename, quiet=quiet, orientImageHDU=orientImageHDU)
+    if ext in ("jpg", "jpeg", "gif", "png", "bmp"):
+        return openimage(filename)
     if ext in ("txt", "csv"):
         return np.loadtxt(filename, delimiter=delimiter if ext == "txt" else ",").astype(
             convert_to)
-    raise NotImplementedError("opening .%s files is not provided (FITS and text only)" % ext)
+    if ext == "reg":
+        return open_ds9_mask(filename, force_reg_size=force_reg_size)
+    raise NotImplementedError("opening .%s files is not provided" % ext)
 
 
 def save(filename, data, header=None, components=None, overwrite=None, do_zip=False,

@@ -66,15 +66,21 @@ def parse():
     ap.add_argument("--kernel-probe", action="store_true",
                     help="only a 256 MiB device copy (HBM counter calibration) and 20 "
                          "iterations of the headline frame, for rocprofv3 --pmc runs")
-    ap.add_argument("--probe-frame", default="single_1024", choices=("single_1024", "large_4096"),
-                    help="the frame of --kernel-probe")
+    ap.add_argument("--probe-frame", default="single_1024",
+                    choices=("single_1024", "large_4096", "trials_512", "trials_1024"),
+                    help="the workload of --kernel-probe: one 1024^2 or 4096^2 frame (headline "
+                         "schedule), 64 trials of the 512^2 frame (RAAR beta 0.87), or 16 "
+                         "trials of the 1024^2 frame (headline schedule)")
     return ap.parse_args()
 
 
 def kernel_probe(ctx, which="single_1024"):
-    """A copy of known size, then 20 iterations of the headline schedule on
-    one frame (the 1024^2 headline frame or the 4096^2 large frame)."""
+    """A copy of known size, then 20 iterations of one workload: the
+    headline schedule on one frame (the 1024^2 headline frame or the 4096^2
+    large frame) or on 16 trials of the 1024^2 frame, or RAAR (beta 0.87)
+    on 64 trials of the 512^2 frame sharing one modulus and support."""
     import torch
+    from als_speckle_amd import _capi
     from als_speckle_amd.device import Plan
     from als_speckle_amd import synth
     nbytes = 256 << 20
@@ -82,19 +88,28 @@ def kernel_probe(ctx, which="single_1024"):
     dst = torch.empty_like(src)
     for _ in range(3):
         ctx.copy_bytes(dst, src)
-    frame = synth.config_frame(which, 0)
+    frame = synth.config_frame("single_1024" if which == "trials_1024" else which, 0)
     n = frame.modulus.shape[0]
-    plan = Plan(ctx, n, batch=1, shared=True)
-    plan.set_modulus(torch.from_numpy(frame.modulus).to(ctx.device))
-    plan.set_support(torch.from_numpy(frame.support).to(ctx.device))
-    np.random.seed(1)
-    g = (np.random.rand(n, n) + 1j * np.random.rand(n, n)).astype(np.complex64)
-    plan.set_psi(torch.from_numpy(g).to(ctx.device))
-    steps, _ = headline_steps(20)
+    batch = {"trials_512": 64, "trials_1024": 16}.get(which, 1)
+    shared = which != "trials_1024"
+    plan = Plan(ctx, n, batch=batch, shared=shared)
+    m = torch.from_numpy(frame.modulus).to(ctx.device)
+    s = torch.from_numpy(frame.support).to(ctx.device)
+    plan.set_modulus(m if shared else m.expand(batch, n, n).contiguous())
+    plan.set_support(s if shared else s.expand(batch, n, n).contiguous())
+    g = np.empty((batch, n, n), np.complex64)
+    for t in range(batch):
+        np.random.seed(1 + t)
+        g[t] = (np.random.rand(n, n) + 1j * np.random.rand(n, n)).astype(np.complex64)
+    plan.set_psi(torch.from_numpy(g if batch > 1 else g[0]).to(ctx.device))
+    if which == "trials_512":
+        steps = [(_capi.RAAR, _capi.SEM_NUMPY, 0.87, 0.0, 0.0)] * 20
+    else:
+        steps, _ = headline_steps(20)
     plan.iterate(steps)
     ctx.sync()
-    print(json.dumps({"kernel_probe": True, "copy_bytes": nbytes, "copies": 3,
-                      "iterations": 20, "n": n}), flush=True)
+    print(json.dumps({"kernel_probe": True, "workload": which, "copy_bytes": nbytes, "copies": 3,
+                      "iterations": 20, "n": n, "frames": batch}), flush=True)
 
 
 def traffic_file(dom):

@@ -84,3 +84,31 @@ def test_progress_printing(capsys):
     out = capsys.readouterr().out
     assert "iteration 5" in out and "iteration 10" in out
     assert r.get_error_trace().shape == (10,)
+
+
+def test_inputs_from_fits_and_png_files(tmp_path):
+    """load() with a modulus read from a FITS file and a support read from a
+    PNG (0/255 greyscale) and from a ds9 region file through io.open
+    (speckle/io.py:35-91): the same iterations as from the arrays."""
+    from PIL import Image
+    from als_speckle_amd import io as sio
+    f = synth.make_frame(128, 9, shape="disk", radius=20, photons=1e6)
+    sio.writefits(str(tmp_path / "m.fits"), f.modulus, overwrite=True)
+    Image.fromarray((f.support * 255).astype(np.uint8), "L").save(tmp_path / "s.png")
+    m = sio.open(str(tmp_path / "m.fits"))
+    s = sio.open(str(tmp_path / "s.png"))
+    assert s.dtype == np.uint8 and np.array_equal(s > 0, f.support > 0)
+    # a disk region of radius 23 at the frame centre (1-based ds9 position)
+    reg = tmp_path / "s.reg"
+    reg.write_text("# Filename: %s\ncircle(65,65,23)\n" % (tmp_path / "m.fits"))
+    sreg = sio.open(str(reg))
+    assert sreg.shape == (128, 128)
+    runs = []
+    for mod, sup in ((f.modulus, f.support), (m, s), (f.modulus, sreg.astype(np.float32))):
+        r = phasing()
+        r.load(modulus=mod, support=sup, numtrials=1)
+        r.seed(4)
+        r.iterate(12, order=(("hio", 10), ("er", 2)), beta=0.9)
+        runs.append((r.get(r.psi_in), r.get_error_trace()))
+    assert np.array_equal(runs[0][0], runs[1][0]) and np.array_equal(runs[0][1], runs[1][1])
+    assert np.all(np.isfinite(runs[2][1])) and runs[2][1][-1] < runs[2][1][0]
