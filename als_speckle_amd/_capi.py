@@ -31,7 +31,8 @@ class SpkStep(ctypes.Structure):
 
 class SpkTune(ctypes.Structure):
     _fields_ = [("tile", c_int32), ("ept", c_int32), ("persist", c_int32),
-                ("sw_separate", c_int32), ("reserved", c_int32 * 4)]
+                ("sw_separate", c_int32), ("resident", c_int32), ("exchange", c_int32),
+                ("reserved", c_int32 * 2)]
 
 
 class SpkError(RuntimeError):
@@ -90,6 +91,15 @@ SIGNATURES = {
     "spk_graph_launch": (c_int, [c_void_p]),
     "spk_graph_destroy": (c_int, [c_void_p]),
     "spk_plan_profile_read": (c_int, [c_void_p, c_int, POINTER(c_int), POINTER(c_double)]),
+    "spk_plan_resident_counts": (c_int, [c_void_p, POINTER(ctypes.c_longlong),
+                                         POINTER(ctypes.c_longlong)]),
+    "spk_group_shape": (c_int, [c_int, POINTER(c_int), POINTER(c_int)]),
+    "spk_group_create": (c_int, [c_void_p, POINTER(c_void_p)]),
+    "spk_group_destroy": (c_int, [c_void_p]),
+    "spk_group_iterate": (c_int, [c_void_p, POINTER(c_void_p), c_int, POINTER(c_void_p), c_int,
+                                  POINTER(c_int), POINTER(c_void_p)]),
+    "spk_group_counts": (c_int, [c_void_p, POINTER(ctypes.c_longlong),
+                                 POINTER(ctypes.c_longlong)]),
 }
 
 _lib = None

@@ -47,6 +47,8 @@ def _units():
                       os.path.join(BUILD, f"spk_kernels_{n}.o"), [f"-DSPK_ONLY_N={n}"]))
     units.append((os.path.join(HERE, "spk_kernels.hip"),
                   os.path.join(BUILD, "spk_kernels_common.o"), ["-DSPK_ONLY_N=0"]))
+    units.append((os.path.join(HERE, "spk_kernels.hip"),
+                  os.path.join(BUILD, "spk_kernels_group.o"), ["-DSPK_ONLY_N=-1"]))
     units.append((os.path.join(HERE, "spk_capi.hip"), os.path.join(BUILD, "spk_capi.o"), []))
     return units
 

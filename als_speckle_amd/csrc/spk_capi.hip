@@ -132,7 +132,7 @@ struct spk_plan {
   hipEvent_t upload_done = nullptr;
   bool upload_pending = false;
   bool profiling = false;
-  std::vector<hipEvent_t> prof_ev[2];  // start/stop pairs per pass
+  std::vector<hipEvent_t> prof_ev[3];  // start/stop pairs: column, row, resident
   bool mod_set = false, sup_set = false, psi_set = false;
   // per frame: g != 0 when phib == rowFFT((s - g (1 - s)) psi) for the
   // current psi and support (dm1 of a DM step with beta = 1/g), 0 when stale
@@ -145,6 +145,11 @@ struct spk_plan {
   int pgrid = 0;
   // shrinkwraps fused into the passes where possible (spk_tune.sw_separate = 0)
   bool sw_fuse = true;
+  // resident single-frame launches (k_resident) and their exchange mode
+  int resident = 0;
+  int xm = 0;
+  unsigned* rsync = nullptr;   // barrier flags of the resident launches (a line per band)
+  long long res_launches = 0, res_steps = 0;
   long long sw_fused_count = 0, sw_separate_count = 0;
   unsigned* pbar = nullptr;
   // partial coherence: ipsf (tiled, shared by the frames) and the per-frame
@@ -153,6 +158,11 @@ struct spk_plan {
   c32* dbuf = nullptr;
   c32* dbufc = nullptr;
   bool ipsf_on = false;
+  // the same per-frame work buffers for DM's second projection (of phib)
+  c32* dbuf2 = nullptr;
+  c32* dbufc2 = nullptr;
+  float* imap2 = nullptr;
+  float* dmap2 = nullptr;
   // spectral rescale: factors and weights (device), per-frame intensity maps
   float* spec = nullptr;
   int nspec = 0;
@@ -264,6 +274,10 @@ PassArgs base_args(spk_plan* p) {
   a.dbufc = p->dual() ? p->dbufc : p->dbuf;
   a.imap = p->imap;
   a.dmap = p->dmap;
+  a.dbuf2 = p->dbuf2;
+  a.dbufc2 = p->dual() ? p->dbufc2 : p->dbuf2;
+  a.imap2 = p->imap2;
+  a.dmap2 = p->dmap2;
   a.sw_tmp = p->sw_tmp;
   a.sw_blur = p->sw_blur;
   a.sw_maxp = p->sw_maxp;
@@ -275,6 +289,20 @@ int ensure_phib(spk_plan* p) {
   int rc = dalloc(&p->phib, (size_t)p->batch * p->nn());
   if (rc || !p->dual()) return rc;
   return dalloc(&p->phibc, (size_t)p->batch * p->nn());
+}
+
+// work buffers of DM's partially coherent second projection
+int ensure_dm_pc(spk_plan* p) {
+  const size_t count = (size_t)p->batch * p->nn();
+  int rc;
+  if (p->ipsf_on && !p->dbuf2) {
+    if ((rc = dalloc(&p->dbuf2, count))) return rc;
+    if (p->dual() && (rc = dalloc(&p->dbufc2, count))) return rc;
+  }
+  if (p->spec_on && !p->imap2) {
+    if ((rc = dalloc(&p->imap2, count)) || (rc = dalloc(&p->dmap2, count))) return rc;
+  }
+  return SPK_OK;
 }
 
 int ensure_sw(spk_plan* p) {
@@ -501,6 +529,9 @@ int spk_plan_bytes(int n, int batch, int shared, int options, size_t* bytes) {
     b += 2 * nn * batch * sizeof(float) + (size_t)batch * bands * sizeof(float);
   if (options & SPK_PB_IPSF) b += nn * sizeof(c32) + lay * nn * batch * sizeof(c32);
   if (options & SPK_PB_SPECTRUM) b += 2 * nn * batch * sizeof(float) + 2 * 4096 * sizeof(float);
+  // DM with a partially coherent divisor: the work buffers of its second projection
+  if ((options & SPK_PB_DM) && (options & SPK_PB_IPSF)) b += lay * nn * batch * sizeof(c32);
+  if ((options & SPK_PB_DM) && (options & SPK_PB_SPECTRUM)) b += 2 * nn * batch * sizeof(float);
   if (options & SPK_PB_TRACE) b += 64 * (size_t)batch * bands * sizeof(double);
   b += 64 * 1024;                                      // step-table staging
   *bytes = b;
@@ -533,6 +564,13 @@ int spk_plan_create(spk_ctx* ctx, int n, int batch, int shared, const spk_tune* 
   p->bands = n / tile;
   p->persist = (tune && tune->persist > 0) ? 1 : 0;
   p->phib_g.assign(batch, 0.0f);
+  // resident launches by default for one frame where they are built: 15-25 %
+  // fewer microseconds per iteration than one launch per pass at 256^2,
+  // 512^2 and 1024^2 on MI355X; the exchange of Phi by device-scope
+  // accesses without fences is the default (the fenced forms are slower)
+  const int res_mode = tune ? tune->resident : 0;
+  p->resident = (res_mode != 2 && batch == 1 && resident_supported(n, tile, ept)) ? 1 : 0;
+  p->xm = (tune && tune->exchange >= 1 && tune->exchange <= 3) ? tune->exchange - 1 : 2;
   p->sw_fuse = !(tune && tune->sw_separate > 0);
   const size_t nn = (size_t)n * n;
   const size_t fms = shared ? 1 : (size_t)batch;
@@ -591,6 +629,13 @@ int spk_plan_sw_counts(const spk_plan* p, long long* fused, long long* separate)
   return SPK_OK;
 }
 
+int spk_plan_resident_counts(const spk_plan* p, long long* launches, long long* steps) {
+  SPK_REQUIRE(p, SPK_EINVAL, "null plan");
+  if (launches) *launches = p->res_launches;
+  if (steps) *steps = p->res_steps;
+  return SPK_OK;
+}
+
 int spk_plan_check(spk_plan* p) {
   SPK_REQUIRE(p, SPK_EINVAL, "null plan");
   int rc = set_device(p->ctx);
@@ -618,6 +663,7 @@ int spk_plan_destroy(spk_plan* p) {
   dfree(p->mod); dfree(p->sup);
   dfree(p->tw); dfree(p->summ2); dfree(p->sumpart); dfree(p->errp); dfree(p->pbar);
   dfree(p->ipsf); dfree(p->dbuf); dfree(p->dbufc);
+  dfree(p->dbuf2); dfree(p->dbufc2); dfree(p->imap2); dfree(p->dmap2); dfree(p->rsync);
   dfree(p->spec); dfree(p->imap); dfree(p->dmap);
   dfree(p->coh_ac); dfree(p->coh_w); dfree(p->coh_cb); dfree(p->coh_m); dfree(p->coh_part);
   dfree(p->sw_tmp); dfree(p->sw_blur); dfree(p->sw_maxp); dfree(p->fftbuf);
@@ -626,6 +672,7 @@ int spk_plan_destroy(spk_plan* p) {
   if (p->cap_h) (void)hipHostFree(p->cap_h);
   prof_clear(p, 0);
   prof_clear(p, 1);
+  prof_clear(p, 2);
   if (p->stage_h) (void)hipHostFree(p->stage_h);
   if (p->upload_done) (void)hipEventDestroy(p->upload_done);
   if (p->ctx) ctx_release(p->ctx);
@@ -776,24 +823,59 @@ int spk_set_spectrum(spk_plan* p, const float* rescales, const float* weights, i
   return SPK_OK;
 }
 
-int spk_iterate(spk_plan* p, const spk_step* steps, int nsteps, int per_frame,
-                double* err_trace) {
-  SPK_REQUIRE(p, SPK_EINVAL, "null plan");
-  int rc = set_device(p->ctx);
-  if (rc) return rc;
-  SPK_REQUIRE(nsteps >= 0, SPK_EINVAL, "nsteps must be >= 0");
-  if (nsteps == 0) return SPK_OK;
-  SPK_REQUIRE(steps, SPK_EINVAL, "null steps");
+// ---------------------------------------------------------------------------
+// One spk_iterate call in three parts, shared with spk_group_iterate:
+// iter_prepare checks the step table and stages the call's per-step tables
+// (one upload), iter_step queues the launches of one step (or of the resident
+// or persistent block that starts there), iter_finish queues the error
+// finalisation and records the DM estimates' state.
+// ---------------------------------------------------------------------------
+namespace {
+
+struct IterRun {
+  spk_plan* p = nullptr;
+  int nsteps = 0, B = 0, W = 0;
+  bool per_frame = false;
+  double* err_trace = nullptr;
+  std::vector<Coef> coefs;     // [step][frame entry]
+  std::vector<int> act;        // [step][frame entry]: not NOOP
+  std::vector<char> any_dm, any_sw, emits, in_res, sw_fuse, noout, dm_all;
+  std::vector<size_t> map_off, sw_off, swp_off, dm_off, dmall_off, res_off, res_sw_off, swt_off,
+      block_off;
+  std::vector<int> map_n, sw_n, sw_r, dm_n, dmall_n, res_end, res_n, res_fused, block_end;
+  std::vector<float> dm_g, fr;
+  std::vector<SwParam> sw_first;
+  std::vector<std::vector<SwParam>> sw_prm;
+  size_t coef_off = 0, act_off = 0;
+  unsigned char* dbase = nullptr;
+  unsigned char* dptr(size_t off) const { return dbase + off; }
+};
+
+// blocks: allow resident and persistent blocks (not in a group call, whose
+// steps launch one pass at a time for every member)
+int iter_prepare(spk_plan* p, const spk_step* steps, int nsteps, int per_frame,
+                 double* err_trace, bool blocks, IterRun& r) {
+  int rc = SPK_OK;
   SPK_REQUIRE(p->mod_set && p->sup_set && p->psi_set, SPK_ESTATE,
               "cant iterate before loading modulus, support and a seed");
-  hipStream_t st = p->ctx->stream;
   const int B = p->batch;
   const int W = per_frame ? B : 1;  // step entries per iteration
+  r.p = p;
+  r.nsteps = nsteps;
+  r.B = B;
+  r.W = W;
+  r.per_frame = per_frame != 0;
+  r.err_trace = err_trace;
 
   // coefficients for every (step, frame entry)
-  std::vector<Coef> coefs((size_t)nsteps * W);
-  std::vector<int> act((size_t)nsteps * W);
-  std::vector<char> any_dm(nsteps, 0), any_sw(nsteps, 0);
+  auto& coefs = r.coefs;
+  auto& act = r.act;
+  auto& any_dm = r.any_dm;
+  auto& any_sw = r.any_sw;
+  coefs.assign((size_t)nsteps * W, Coef{});
+  act.assign((size_t)nsteps * W, 0);
+  any_dm.assign(nsteps, 0);
+  any_sw.assign(nsteps, 0);
   for (int k = 0; k < nsteps; ++k) {
     for (int f = 0; f < W; ++f) {
       const spk_step& s = steps[(size_t)k * W + f];
@@ -817,9 +899,8 @@ int spk_iterate(spk_plan* p, const spk_step* steps, int nsteps, int per_frame,
   }
   bool dm_any = false;
   for (int k = 0; k < nsteps; ++k) dm_any |= any_dm[k] != 0;
-  SPK_REQUIRE(!(dm_any && (p->ipsf_on || p->spec_on)), SPK_EUNSUP,
-              "difference map with a partial-coherence ipsf or spectrum is not built");
   if (dm_any && (rc = ensure_phib(p))) return rc;
+  if (dm_any && (p->ipsf_on || p->spec_on) && (rc = ensure_dm_pc(p))) return rc;
   bool sw_any = false;
   for (int k = 0; k < nsteps; ++k) sw_any |= any_sw[k] != 0;
   if (sw_any) {
@@ -833,17 +914,29 @@ int spk_iterate(spk_plan* p, const spk_step* steps, int nsteps, int per_frame,
   // Frame activity [step][frame] is staged in both modes: the error trace of
   // an idle frame is NaN.
   Stage stg;
-  std::vector<size_t> map_off(nsteps), sw_off(nsteps), swp_off(nsteps);
-  std::vector<int> map_n(nsteps, B), sw_n(nsteps, 0), sw_r(nsteps, 0);
-  std::vector<float> dm_g(nsteps, 0.0f);
-  std::vector<SwParam> sw_first(nsteps);
-  std::vector<std::vector<SwParam>> sw_prm(per_frame ? nsteps : 0);
-  size_t coef_off = 0, act_off = 0;
+  auto& map_off = r.map_off;
+  auto& sw_off = r.sw_off;
+  auto& swp_off = r.swp_off;
+  auto& map_n = r.map_n;
+  auto& sw_n = r.sw_n;
+  auto& sw_r = r.sw_r;
+  auto& dm_g = r.dm_g;
+  auto& sw_first = r.sw_first;
+  auto& sw_prm = r.sw_prm;
+  map_off.assign(nsteps, 0);
+  sw_off.assign(nsteps, 0);
+  swp_off.assign(nsteps, 0);
+  map_n.assign(nsteps, B);
+  sw_n.assign(nsteps, 0);
+  sw_r.assign(nsteps, 0);
+  dm_g.assign(nsteps, 0.0f);
+  sw_first.assign(nsteps, SwParam{});
+  sw_prm.assign(per_frame ? nsteps : 0, {});
   if (err_trace) {
     std::vector<int> act_all((size_t)nsteps * B);
     for (int k = 0; k < nsteps; ++k)
       for (int f = 0; f < B; ++f) act_all[(size_t)k * B + f] = act[(size_t)k * W + (per_frame ? f : 0)];
-    act_off = stg.add(act_all.data(), act_all.size());
+    r.act_off = stg.add(act_all.data(), act_all.size());
   }
   // DM steps: one beta per step (the dm1 map of the refresh kernel is shared)
   for (int k = 0; k < nsteps; ++k) {
@@ -901,10 +994,29 @@ int spk_iterate(spk_plan* p, const spk_step* steps, int nsteps, int per_frame,
   // otherwise the refresh kernel recomputes it for exactly those frames
   // before the step (frame list dm_off[k], dm_n[k] entries; dm_all: every
   // frame, no list).  fr tracks the state through the call.
-  std::vector<size_t> dm_off(nsteps, 0);
-  std::vector<int> dm_n(nsteps, 0);
-  std::vector<char> dm_all(nsteps, 0);
-  std::vector<float> fr(p->phib_g);
+  auto& dm_off = r.dm_off;
+  auto& dm_n = r.dm_n;
+  auto& dm_all = r.dm_all;
+  auto& fr = r.fr;
+  dm_off.assign(nsteps, 0);
+  dm_n.assign(nsteps, 0);
+  dm_all.assign(nsteps, 0);
+  fr = p->phib_g;
+  // every DM frame of a DM step (per-frame tables): the frames of the second
+  // partially coherent projection's row-direction kernels
+  r.dmall_off.assign(nsteps, 0);
+  r.dmall_n.assign(nsteps, 0);
+  if (per_frame && (p->ipsf_on || p->spec_on)) {
+    std::vector<int> lst;
+    for (int k = 0; k < nsteps; ++k) {
+      if (!any_dm[k]) continue;
+      lst.clear();
+      for (int f = 0; f < B; ++f)
+        if (steps[(size_t)k * W + f].alg == SPK_DM) lst.push_back(f);
+      r.dmall_n[k] = (int)lst.size();
+      r.dmall_off[k] = stg.add(lst.data(), lst.size());
+    }
+  }
   {
     std::vector<int> lst;
     for (int k = 0; k < nsteps; ++k) {
@@ -925,20 +1037,87 @@ int spk_iterate(spk_plan* p, const spk_step* steps, int nsteps, int per_frame,
       }
     }
   }
-  std::vector<char> emits(nsteps, 0);
+  auto& emits = r.emits;
+  emits.assign(nsteps, 0);
   for (int k = 0; k < nsteps; ++k)
     for (int f = 0; f < W; ++f) emits[k] |= (coefs[(size_t)k * W + f].flags & kEmitB) != 0;
+  // Resident blocks (k_resident, one frame): runs of steps with no DM work
+  // (no DM step, no emission).  A shrinkwrap after a step of the run runs
+  // inside the block (row blur in that step's row phase, column blur in the
+  // next active step's column phase, threshold in its row phase) when its
+  // blur radius is within the fused window and the run goes on; otherwise
+  // the run ends at that step and the shrinkwrap runs as stand-alone kernels
+  // after the block.  Idle steps change nothing and are left out of the
+  // block's table.  res_end[k0] = k1 for a block of steps [k0, k1).
+  auto& res_end = r.res_end;
+  auto& in_res = r.in_res;
+  res_end.assign(nsteps, 0);
+  r.res_n.assign(nsteps, 0);
+  r.res_fused.assign(nsteps, 0);
+  r.res_off.assign(nsteps, 0);
+  r.res_sw_off.assign(nsteps, 0);
+  in_res.assign(nsteps, 0);
+  if (blocks && p->resident && B == 1 && !p->ipsf_on && !p->spec_on && !p->persist &&
+      !p->capturing) {
+    auto fits = [&](int j) { return !any_dm[j] && !emits[j]; };
+    std::vector<RStep> rs;
+    std::vector<SwParam> sp;
+    for (int k = 0; k < nsteps;) {
+      if (!fits(k)) { ++k; continue; }
+      rs.clear();
+      sp.clear();
+      int k1 = k, fused = 0, pending = -1;   // pending: sw index for the next active step
+      while (k1 < nsteps && fits(k1)) {
+        const int j = k1++;
+        if (!act[(size_t)j * W]) continue;
+        RStep rsj{coefs[(size_t)j * W], err_trace ? 1 : 0, j, -1, -1};
+        rsj.c.flags &= ~(kSwRow | kSwFin);
+        if (pending >= 0) {
+          rsj.c.flags |= kSwFin;
+          rsj.swf = pending;
+          pending = -1;
+        }
+        if (any_sw[j]) {
+          // fused when a later active step of the run takes its second half
+          int nx = j + 1;
+          while (nx < nsteps && fits(nx) && !act[(size_t)nx * W]) ++nx;
+          const bool goes_on = nx < nsteps && fits(nx);
+          if (!goes_on || sw_r[j] > kSwFuseRmax || !p->sw_fuse) {
+            rs.push_back(rsj);
+            break;                        // stand-alone shrinkwrap after the block
+          }
+          rsj.c.flags |= kSwRow;
+          rsj.swr = (int)sp.size();
+          pending = rsj.swr;
+          sp.push_back(per_frame ? sw_prm[j][0] : sw_first[j]);   // the one frame's
+          ++fused;
+        }
+        rs.push_back(rsj);
+      }
+      if (rs.size() >= 2) {
+        res_end[k] = k1;
+        r.res_n[k] = (int)rs.size();
+        r.res_fused[k] = fused;
+        r.res_off[k] = stg.add(rs.data(), rs.size());
+        if (!sp.empty()) r.res_sw_off[k] = stg.add(sp.data(), sp.size());
+        for (int j = k; j < k1; ++j) in_res[j] = 1;
+      }
+      k = k1;
+    }
+  }
   // Fused shrinkwraps: the shrinkwrap after step k runs inside the passes of
   // steps k and k+1 (row blur in row pass k, column blur in column pass k+1,
   // threshold in row pass k+1, the only reader of the support in between)
   // when both steps are single projections without DM, every frame with the
   // shrinkwrap is active at k+1 and the blur radius is within the fused
   // form's window.  Parameter tables by frame (per_frame) or one entry.
-  std::vector<char> sw_fuse(nsteps, 0);
-  std::vector<size_t> swt_off(nsteps, 0);
+  auto& sw_fuse = r.sw_fuse;
+  sw_fuse.assign(nsteps, 0);
+  r.swt_off.assign(nsteps, 0);
   if (p->sw_fuse && !p->persist && !p->ipsf_on && !p->spec_on) {
     for (int k = 0; k + 1 < nsteps; ++k) {
-      if (!any_sw[k] || sw_r[k] > kSwFuseRmax || any_dm[k] || any_dm[k + 1] || emits[k + 1])
+      if (!any_sw[k] || sw_r[k] > kSwFuseRmax || any_dm[k] || any_dm[k + 1] || emits[k + 1] ||
+          in_res[k] || in_res[k + 1])
         continue;
       const size_t row = (size_t)k * W, nrow = (size_t)(k + 1) * W;
       std::vector<SwParam> tab(per_frame ? B : 1);
@@ -965,15 +1144,16 @@ int spk_iterate(spk_plan* p, const spk_step* steps, int nsteps, int per_frame,
         coefs[nrow].flags |= kSwFin;
       }
       sw_fuse[k] = 1;
-      swt_off[k] = stg.add(tab.data(), tab.size());
+      r.swt_off[k] = stg.add(tab.data(), tab.size());
     }
   }
-  if (per_frame) coef_off = stg.add(coefs.data(), coefs.size());
+  if (per_frame) r.coef_off = stg.add(coefs.data(), coefs.size());
   // Steps whose updates all ignore psi_out (ER with numpy semantics): the
   // column pass only forms the error partials (or is skipped without an
   // error trace) and the row pass skips the row IFFT of Phi.  Not with DM
   // work, a fused shrinkwrap or a partially coherent projection in the step.
-  std::vector<char> noout(nsteps, 0);
+  auto& noout = r.noout;
+  noout.assign(nsteps, 0);
   if (!p->ipsf_on && !p->spec_on) {
     for (int k = 0; k < nsteps; ++k) {
       if (any_dm[k] || emits[k] || sw_fuse[k] || (k > 0 && sw_fuse[k - 1])) continue;
@@ -990,9 +1170,9 @@ int spk_iterate(spk_plan* p, const spk_step* steps, int nsteps, int per_frame,
   // Persistent blocks: runs of at least two steps with no DM step and no DM
   // successor, each run ending at a step with a shrinkwrap (which follows the
   // block).  block_end[k0] = k1 for a block of steps [k0, k1).
-  std::vector<int> block_end(nsteps, 0);
-  std::vector<size_t> block_off(nsteps, 0);
-  if (p->persist && !p->ipsf_on && !p->spec_on) {
+  r.block_end.assign(nsteps, 0);
+  r.block_off.assign(nsteps, 0);
+  if (blocks && p->persist && !p->ipsf_on && !p->spec_on) {
     auto fits = [&](int j) { return !any_dm[j] && !emits[j]; };
     for (int k = 0; k < nsteps;) {
       if (!fits(k)) { ++k; continue; }
@@ -1006,131 +1186,373 @@ int spk_iterate(spk_plan* p, const spk_step* steps, int nsteps, int per_frame,
         for (int j = k; j < k1; ++j) {
           PStep& q = ps[j - k];
           q.c = coefs[(size_t)j * W];
-          q.tab_off = per_frame ? (long long)(coef_off + (size_t)j * W * sizeof(Coef)) : -1;
+          q.tab_off = per_frame ? (long long)(r.coef_off + (size_t)j * W * sizeof(Coef)) : -1;
           q.fmap_off = per_frame ? (long long)map_off[j] : -1;
           q.nact = map_n[j];
           q.pad_ = 0;
         }
-        block_end[k] = k1;
-        block_off[k] = stg.add(ps.data(), ps.size());
+        r.block_end[k] = k1;
+        r.block_off[k] = stg.add(ps.data(), ps.size());
       }
       k = k1;
     }
   }
   if ((rc = upload_stage(p, stg))) return rc;
-  unsigned char* dbase = p->stage_cur;
-  auto dptr = [&](size_t off) { return dbase + off; };
+  r.dbase = p->stage_cur;
 
   // error partials [step][frame][band]
   if (err_trace) {
     const size_t need = (size_t)nsteps * B * p->bands;
     if (need > p->errp_cap) {
-      SPK_HIP(hipStreamSynchronize(st));
+      SPK_HIP(hipStreamSynchronize(p->ctx->stream));
       dfree(p->errp);
       p->errp_cap = 0;
       if ((rc = dalloc(&p->errp, need))) return rc;
       p->errp_cap = need;
     }
   }
-
   // until every launch is queued the DM estimates count as stale (a failed
   // call leaves them to be recomputed)
   std::fill(p->phib_g.begin(), p->phib_g.end(), 0.0f);
+  return SPK_OK;
+}
+
+int iter_shrinkwrap_after(IterRun& r, int k) {
+  spk_plan* p = r.p;
+  hipStream_t st = p->ctx->stream;
+  if (r.per_frame) {
+    SPK_HIP(launch_shrinkwrap(p->n, p->tile, p->psi, p->sup, p->mstride(), p->sw_tmp, p->sw_blur,
+                              p->sw_maxp, (const SwParam*)r.dptr(r.swp_off[k]), SwParam{},
+                              (const int*)r.dptr(r.sw_off[k]), 0, r.sw_n[k], r.sw_r[k], st));
+  } else {
+    SPK_HIP(launch_shrinkwrap(p->n, p->tile, p->psi, p->sup, p->mstride(), p->sw_tmp, p->sw_blur,
+                              p->sw_maxp, nullptr, r.sw_first[k], nullptr, 0, r.B, r.sw_r[k], st));
+  }
+  p->sw_separate_count++;
+  return SPK_OK;
+}
+
+// The pass arguments of step k (tables, frame list, error partials, fused
+// shrinkwrap tables) and its fused-shrinkwrap bits: 1 = this step's row blur,
+// 2 = the previous step's column blur and threshold.
+PassArgs iter_step_args(const IterRun& r, int k, int* swbits) {
+  spk_plan* p = r.p;
   PassArgs a = base_args(p);
-  auto shrinkwrap_after = [&](int k) -> int {
-    if (per_frame) {
-      SPK_HIP(launch_shrinkwrap(p->n, p->tile, p->psi, p->sup, p->mstride(), p->sw_tmp,
-                                p->sw_blur, p->sw_maxp, (const SwParam*)dptr(swp_off[k]),
-                                SwParam{}, (const int*)dptr(sw_off[k]), 0, sw_n[k], sw_r[k],
-                                st));
-    } else {
-      SPK_HIP(launch_shrinkwrap(p->n, p->tile, p->psi, p->sup, p->mstride(), p->sw_tmp,
-                                p->sw_blur, p->sw_maxp, nullptr, sw_first[k], nullptr, 0, B,
-                                sw_r[k], st));
-    }
-    p->sw_separate_count++;
+  const size_t row = (size_t)k * r.W;
+  if (r.per_frame) {
+    a.tab = (const Coef*)r.dptr(r.coef_off) + row;
+    a.fmap = (const int*)r.dptr(r.map_off[k]);
+  } else {
+    a.tab = nullptr;
+    a.fmap = nullptr;
+    a.c = r.coefs[row];
+  }
+  a.errp = r.err_trace ? p->errp + (size_t)k * r.B * p->bands : nullptr;
+  const int sb = (r.sw_fuse[k] ? 1 : 0) | (k > 0 && r.sw_fuse[k - 1] ? 2 : 0);
+  // streaming row-pass stores for one 1024^2 frame in 4- or 8-wide tiles
+  // (whole 128-byte lines per wavefront; measured -3 % per iteration there,
+  // +1..5 % at 256^2 and 512^2)
+  a.nt_store = (r.map_n[k] == 1 && p->n == 1024 && p->tile >= 4) ? 1 : 0;
+  a.swr_tab = (sb & 1) ? (const SwParam*)r.dptr(r.swt_off[k]) : nullptr;
+  a.swf_tab = (sb & 2) ? (const SwParam*)r.dptr(r.swt_off[k - 1]) : nullptr;
+  a.sw_pf = r.per_frame ? 1 : 0;
+  *swbits = sb;
+  return a;
+}
+
+// after the row pass of step k: the stand-alone shrinkwrap when it did not
+// run fused, and the counters
+int iter_step_after(IterRun& r, int k) {
+  if (r.sw_fuse[k]) r.p->sw_fused_count++;
+  if (r.any_sw[k] && !r.sw_fuse[k]) return iter_shrinkwrap_after(r, k);
+  return SPK_OK;
+}
+
+// Queue step k (or the block that starts there); *next = the step after it.
+int iter_step(IterRun& r, int k, int* next) {
+  spk_plan* p = r.p;
+  hipStream_t st = p->ctx->stream;
+  const int B = r.B;
+  int rc = SPK_OK;
+  *next = k + 1;
+  if (r.res_end[k]) {
+    const int k1 = r.res_end[k];
+    if (!p->rsync && (rc = dalloc(&p->rsync, (size_t)p->bands * 32))) return rc;
+    if ((rc = prof_arm(p, 2))) return rc;
+    PassArgs ra = base_args(p);
+    ra.swr_tab = r.res_fused[k] ? (const SwParam*)r.dptr(r.res_sw_off[k]) : nullptr;
+    SPK_HIP(launch_resident(p->n, p->tile, p->ept, p->xm, ra, (const RStep*)r.dptr(r.res_off[k]),
+                            r.res_n[k], r.err_trace ? p->errp : nullptr, (long long)B * p->bands,
+                            p->rsync, (int*)(p->pbar + 1), st));
+    p->res_launches++;
+    p->res_steps += r.res_n[k];
+    p->sw_fused_count += r.res_fused[k];
+    if (r.any_sw[k1 - 1] && (rc = iter_shrinkwrap_after(r, k1 - 1))) return rc;
+    *next = k1;
     return SPK_OK;
-  };
+  }
+  if (r.block_end[k]) {
+    const int k1 = r.block_end[k];
+    if (!p->pgrid) SPK_HIP(persist_grid(p->n, p->tile, p->ept, &p->pgrid));
+    SPK_REQUIRE(p->pgrid > 0, SPK_EUNSUP, "persistent kernel does not fit on this device");
+    const int grid = (int)std::min<long long>(p->pgrid, (long long)B * p->bands);
+    PassArgs pa = base_args(p);
+    SPK_HIP(launch_persist(p->n, p->tile, p->ept, pa, r.dbase,
+                           (const PStep*)r.dptr(r.block_off[k]), k1 - k,
+                           r.err_trace ? p->errp + (size_t)k * B * p->bands : nullptr,
+                           (long long)B * p->bands, p->pbar, (int*)(p->pbar + 1), grid, st));
+    if (r.any_sw[k1 - 1] && (rc = iter_shrinkwrap_after(r, k1 - 1))) return rc;
+    *next = k1;
+    return SPK_OK;
+  }
+  if (r.map_n[k] == 0) return SPK_OK;     // every frame idle this step
+  int swbits = 0;
+  PassArgs a = iter_step_args(r, k, &swbits);
+  const int nact = r.map_n[k];
+  if (r.any_dm[k] && r.dm_n[k]) {
+    // phib = rowFFT((s - gamma_m (1-s)) psi), dm1 (phasing_dm1.cl:16), for
+    // the DM frames of the step whose phib is not fresh
+    SPK_HIP(launch_rowfft(p->n, p->tile, p->ept, 2, p->psi, p->phib_in(), p->sup, p->mstride(),
+                          r.dm_g[k], p->tw,
+                          r.dm_all[k] ? nullptr : (const int*)r.dptr(r.dm_off[k]), r.dm_n[k], st));
+  }
+  if (p->ipsf_on || p->spec_on) {
+    // partially coherent projection.  ipsf: column pass A (|Psi|^2 ->
+    // column FFT) and the row convolution with the ipsf.  spectrum: the
+    // squared divisor as a map (coherent: MODE 6; after the ipsf: MODE 7),
+    // its spectral blur.  Then column pass B (divisor, replacement,
+    // error, column IFFT).
+    // With DM the same runs on phib (grid z = 1, its own work buffers) for
+    // the DM frames: t = P_M[dm1 psi] with the divisor of dm1 psi
+    // (phasing.py:1293-1313 calls _fourier_constraint on it).
+    const int zd = r.any_dm[k] ? 2 : 1;
+    const int* dmmap = r.per_frame ? (const int*)r.dptr(r.dmall_off[k]) : nullptr;
+    const int ndm = r.per_frame ? r.dmall_n[k] : B;
+    if (p->ipsf_on) {
+      SPK_HIP(launch_colpass(p->n, p->tile, p->ept, 4, a, nact, zd, st));
+      SPK_HIP(launch_rowconv(p->n, p->tile, p->ept, p->dbuf, p->dual() ? p->dbufc : p->dbuf,
+                             p->ipsf, p->tw, a.fmap, nact, st));
+      if (zd == 2)
+        SPK_HIP(launch_rowconv(p->n, p->tile, p->ept, p->dbuf2, p->dual() ? p->dbufc2 : p->dbuf2,
+                               p->ipsf, p->tw, dmmap, ndm, st));
+    }
+    if (p->spec_on) {
+      SPK_HIP(launch_colpass(p->n, p->tile, p->ept, p->ipsf_on ? 7 : 6, a, nact, zd, st));
+      SPK_HIP(launch_rescale(p->n, p->tile, p->imap, p->dmap, p->spec, p->spec_k, a.fmap, nact,
+                             st));
+      if (zd == 2)
+        SPK_HIP(launch_rescale(p->n, p->tile, p->imap2, p->dmap2, p->spec, p->spec_k, dmmap, ndm,
+                               st));
+    }
+    if ((rc = prof_arm(p, 0))) return rc;
+    SPK_HIP(launch_colpass(p->n, p->tile, p->ept, p->spec_on ? 8 : 5, a, nact, zd, st));
+  } else if (r.noout[k]) {
+    // not profiled: the pass timings cover the full passes
+    if (a.errp) SPK_HIP(launch_colpass(p->n, p->tile, p->ept, 10, a, nact, 1, st));
+  } else {
+    if ((rc = prof_arm(p, 0))) return rc;
+    SPK_HIP(launch_colpass(p->n, p->tile, p->ept, (swbits & 2) ? 9 : 0, a, nact,
+                           r.any_dm[k] ? 2 : 1, st));
+  }
+  a.errp = nullptr;
+  if (!r.noout[k] && (rc = prof_arm(p, 1))) return rc;
+  SPK_HIP(launch_rowpass(p->n, p->tile, p->ept, r.any_dm[k] != 0, r.emits[k] != 0, swbits,
+                         r.noout[k] != 0, a, nact, st));
+  return iter_step_after(r, k);
+}
+
+int iter_finish(IterRun& r) {
+  spk_plan* p = r.p;
+  if (r.err_trace) {
+    SPK_HIP(launch_err_finalize(p->errp, r.nsteps, r.B, p->bands, p->summ2, r.err_trace,
+                                (const int*)r.dptr(r.act_off), p->ctx->stream));
+  }
+  p->phib_g = r.fr;
+  return SPK_OK;
+}
+
+}  // namespace
+
+int spk_iterate(spk_plan* p, const spk_step* steps, int nsteps, int per_frame,
+                double* err_trace) {
+  SPK_REQUIRE(p, SPK_EINVAL, "null plan");
+  int rc = set_device(p->ctx);
+  if (rc) return rc;
+  SPK_REQUIRE(nsteps >= 0, SPK_EINVAL, "nsteps must be >= 0");
+  if (nsteps == 0) return SPK_OK;
+  SPK_REQUIRE(steps, SPK_EINVAL, "null steps");
+  IterRun r;
+  if ((rc = iter_prepare(p, steps, nsteps, per_frame, err_trace, true, r))) return rc;
   for (int k = 0; k < nsteps;) {
-    if (block_end[k]) {
-      const int k1 = block_end[k];
-      if (!p->pgrid) SPK_HIP(persist_grid(p->n, p->tile, p->ept, &p->pgrid));
-      SPK_REQUIRE(p->pgrid > 0, SPK_EUNSUP, "persistent kernel does not fit on this device");
-      const int grid = (int)std::min<long long>(p->pgrid, (long long)B * p->bands);
-      PassArgs pa = base_args(p);
-      SPK_HIP(launch_persist(p->n, p->tile, p->ept, pa, dbase, (const PStep*)dptr(block_off[k]),
-                             k1 - k, err_trace ? p->errp + (size_t)k * B * p->bands : nullptr,
-                             (long long)B * p->bands, p->pbar, (int*)(p->pbar + 1), grid, st));
-      if (any_sw[k1 - 1] && (rc = shrinkwrap_after(k1 - 1))) return rc;
-      k = k1;
-      continue;
-    }
-    const size_t row = (size_t)k * W;
-    if (map_n[k] == 0) { ++k; continue; }     // every frame idle this step
-    if (per_frame) {
-      a.tab = (const Coef*)dptr(coef_off) + row;
-      a.fmap = (const int*)dptr(map_off[k]);
-    } else {
-      a.tab = nullptr;
-      a.fmap = nullptr;
-      a.c = coefs[row];
-    }
-    const int nact = map_n[k];
-    if (any_dm[k] && dm_n[k]) {
-      // phib = rowFFT((s - gamma_m (1-s)) psi), dm1 (phasing_dm1.cl:16), for
-      // the DM frames of the step whose phib is not fresh
-      SPK_HIP(launch_rowfft(p->n, p->tile, p->ept, 2, p->psi, p->phib_in(), p->sup, p->mstride(),
-                            dm_g[k], p->tw, dm_all[k] ? nullptr : (const int*)dptr(dm_off[k]),
-                            dm_n[k], st));
-    }
-    a.errp = err_trace ? p->errp + (size_t)k * B * p->bands : nullptr;
-    const int swbits = (sw_fuse[k] ? 1 : 0) | (k > 0 && sw_fuse[k - 1] ? 2 : 0);
-    // streaming row-pass stores for one 1024^2 frame in 4- or 8-wide tiles
-    // (whole 128-byte lines per wavefront; measured -3 % per iteration there,
-    // +1..5 % at 256^2 and 512^2)
-    a.nt_store = (nact == 1 && p->n == 1024 && p->tile >= 4) ? 1 : 0;
-    a.swr_tab = (swbits & 1) ? (const SwParam*)dptr(swt_off[k]) : nullptr;
-    a.swf_tab = (swbits & 2) ? (const SwParam*)dptr(swt_off[k - 1]) : nullptr;
-    a.sw_pf = per_frame ? 1 : 0;
-    if (p->ipsf_on || p->spec_on) {
-      // partially coherent projection.  ipsf: column pass A (|Psi|^2 ->
-      // column FFT) and the row convolution with the ipsf.  spectrum: the
-      // squared divisor as a map (coherent: MODE 6; after the ipsf: MODE 7),
-      // its spectral blur.  Then column pass B (divisor, replacement,
-      // error, column IFFT).
-      if (p->ipsf_on) {
-        SPK_HIP(launch_colpass(p->n, p->tile, p->ept, 4, a, nact, 1, st));
-        SPK_HIP(launch_rowconv(p->n, p->tile, p->ept, p->dbuf, p->dual() ? p->dbufc : p->dbuf,
-                               p->ipsf, p->tw, a.fmap, nact, st));
-      }
-      if (p->spec_on) {
-        SPK_HIP(launch_colpass(p->n, p->tile, p->ept, p->ipsf_on ? 7 : 6, a, nact, 1, st));
-        SPK_HIP(launch_rescale(p->n, p->tile, p->imap, p->dmap, p->spec, p->spec_k, a.fmap,
-                               nact, st));
-      }
-      if ((rc = prof_arm(p, 0))) return rc;
-      SPK_HIP(launch_colpass(p->n, p->tile, p->ept, p->spec_on ? 8 : 5, a, nact, 1, st));
-    } else if (noout[k]) {
-      // not profiled: the pass timings cover the full passes
-      if (a.errp) SPK_HIP(launch_colpass(p->n, p->tile, p->ept, 10, a, nact, 1, st));
-    } else {
-      if ((rc = prof_arm(p, 0))) return rc;
-      SPK_HIP(launch_colpass(p->n, p->tile, p->ept, (swbits & 2) ? 9 : 0, a, nact,
-                             any_dm[k] ? 2 : 1, st));
-    }
-    a.errp = nullptr;
-    if (!noout[k] && (rc = prof_arm(p, 1))) return rc;
-    SPK_HIP(launch_rowpass(p->n, p->tile, p->ept, any_dm[k] != 0, emits[k] != 0, swbits,
-                           noout[k] != 0, a, nact, st));
-    if (sw_fuse[k]) p->sw_fused_count++;
-    if (any_sw[k] && !sw_fuse[k] && (rc = shrinkwrap_after(k))) return rc;
-    ++k;
+    int next = k + 1;
+    if ((rc = iter_step(r, k, &next))) return rc;
+    k = next;
   }
-  if (err_trace) {
-    SPK_HIP(launch_err_finalize(p->errp, nsteps, B, p->bands, p->summ2, err_trace,
-                                (const int*)dptr(act_off), st));
+  return iter_finish(r);
+}
+
+// ---------------------------------------------------------------------------
+// Mixed-size iteration (spk_group_*): the member plans' spk_iterate calls
+// with their single-projection passes merged, per step, into one column-pass
+// and one row-pass launch for all members.
+// ---------------------------------------------------------------------------
+struct spk_group {
+  spk_ctx* ctx = nullptr;
+  unsigned char* tab_d = nullptr;   // per-call pass-argument tables
+  unsigned char* tab_h = nullptr;
+  size_t cap = 0;
+  hipEvent_t upload_done = nullptr;
+  bool upload_pending = false;
+  long long launches = 0, member_passes = 0;
+};
+
+int spk_group_shape(int n, int* tile, int* ept) {
+  SPK_REQUIRE(tile && ept, SPK_EINVAL, "null argument");
+  SPK_REQUIRE(group_tile(n) > 0, SPK_EUNSUP,
+              "mixed-size launches take frames of 256, 512 and 1024 (got " + std::to_string(n) +
+                  ")");
+  *tile = group_tile(n);
+  *ept = kGroupEpt;
+  return SPK_OK;
+}
+
+int spk_group_create(spk_ctx* ctx, spk_group** out) {
+  SPK_REQUIRE(ctx && out, SPK_EINVAL, "null argument");
+  int rc = set_device(ctx);
+  if (rc) return rc;
+  spk_group* g = new spk_group;
+  g->ctx = ctx;
+  if (hipEventCreateWithFlags(&g->upload_done, hipEventDisableTiming) != hipSuccess) {
+    delete g;
+    return fail(SPK_EHIP, "hipEventCreate failed");
   }
-  p->phib_g = fr;
+  ctx->refs++;
+  *out = g;
+  return SPK_OK;
+}
+
+int spk_group_destroy(spk_group* g) {
+  if (!g) return SPK_OK;
+  (void)hipSetDevice(g->ctx->device);
+  (void)hipStreamSynchronize(g->ctx->stream);
+  dfree(g->tab_d);
+  if (g->tab_h) (void)hipHostFree(g->tab_h);
+  if (g->upload_done) (void)hipEventDestroy(g->upload_done);
+  ctx_release(g->ctx);
+  delete g;
+  return SPK_OK;
+}
+
+int spk_group_counts(const spk_group* g, long long* launches, long long* member_passes) {
+  SPK_REQUIRE(g && launches && member_passes, SPK_EINVAL, "null argument");
+  *launches = g->launches;
+  *member_passes = g->member_passes;
+  return SPK_OK;
+}
+
+int spk_group_iterate(spk_group* g, spk_plan* const* plans, int count,
+                      const spk_step* const* steps, int nsteps, const int* per_frame,
+                      double* const* err_trace) {
+  SPK_REQUIRE(g, SPK_EINVAL, "null group");
+  int rc = set_device(g->ctx);
+  if (rc) return rc;
+  SPK_REQUIRE(count >= 1 && count <= kGroupMax, SPK_EINVAL,
+              "a mixed-size call takes 1 to " + std::to_string(kGroupMax) + " plans");
+  SPK_REQUIRE(plans && steps, SPK_EINVAL, "null plans or steps");
+  SPK_REQUIRE(nsteps >= 0, SPK_EINVAL, "nsteps must be >= 0");
+  for (int m = 0; m < count; ++m) {
+    const spk_plan* p = plans[m];
+    SPK_REQUIRE(p && steps[m], SPK_EINVAL, "null plan or step table");
+    SPK_REQUIRE(p->ctx == g->ctx, SPK_EINVAL, "every plan of a mixed-size call uses the group's context");
+    SPK_REQUIRE(p->tile == group_tile(p->n) && p->ept == kGroupEpt, SPK_EUNSUP,
+                "plan of size " + std::to_string(p->n) +
+                    " is not in the mixed-size shape (spk_group_shape)");
+    SPK_REQUIRE(!p->persist && !p->capturing, SPK_EUNSUP,
+                "mixed-size calls take neither persistent plans nor graph capture");
+    for (int j = 0; j < m; ++j) SPK_REQUIRE(plans[j] != p, SPK_EINVAL, "a plan appears twice");
+  }
+  if (nsteps == 0) return SPK_OK;
+  // every member's table is checked and staged before the first launch
+  std::vector<IterRun> runs(count);
+  for (int m = 0; m < count; ++m)
+    if ((rc = iter_prepare(plans[m], steps[m], nsteps, per_frame ? per_frame[m] : 0,
+                           err_trace ? err_trace[m] : nullptr, false, runs[m])))
+      return rc;
+  auto groupable = [&](int m, int k) {
+    const IterRun& r = runs[m];
+    return r.map_n[k] > 0 && !r.any_dm[k] && !r.emits[k] && !r.p->ipsf_on && !r.p->spec_on;
+  };
+  // the pass arguments of every merged (step, member), column then row
+  // copies (the row pass writes no error partials), in one upload
+  std::vector<PassArgs> tab;
+  std::vector<GroupPass> col(nsteps), row(nsteps);
+  std::vector<int> colmode(nsteps, 0), swb(nsteps, 0);
+  std::vector<size_t> base(nsteps, 0);
+  for (int k = 0; k < nsteps; ++k) {
+    GroupPass& gc = col[k];
+    gc.count = 0;
+    gc.start[0] = 0;
+    base[k] = tab.size();
+    std::vector<PassArgs> rows;
+    for (int m = 0; m < count; ++m) {
+      if (!groupable(m, k)) continue;
+      int sb = 0;
+      PassArgs a = iter_step_args(runs[m], k, &sb);
+      a.nt_store = 0;
+      swb[k] |= sb;
+      if (sb & 2) colmode[k] = 9;
+      const int j = gc.count++;
+      gc.n[j] = runs[m].p->n;
+      gc.start[j + 1] = gc.start[j] + runs[m].map_n[k] * runs[m].p->bands;
+      tab.push_back(a);
+      a.errp = nullptr;
+      rows.push_back(a);
+    }
+    tab.insert(tab.end(), rows.begin(), rows.end());
+    row[k] = gc;
+  }
+  if (!tab.empty()) {
+    const size_t need = tab.size() * sizeof(PassArgs);
+    if (g->upload_pending) SPK_HIP(hipEventSynchronize(g->upload_done));
+    g->upload_pending = false;
+    if (need > g->cap) {
+      SPK_HIP(hipStreamSynchronize(g->ctx->stream));
+      dfree(g->tab_d);
+      if (g->tab_h) (void)hipHostFree(g->tab_h);
+      g->tab_h = nullptr;
+      g->cap = 0;
+      const size_t cap = std::max<size_t>(need, 64 * 1024);
+      if ((rc = dalloc(&g->tab_d, cap))) return rc;
+      SPK_HIP(hipHostMalloc((void**)&g->tab_h, cap, hipHostMallocDefault));
+      g->cap = cap;
+    }
+    std::memcpy(g->tab_h, tab.data(), need);
+    SPK_HIP(hipMemcpyAsync(g->tab_d, g->tab_h, need, hipMemcpyHostToDevice, g->ctx->stream));
+    SPK_HIP(hipEventRecord(g->upload_done, g->ctx->stream));
+    g->upload_pending = true;
+  }
+  const PassArgs* dtab = reinterpret_cast<const PassArgs*>(g->tab_d);
+  hipStream_t st = g->ctx->stream;
+  for (int k = 0; k < nsteps; ++k) {
+    // members whose step does not merge (DM work, a partially coherent
+    // projection) run it with their own launches; an idle step runs nothing
+    for (int m = 0; m < count; ++m) {
+      if (groupable(m, k)) continue;
+      int next = k + 1;
+      if ((rc = iter_step(runs[m], k, &next))) return rc;
+    }
+    if (col[k].count == 0) continue;
+    col[k].args = dtab + base[k];
+    row[k].args = dtab + base[k] + col[k].count;
+    SPK_HIP(launch_colpass_group(colmode[k], col[k], st));
+    SPK_HIP(launch_rowpass_group(swb[k], row[k], st));
+    g->launches += 2;
+    g->member_passes += 2 * col[k].count;
+    for (int m = 0; m < count; ++m)
+      if (groupable(m, k) && (rc = iter_step_after(runs[m], k))) return rc;
+  }
+  for (int m = 0; m < count; ++m)
+    if ((rc = iter_finish(runs[m]))) return rc;
   return SPK_OK;
 }
 
@@ -1683,7 +2105,8 @@ int spk_plan_profile(spk_plan* p, int enable) {
 
 int spk_plan_profile_read(spk_plan* p, int pass, int* launches, double* total_ms) {
   SPK_REQUIRE(p && launches && total_ms, SPK_EINVAL, "null argument");
-  SPK_REQUIRE(pass == 0 || pass == 1, SPK_EINVAL, "pass must be 0 (column) or 1 (row)");
+  SPK_REQUIRE(pass >= 0 && pass <= 2, SPK_EINVAL,
+              "pass must be 0 (column), 1 (row) or 2 (resident launches)");
   int rc = set_device(p->ctx);
   if (rc) return rc;
   std::vector<hipEvent_t>& ev = p->prof_ev[pass];

@@ -74,6 +74,11 @@ struct PassArgs {
   // column-band-major like the modulus
   float* imap;
   float* dmap;
+  // the same work buffers for DM's second projection (of phib, grid z = 1)
+  c32* dbuf2;
+  c32* dbufc2;
+  float* imap2;
+  float* dmap2;
   // fused shrinkwrap (kSwRow / kSwFin): work maps, per-frame band maxima and
   // the blur/threshold parameters of the shrinkwrap (per frame, or one set)
   float* sw_tmp;
@@ -98,6 +103,35 @@ struct PStep {
 
 
 
+// One pass of a mixed-size launch (k_colpass_group / k_rowpass_group): the
+// frames of up to kGroupMax plans of different sizes in one launch.  Member
+// m's workgroups are [start[m], start[m+1]): its active frames (args[m].fmap,
+// or all) times its bands, frame-major.  Each size runs in the layout tile
+// that gives a 256-thread workgroup with eight values per thread
+// (group_tile), so that one workgroup shape serves every member.
+constexpr int kGroupMax = 16;
+constexpr int kGroupThreads = 256;
+struct GroupPass {
+  const PassArgs* args;        // per member, device memory
+  int count;
+  int start[kGroupMax + 1];
+  int n[kGroupMax];
+};
+// the layout tile of size n in a mixed-size launch (0: not a member size)
+constexpr int group_tile(int n) {
+  return n == 256 ? 8 : n == 512 ? 4 : n == 1024 ? 2 : 0;
+}
+constexpr int kGroupEpt = 8;
+
+// One step of a resident single-frame iteration block (k_resident).
+struct RStep {
+  Coef c;          // kActive, kNoOut, kSwRow, kSwFin; coefficients of the update
+  int err;         // write this step's error partials
+  int kidx;        // the step's index in the call (row of the error partials)
+  int swr;         // kSwRow: this step's shrinkwrap parameters (index into swr_tab)
+  int swf;         // kSwFin: the previous step's shrinkwrap parameters
+};
+
 // Kernel timing: when set, the next column-pass or row-pass launch on this
 // thread is made with hipExtLaunchKernelGGL and records these start/stop
 // events with the dispatch itself (the kernel's own execution interval, as a
@@ -117,6 +151,11 @@ hipError_t launch_colpass(int n, int tile, int ept, int mode, const PassArgs& a,
 // transform and error partials only)
 hipError_t launch_rowpass(int n, int tile, int ept, bool use_t, bool emit_b, int sw,
                           bool noout, const PassArgs& a, int frames, hipStream_t st);
+// mixed-size passes over the members of g (single projections, no DM):
+// column pass mode 0 or 9, row pass with fused-shrinkwrap bits sw; blocks =
+// g.start[g.count]
+hipError_t launch_colpass_group(int mode, const GroupPass& g, hipStream_t st);
+hipError_t launch_rowpass_group(int sw, const GroupPass& g, hipStream_t st);
 // persistent iteration block: one cooperative launch of `grid` workgroups runs
 // nsteps (column pass, grid barrier, row pass) steps; bar = barrier counter
 // (zeroed by the launcher), flag set to 1 if a barrier wait gave up
@@ -148,6 +187,17 @@ hipError_t launch_com_partial(const c32* data, int frames, int rows, int cols, d
                               int nblk, hipStream_t st);
 hipError_t launch_roll_sum(const c32* data, int frames, int rows, int cols, const int* shift,
                            c32* out, hipStream_t st);
+// resident single-frame iteration block: one cooperative launch of n/tile
+// workgroups runs nsteps steps with the frame's modulus, support and iterate
+// in registers (k_resident); xm: exchange mode of Phi (0 plain + fences,
+// 1 device-scope + fences, 2 device-scope, no fences); errp0[kidx *
+// errp_step + band] gets the error partials of the steps with err set;
+// flags: n/tile * 32 words of barrier flags (zeroed by the launcher); flag:
+// this launch's give-up flag, then the sticky one
+bool resident_supported(int n, int tile, int ept);
+hipError_t launch_resident(int n, int tile, int ept, int xm, const PassArgs& a,
+                           const RStep* steps, int nsteps, double* errp0, long long errp_step,
+                           unsigned* flags, int* flag, hipStream_t st);
 // workgroups of the persistent kernel that fit on the current device at once
 hipError_t persist_grid(int n, int tile, int ept, int* out);
 // mode 0: out = rowFFT(in); 1: out = rowIFFT(in) (unscaled);

@@ -307,6 +307,47 @@ __device__ __forceinline__ float sw_blur_lines(const float* line, const float (&
   return mx;
 }
 
+// sw_blur_lines with every tap and weight read from LDS instead of a
+// register window: the same terms summed in the same order (w[0] x[j], then
+// d = 1 .. 24 with zero weights past the radius), so the same results, with
+// few live registers (for the resident kernel, whose iterate, support and
+// modulus stay in registers).  wl: the kSwFuseRmax + 1 weights in LDS.
+template <int N, int TT, class OUT>
+__device__ __forceinline__ float sw_blur_lines_lds(const float* line, const float* wl, OUT out) {
+  constexpr int K = 8;
+  const int nt = blockDim.x;
+  float mx = -INFINITY;
+#pragma unroll 1
+  for (int seg = threadIdx.x; seg < N * TT / K; seg += nt) {
+    const int l = seg / (N / K), j0 = (seg % (N / K)) * K;
+    const float* ln = line + l * sw_line<N>();
+    float acc[K];
+    const float w0 = wl[0];
+#pragma unroll
+    for (int k = 0; k < K; ++k) acc[k] = w0 * ln[blur_slot((j0 + k) & (N - 1))];
+#pragma unroll 1
+    for (int d = 1; d <= kSwFuseRmax; ++d) {
+      const float wd = wl[d];
+#pragma unroll
+      for (int k = 0; k < K; ++k)
+        acc[k] = fmaf(wd, ln[blur_slot((j0 + k + d) & (N - 1))] + ln[blur_slot((j0 + k - d) & (N - 1))],
+                      acc[k]);
+    }
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      out(l, j0 + k, acc[k]);
+      mx = fmaxf(mx, acc[k]);
+    }
+  }
+  return mx;
+}
+
+// the fused window's weights of one shrinkwrap into LDS (zero past the radius)
+__device__ __forceinline__ void sw_weights_lds(const BlurW& bw, float* wl) {
+  if (threadIdx.x <= kSwFuseRmax)
+    wl[threadIdx.x] = (int)threadIdx.x <= bw.radius ? bw.w[threadIdx.x] : 0.0f;
+}
+
 // Modulus replacement m * Psi / |Psi| (phasing_fourier.cl:12-15,
 // phasing.py:1217) through one reciprocal square root of |Psi|^2 (v_rsq_f32,
 // about 1 ulp); 0 where |Psi|^2 is below the smallest normal float, i.e.
@@ -474,13 +515,15 @@ __device__ __forceinline__ void colpass_item(const PassArgs& a, int f, int band,
 
 // Partial coherence, column pass A (phasing.py:1175-1180 with ipsf):
 // Psi = colFFT(Phi); Q = colFFT(|Psi|^2) -> dbuf (tiled).  Phi is kept.
+// z = 1: the same for DM's estimate phib, into dbuf2.
 template <int N, int TT, int E>
-__device__ __forceinline__ void colpass_ipsf_a(const PassArgs& a, int f, int band, c32* lds) {
+__device__ __forceinline__ void colpass_ipsf_a(const PassArgs& a, int f, int band, int z,
+                                               c32* lds) {
   using L = LdsPlan<N, TT>;
   constexpr int NT = (N / E) * TT;
   constexpr bool DUAL = dual_layout<N>();
   const size_t fo = (size_t)f * N * N;
-  const c32* src = (DUAL ? a.phic : a.phi) + fo;
+  const c32* src = (z ? (DUAL ? a.phibc : a.phib) : (DUAL ? a.phic : a.phi)) + fo;
   const int tid = threadIdx.x, b = tid % TT, t = tid / TT;
   TwStage<N, TT, NT> tws;
   tws.fetch(a.tw);
@@ -496,7 +539,7 @@ __device__ __forceinline__ void colpass_ipsf_a(const PassArgs& a, int f, int ban
   for (int i = 0; i < E; ++i) x[i] = c32{fmaf(x[i].x, x[i].x, x[i].y * x[i].y), 0.0f};
   fft_regs<N, E, TT, false, false>(x, t, b, lds, tw);
 #pragma unroll
-  for (int i = 0; i < E; ++i) gst(a.dbuf + fo, col_pos<N, TT, E>(band, b, t, i), x[i]);
+  for (int i = 0; i < E; ++i) gst((z ? a.dbuf2 : a.dbuf) + fo, col_pos<N, TT, E>(band, b, t, i), x[i]);
 }
 
 // Intensity map for the spectral blur (MODE 6, 7): the squared divisor,
@@ -504,12 +547,15 @@ __device__ __forceinline__ void colpass_ipsf_a(const PassArgs& a, int f, int ban
 // coherent divisor squared); MODE 7: |colIFFT(D)| / n^2 with D from the ipsf
 // row convolution (the partially coherent divisor squared).
 template <int N, int TT, int E, int MODE>
-__device__ __forceinline__ void colpass_intensity(const PassArgs& a, int f, int band, c32* lds) {
+__device__ __forceinline__ void colpass_intensity(const PassArgs& a, int f, int band, int z,
+                                                  c32* lds) {
   using L = LdsPlan<N, TT>;
   constexpr int NT = (N / E) * TT;
   constexpr bool DUAL = dual_layout<N>();
   const size_t fo = (size_t)f * N * N;
-  const c32* src = MODE == 6 ? (DUAL ? a.phic : a.phi) + fo : (DUAL ? a.dbufc : a.dbuf) + fo;
+  // z = 1: DM's estimate (phib, dbuf2, imap2)
+  const c32* src = MODE == 6 ? (z ? (DUAL ? a.phibc : a.phib) : (DUAL ? a.phic : a.phi)) + fo
+                             : (z ? (DUAL ? a.dbufc2 : a.dbuf2) : (DUAL ? a.dbufc : a.dbuf)) + fo;
   const int tid = threadIdx.x, b = tid % TT, t = tid / TT;
   TwStage<N, TT, NT> tws;
   tws.fetch(a.tw);
@@ -520,7 +566,7 @@ __device__ __forceinline__ void colpass_intensity(const PassArgs& a, int f, int 
   c32* twl = lds + L::NBUF * L::BUF;
   tws.put(twl);
   fft_regs<N, E, TT, MODE == 7, true>(x, t, b, lds, twiddle_view<N, TT>(twl));
-  float* dst = a.imap + fo;
+  float* dst = (z ? a.imap2 : a.imap) + fo;
 #pragma unroll
   for (int i = 0; i < E; ++i) {
     const float p2 = fmaf(x[i].x, x[i].x, x[i].y * x[i].y);
@@ -537,27 +583,28 @@ __device__ __forceinline__ void colpass_intensity(const PassArgs& a, int f, int 
 //   SRC 1 (spectrum, MODE 8): div = sqrt(dmap), the spectrally blurred
 //     squared divisor (phasing.py:1140-1156).
 template <int N, int TT, int E, int SRC>
-__device__ __forceinline__ void colpass_div_b(const PassArgs& a, int f, int band,
+__device__ __forceinline__ void colpass_div_b(const PassArgs& a, int f, int band, int z,
                                               double* errp, c32* lds) {
   using L = LdsPlan<N, TT>;
   constexpr int NB = N / TT;
   constexpr int NT = (N / E) * TT;
   constexpr bool DUAL = dual_layout<N>();
   const size_t fo = (size_t)f * N * N;
-  const c32* src = (DUAL ? a.phic : a.phi) + fo;
+  // z = 1: the projection of DM's estimate phib with its own divisor (no error)
+  const c32* src = (z ? (DUAL ? a.phibc : a.phib) : (DUAL ? a.phic : a.phi)) + fo;
   const int tid = threadIdx.x, b = tid % TT, t = tid / TT;
   TwStage<N, TT, NT> tws;
   tws.fetch(a.tw);
   c32 d[SRC == 0 ? E : 1];
   float div[E];
   if constexpr (SRC == 0) {
-    const c32* dsrc = (DUAL ? a.dbufc : a.dbuf) + fo;
+    const c32* dsrc = (z ? (DUAL ? a.dbufc2 : a.dbuf2) : (DUAL ? a.dbufc : a.dbuf)) + fo;
 #pragma unroll
     for (int i = 0; i < E; ++i)
       d[i] = gld(dsrc, DUAL ? mcol_pos<N, TT, E>(band, b, t, i) : col_pos<N, TT, E>(band, b, t, i));
   } else {
 #pragma unroll
-    for (int i = 0; i < E; ++i) div[i] = gld(a.dmap + fo, mcol_pos<N, TT, E>(band, b, t, i));
+    for (int i = 0; i < E; ++i) div[i] = gld((z ? a.dmap2 : a.dmap) + fo, mcol_pos<N, TT, E>(band, b, t, i));
   }
   c32 x[E];
 #pragma unroll
@@ -589,8 +636,8 @@ __device__ __forceinline__ void colpass_div_b(const PassArgs& a, int f, int band
   }
   fft_regs<N, E, TT, true, false>(x, t, b, lds, tw);
 #pragma unroll
-  for (int i = 0; i < E; ++i) gst(a.phi + fo, col_pos<N, TT, E>(band, b, t, i), x[i]);
-  if (errp) {
+  for (int i = 0; i < E; ++i) gst((z ? a.phib : a.phi) + fo, col_pos<N, TT, E>(band, b, t, i), x[i]);
+  if (errp && z == 0) {
     const double s = block_sum(err, reinterpret_cast<double*>(lds));
     if (tid == 0) errp[(size_t)f * NB + band] = s;
   }
@@ -609,13 +656,13 @@ __global__ void __launch_bounds__((N / E) * TT) k_colpass(PassArgs a) {
   if (!(c.flags & kActive)) return;
   if (z && !(c.flags & kUseT)) return;
   if constexpr (MODE == 4) {
-    colpass_ipsf_a<N, TT, E>(a, f, band, lds);
+    colpass_ipsf_a<N, TT, E>(a, f, band, z, lds);
   } else if constexpr (MODE == 5) {
-    colpass_div_b<N, TT, E, 0>(a, f, band, a.errp, lds);
+    colpass_div_b<N, TT, E, 0>(a, f, band, z, a.errp, lds);
   } else if constexpr (MODE == 6 || MODE == 7) {
-    colpass_intensity<N, TT, E, MODE>(a, f, band, lds);
+    colpass_intensity<N, TT, E, MODE>(a, f, band, z, lds);
   } else if constexpr (MODE == 8) {
-    colpass_div_b<N, TT, E, 1>(a, f, band, a.errp, lds);
+    colpass_div_b<N, TT, E, 1>(a, f, band, z, a.errp, lds);
   } else {
     colpass_item<N, TT, E, MODE, true>(a, f, band, z, a.errp, lds,
                                        MODE == 9 && (c.flags & kSwFin));
@@ -957,6 +1004,282 @@ __global__ void __launch_bounds__((N / E) * TT)
       if (c.flags & kActive) rowpass_item<N, TT, E, false, false, false>(a, f, band, c, lds);
     }
     if (k + 1 < nsteps) grid_sync(bar, ++epoch * G, flag);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Resident single-frame iteration block (k_resident).  One cooperative
+// launch of one workgroup per band runs a block of steps; the workgroup of
+// band q owns column band q (column phase) and row band q (row phase) for the
+// whole launch and keeps the frame's modulus (its column band) and support
+// and iterate (its row band) in registers, loading them once.  Per step only
+// Phi crosses between workgroups, through global memory:
+//   column phase: Phi column band -> FFT -> projection (+ error partial) ->
+//                 IFFT -> Phi column band   (error only for kNoOut steps)
+//   grid barrier
+//   row phase:    Phi row band -> IFFT -> update with psi, s (registers) ->
+//                 FFT -> Phi row band
+//   grid barrier
+// The arithmetic is that of colpass_item / rowpass_item (same transforms,
+// projection and update), so results equal the per-pass launches bitwise.
+// At the end the workgroup stores its rows of psi.
+//
+// Exchange mode XM (how Phi crosses the grid barrier between XCDs, each with
+// its own L2):
+//   0  plain loads/stores; the barrier fences at device scope (write back
+//      the L2's dirty lines before arriving, invalidate after the wait)
+//   1  device-scope (sc1) loads and stores of Phi, and the fences of 0
+//   2  device-scope loads and stores of Phi; the barrier orders them without
+//      fences (each wave waits for its stores before the arrival)
+// ---------------------------------------------------------------------------
+template <int XM>
+__device__ __forceinline__ c32 xld(const c32* base, unsigned i) {
+  if constexpr (XM == 0) {
+    return gld(base, i);
+  } else {
+    const uint64_t v = __hip_atomic_load(reinterpret_cast<const uint64_t*>(base) + i,
+                                         __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return c32{__uint_as_float((unsigned)v), __uint_as_float((unsigned)(v >> 32))};
+  }
+}
+template <int XM>
+__device__ __forceinline__ void xst(c32* base, unsigned i, const c32& v) {
+  if constexpr (XM == 0) {
+    gst(base, i, v);
+  } else {
+    const uint64_t w = (uint64_t)__float_as_uint(v.x) | ((uint64_t)__float_as_uint(v.y) << 32);
+    __hip_atomic_store(reinterpret_cast<uint64_t*>(base) + i, w, __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+// Float exchange of the resident kernel's shrinkwrap maps, device scope in
+// the exchange modes without fences (as xld / xst)
+template <int XM>
+__device__ __forceinline__ float xldf(const float* p, unsigned i) {
+  if constexpr (XM != 2) return gld(p, i);
+  else return __hip_atomic_load(p + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <int XM>
+__device__ __forceinline__ void xstf(float* p, unsigned i, float v) {
+  if constexpr (XM != 2) gst(p, i, v);
+  else __hip_atomic_store(p + i, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// sw_store_staged for the resident kernel: float4 stores, or in the fenceless
+// exchange mode two device-scope 8-byte stores per group of four
+template <int N, int TT, int XM, class OFF>
+__device__ __forceinline__ void sw_store_staged_x(const float* stage, float* dst, OFF off) {
+  for (int v = threadIdx.x; v < N * TT / 4; v += blockDim.x) {
+    if constexpr (XM != 2) {
+      *reinterpret_cast<float4*>(dst + off(v)) = *reinterpret_cast<const float4*>(stage + 4 * v);
+    } else {
+      const uint64_t* s2 = reinterpret_cast<const uint64_t*>(stage + 4 * v);
+      uint64_t* d2 = reinterpret_cast<uint64_t*>(dst + off(v));
+      __hip_atomic_store(d2, s2[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(d2 + 1, s2[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+  }
+}
+
+// Grid barrier of the resident kernel.  Every workgroup publishes the
+// barrier's epoch in its own flag (one 128-byte line per workgroup, so the
+// arrivals do not contend for one counter), and the first wavefront of every
+// workgroup reads all the flags until each holds the epoch.  Measured on
+// MI355X for 256 workgroups: 2.2 us per barrier, against 8 us with one
+// shared arrival counter.  Device-scope fences as the exchange mode needs
+// them (release before the flag, acquire after the wait).  Waits are
+// bounded as in grid_sync: after 2^18 polls flag[0] (this launch's) and
+// flag[1] (sticky) are set and every later wait of the launch passes.
+template <int XM>
+__device__ __forceinline__ void resident_sync(unsigned* flags, unsigned epoch, int* flag) {
+  constexpr int LINE = 32;   // unsigned words per 128-byte line
+  constexpr bool fence = XM != 2;
+  __builtin_amdgcn_s_waitcnt(0x0F70);   // this wave's stores have completed
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    if (fence) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __hip_atomic_store(flags + blockIdx.x * LINE, epoch, __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+  }
+  if (threadIdx.x < 64) {
+    bool gave = __hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
+    for (unsigned j = threadIdx.x; j < gridDim.x; j += 64) {
+      unsigned polls = 0;
+      while (!gave && __hip_atomic_load(flags + j * LINE, __ATOMIC_RELAXED,
+                                        __HIP_MEMORY_SCOPE_AGENT) < epoch) {
+        __builtin_amdgcn_s_sleep(1);
+        if (++polls > (1u << 18)) {
+          __hip_atomic_store(flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          __hip_atomic_store(flag + 1, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          gave = true;
+        }
+      }
+    }
+    if (fence) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  }
+  __syncthreads();
+}
+
+// LDS of the resident kernel: the pass LDS, padded so that one workgroup
+// occupies a CU (the grid is one workgroup per band, at most one per CU)
+template <int N, int TT>
+constexpr int resident_lds_elems() {
+  constexpr int base = LdsPlan<N, TT>::TOTAL;
+  constexpr int half = (kLdsBytes / 2) / 8 + 64;   // more than half a CU's LDS
+  return base > half ? base : half;
+}
+
+template <int N, int TT, int E, int XM>
+__global__ void __launch_bounds__((N / E) * TT)
+    k_resident(PassArgs a, const RStep* __restrict__ steps, int nsteps, double* errp0,
+               long long errp_step, unsigned* flags, int* flag) {
+  constexpr int NT = (N / E) * TT;
+  constexpr int NB = N / TT;
+  constexpr int T = N / E;
+  using L = LdsPlan<N, TT>;
+  __shared__ c32 lds[resident_lds_elems<N, TT>()];
+  // the shrinkwrap blur lines and staged outputs live after the twiddle
+  // table, which stays in LDS for the whole launch
+  static_assert(L::TOTAL * 8 + (sw_lds_floats<N, TT>() + 32) * 4 <=
+                    resident_lds_elems<N, TT>() * 8,
+                "resident LDS holds the pass LDS, the blur lines and weights");
+  float* swl = reinterpret_cast<float*>(lds + L::TOTAL);
+  float* swwl = swl + sw_lds_floats<N, TT>();   // blur weights
+  const int tid = threadIdx.x, b = tid % TT, t = tid / TT;
+  const int cband = xcd_band<NB>(blockIdx.x);   // as k_colpass
+  const int rband = blockIdx.x;                 // as k_rowpass
+  c32* twl = lds + L::NBUF * L::BUF;
+  {
+    TwStage<N, TT, NT> tws;
+    tws.fetch(a.tw);
+    tws.put(twl);
+  }
+  const auto tw = twiddle_view<N, TT>(twl);
+  // resident state: modulus of the column band, support and iterate of the row band
+  float m[E], sv[E];
+  c32 pin[E];
+#pragma unroll
+  for (int i = 0; i < E; ++i) {
+    m[i] = gld(a.mod, mcol_pos<N, TT, E>(cband, b, t, i));
+    sv[i] = gld(a.sup, row_pos<N, TT, E>(rband, b, t, i));
+    pin[i] = gld(a.psi, row_pos<N, TT, E>(rband, b, t, i));
+  }
+  c32* phi = a.phi;
+  unsigned epoch = 0;
+  bool sup_changed = false;
+  for (int k = 0; k < nsteps; ++k) {
+    const RStep st = steps[k];
+    const bool noout = (st.c.flags & kNoOut) != 0;
+    const bool swrow = (st.c.flags & kSwRow) != 0;   // shrinkwrap after this step
+    const bool swfin = (st.c.flags & kSwFin) != 0;   // ... after the previous one
+    const bool col = !noout || st.err || swfin;
+    c32 x[E];
+    if (col) {
+      // column phase (colpass_item MODE 0, MODE 10 for kNoOut, MODE 9's blur)
+      __syncthreads();   // the LDS of the previous phase is free
+#pragma unroll
+      for (int i = 0; i < E; ++i) x[i] = xld<XM>(phi, col_pos<N, TT, E>(cband, b, t, i));
+      double err = 0.0;
+      fft_regs<N, E, TT, false, true>(x, t, b, lds, tw);
+#pragma unroll
+      for (int i = 0; i < E; ++i) x[i] = project_modulus(x[i], m[i], err);
+      if (!noout) {
+        fft_regs<N, E, TT, true, false>(x, t, b, lds, tw);
+#pragma unroll
+        for (int i = 0; i < E; ++i) xst<XM>(phi, col_pos<N, TT, E>(cband, b, t, i), x[i]);
+      }
+      if (st.err && errp0) {
+        const double s = block_sum(err, reinterpret_cast<double*>(lds));
+        if (tid == 0) errp0[(size_t)st.kidx * errp_step + cband] = s;
+      }
+      if (swfin) {
+        // the column half of the previous step's shrinkwrap blur (as
+        // colpass_item MODE 9): the band's columns of the row-blurred |psi|
+        // -> sw_blur (tiled) and the band maximum
+        float rb[E];
+#pragma unroll
+        for (int i = 0; i < E; ++i) rb[i] = xldf<XM>(a.sw_tmp, col_pos<N, TT, E>(cband, b, t, i));
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < E; ++i) swl[b * sw_line<N>() + blur_slot(t + i * T)] = rb[i];
+        sw_weights_lds(a.swr_tab[st.swf].w, swwl);
+        __syncthreads();
+        float* stage = swl + sw_stage_off<N, TT>();
+        const float mx = sw_blur_lines_lds<N, TT>(swl, swwl, [&](int l, int row, float v) {
+          stage[(row / TT) * (TT * TT) + (row % TT) * TT + l] = v;
+        });
+        __syncthreads();
+        sw_store_staged_x<N, TT, XM>(stage, a.sw_blur, [&](int v) {
+          const int q = (4 * v) / (TT * TT), within = (4 * v) % (TT * TT);
+          return (q * NB + cband) * (TT * TT) + within;
+        });
+        const float bm = block_max(mx, swl);
+        if (tid == 0) xstf<XM>(a.sw_maxp, cband, bm);
+      }
+      resident_sync<XM>(flags, ++epoch, flag);
+    }
+    // row phase (rowpass_item; kNoOut skips Phi)
+    __syncthreads();
+    if (swfin) {
+      // the previous step's threshold: s = 0 where blur <= tau * max, with
+      // the frame maximum from the column bands' maxima (FP64 threshold)
+      constexpr int MPT = (NB + NT - 1) / NT;
+      float mx = -INFINITY;
+#pragma unroll
+      for (int j = 0; j < MPT; ++j) {
+        const int q = tid + j * NT;
+        if (q < NB) mx = fmaxf(mx, xldf<XM>(a.sw_maxp, q));
+      }
+      const double thr = (double)a.swr_tab[st.swf].tau * (double)block_max(mx, swl);
+#pragma unroll
+      for (int i = 0; i < E; ++i) {
+        const float bl = xldf<XM>(a.sw_blur, row_pos<N, TT, E>(rband, b, t, i));
+        if (!((double)bl > thr)) sv[i] = 0.0f;
+      }
+      sup_changed = true;
+      __syncthreads();
+    }
+    if (!noout) {
+#pragma unroll
+      for (int i = 0; i < E; ++i) x[i] = xld<XM>(phi, row_pos<N, TT, E>(rband, b, t, i));
+      fft_regs<N, E, TT, true, true>(x, t, b, lds, tw);
+    }
+    float av[E];
+#pragma unroll
+    for (int i = 0; i < E; ++i) {
+      const c32 pout = noout ? c32{0.0f, 0.0f} : scale(x[i], a.inv_nn);
+      const c32 nv = update_value<false>(st.c, pout, pin[i], sv[i], c32{0.0f, 0.0f});
+      pin[i] = nv;
+      x[i] = nv;
+      av[i] = sqrtf(nv.x * nv.x + nv.y * nv.y);
+    }
+    fft_regs<N, E, TT, false, false>(x, t, b, lds, tw);
+#pragma unroll
+    for (int i = 0; i < E; ++i) xst<XM>(phi, row_pos<N, TT, E>(rband, b, t, i), x[i]);
+    if (swrow) {
+      // the row half of this step's shrinkwrap blur (as rowpass_item SW 1):
+      // the band's rows of |new psi| -> sw_tmp
+      __syncthreads();
+#pragma unroll
+      for (int i = 0; i < E; ++i) swl[b * sw_line<N>() + blur_slot(t + i * T)] = av[i];
+      sw_weights_lds(a.swr_tab[st.swr].w, swwl);
+      __syncthreads();
+      float* stage = swl + sw_stage_off<N, TT>();
+      sw_blur_lines_lds<N, TT>(swl, swwl, [&](int l, int c, float v) {
+        stage[(c / TT) * (TT * TT) + l * TT + (c % TT)] = v;
+      });
+      __syncthreads();
+      sw_store_staged_x<N, TT, XM>(stage, a.sw_tmp + (size_t)rband * N * TT,
+                                   [](int v) { return 4 * v; });
+    }
+    if (k + 1 < nsteps) resident_sync<XM>(flags, ++epoch, flag);
+  }
+#pragma unroll
+  for (int i = 0; i < E; ++i) gst(a.psi, row_pos<N, TT, E>(rband, b, t, i), pin[i]);
+  if (sup_changed) {
+    float* sup = const_cast<float*>(a.sup);
+#pragma unroll
+    for (int i = 0; i < E; ++i) gst(sup, row_pos<N, TT, E>(rband, b, t, i), sv[i]);
   }
 }
 
@@ -1755,9 +2078,102 @@ static __global__ void __launch_bounds__(256) k_stack_mean(const c32* data, long
 // ---------------------------------------------------------------------------
 // plain copy (HBM counter calibration)
 // ---------------------------------------------------------------------------
+// Plain device copy (the HBM reference point and the counters' calibration):
+// each thread moves four 16-byte vectors, all four loads issued before the
+// stores, one pass over the buffer (no grid-stride loop).
 static __global__ void __launch_bounds__(256) k_copy16(uint4* dst, const uint4* src, size_t n16) {
-  for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n16; i += (size_t)gridDim.x * 256)
-    dst[i] = src[i];
+  const size_t base = (size_t)blockIdx.x * 1024 + threadIdx.x;
+  uint4 v[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const size_t i = base + (size_t)k * 256;
+    if (i < n16) v[k] = src[i];
+  }
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const size_t i = base + (size_t)k * 256;
+    if (i < n16) dst[i] = v[k];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Mixed-size passes: one launch covers the frames of several plans of
+// different sizes (256^2, 512^2, 1024^2), each in the tile that makes its
+// workgroup 256 threads of eight values (256^2: tile 8, 512^2: tile 4,
+// 1024^2: tile 2), so every workgroup of the launch has the same shape and
+// runs the same per-band code as that size's own pass kernel (bitwise the
+// same results).  The LDS is the largest of the three sizes' plans.  A
+// workgroup finds its member by the start table (at most kGroupMax
+// entries), then its frame and band; bands are numbered per frame as in the
+// single-size launch (XCD-aware for the column pass: every member's first
+// workgroup is a multiple of 8, since every size has a multiple of 8 bands).
+// ---------------------------------------------------------------------------
+template <int N>
+constexpr int group_lds() { return LdsPlan<N, group_tile(N)>::TOTAL; }
+constexpr int kGroupLds = group_lds<256>() > group_lds<512>()
+                              ? (group_lds<256>() > group_lds<1024>() ? group_lds<256>()
+                                                                      : group_lds<1024>())
+                              : (group_lds<512>() > group_lds<1024>() ? group_lds<512>()
+                                                                      : group_lds<1024>());
+
+__device__ __forceinline__ int group_member(const GroupPass& g) {
+  int m = 0;
+  while (m + 1 < g.count && (int)blockIdx.x >= g.start[m + 1]) ++m;
+  return m;
+}
+
+template <int N, int MODE>
+__device__ __forceinline__ void colpass_group_item(const PassArgs& a, int local, c32* lds) {
+  constexpr int TT = group_tile(N), NB = N / TT;
+  static_assert((N / kGroupEpt) * TT == kGroupThreads, "one workgroup shape for every member");
+  static_assert(MODE != 9 || TT * sw_line<N>() * sizeof(float) <= kGroupLds * sizeof(c32),
+                "fused blur lines exceed the pass's LDS");
+  const int fi = local / NB, band = xcd_band<NB>(local % NB);
+  const int f = a.fmap ? a.fmap[fi] : fi;
+  const Coef c = a.tab ? a.tab[f] : a.c;
+  if (!(c.flags & kActive)) return;
+  colpass_item<N, TT, kGroupEpt, MODE, true>(a, f, band, 0, a.errp, lds,
+                                              MODE == 9 && (c.flags & kSwFin));
+}
+
+template <int N, int SW>
+__device__ __forceinline__ void rowpass_group_item(const PassArgs& a, int local, c32* lds) {
+  constexpr int TT = group_tile(N), NB = N / TT;
+  static_assert(SW == 0 || TT * sw_line<N>() * sizeof(float) <= kGroupLds * sizeof(c32),
+                "fused blur lines exceed the pass's LDS");
+  const int fi = local / NB, band = local % NB;
+  const int f = a.fmap ? a.fmap[fi] : fi;
+  const Coef c = a.tab ? a.tab[f] : a.c;
+  if (!(c.flags & kActive)) return;
+  rowpass_item<N, TT, kGroupEpt, false, false, true, SW, false>(a, f, band, c, lds);
+}
+
+template <int MODE>
+__global__ void __launch_bounds__(kGroupThreads) k_colpass_group(GroupPass g) {
+  __shared__ c32 lds[kGroupLds];
+  const int m = group_member(g);
+  const PassArgs a = g.args[m];
+  const int local = (int)blockIdx.x - g.start[m];
+  switch (g.n[m]) {
+    case 256: colpass_group_item<256, MODE>(a, local, lds); break;
+    case 512: colpass_group_item<512, MODE>(a, local, lds); break;
+    case 1024: colpass_group_item<1024, MODE>(a, local, lds); break;
+    default: break;
+  }
+}
+
+template <int SW>
+__global__ void __launch_bounds__(kGroupThreads) k_rowpass_group(GroupPass g) {
+  __shared__ c32 lds[kGroupLds];
+  const int m = group_member(g);
+  const PassArgs a = g.args[m];
+  const int local = (int)blockIdx.x - g.start[m];
+  switch (g.n[m]) {
+    case 256: rowpass_group_item<256, SW>(a, local, lds); break;
+    case 512: rowpass_group_item<512, SW>(a, local, lds); break;
+    case 1024: rowpass_group_item<1024, SW>(a, local, lds); break;
+    default: break;
+  }
 }
 
 // ===========================================================================
@@ -1769,13 +2185,50 @@ static __global__ void __launch_bounds__(256) k_copy16(uint4* dst, const uint4* 
 #ifndef SPK_ONLY_N
 #define SPK_BUILD_COMMON 1
 #define SPK_BUILD_SIZE(n) 1
+#define SPK_BUILD_GROUP 1
 #elif SPK_ONLY_N == 0
 #define SPK_BUILD_COMMON 1
 #define SPK_BUILD_SIZE(n) 0
+#define SPK_BUILD_GROUP 0
+#elif SPK_ONLY_N == -1
+// the mixed-size pass kernels
+#define SPK_BUILD_COMMON 0
+#define SPK_BUILD_SIZE(n) 0
+#define SPK_BUILD_GROUP 1
 #else
 #define SPK_BUILD_COMMON 0
 #define SPK_BUILD_SIZE(n) ((n) == SPK_ONLY_N)
+#define SPK_BUILD_GROUP 0
 #endif
+
+#if SPK_BUILD_GROUP
+hipError_t launch_colpass_group(int mode, const GroupPass& g, hipStream_t st) {
+  if (g.count < 1 || g.count > kGroupMax) return hipErrorInvalidValue;
+  const dim3 grid(g.start[g.count]), block(kGroupThreads);
+  if (grid.x == 0) return hipSuccess;
+  if (mode == 0)
+    hipLaunchKernelGGL(k_colpass_group<0>, grid, block, 0, st, g);
+  else if (mode == 9)
+    hipLaunchKernelGGL(k_colpass_group<9>, grid, block, 0, st, g);
+  else
+    return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
+hipError_t launch_rowpass_group(int sw, const GroupPass& g, hipStream_t st) {
+  if (g.count < 1 || g.count > kGroupMax) return hipErrorInvalidValue;
+  const dim3 grid(g.start[g.count]), block(kGroupThreads);
+  if (grid.x == 0) return hipSuccess;
+  switch (sw) {
+    case 0: hipLaunchKernelGGL(k_rowpass_group<0>, grid, block, 0, st, g); break;
+    case 1: hipLaunchKernelGGL(k_rowpass_group<1>, grid, block, 0, st, g); break;
+    case 2: hipLaunchKernelGGL(k_rowpass_group<2>, grid, block, 0, st, g); break;
+    case 3: hipLaunchKernelGGL(k_rowpass_group<3>, grid, block, 0, st, g); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+#endif  // SPK_BUILD_GROUP
 
 #define SPK_SIZES(X) X(16) X(32) X(64) X(128) X(256) X(512) X(1024) X(2048) X(4096)
 
@@ -1886,6 +2339,45 @@ static hipError_t rowfft_nt(int mode, const c32* in, c32* out, const float* sup,
   return hipGetLastError();
 }
 
+template <int N, int TT, int E>
+static hipError_t resident_nt(int xm, const PassArgs& a, const RStep* steps, int nsteps,
+                              double* errp0, long long errp_step, unsigned* flags, int* flag,
+                              hipStream_t st) {
+  // the workgroups' barrier flags and this launch's give-up flag start at 0
+  constexpr int words = (N / TT) * 32;
+  hipLaunchKernelGGL(k_zero_u32, dim3((words + 255) / 256), dim3(256), 0, st, flags, words);
+  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, reinterpret_cast<unsigned*>(flag), 1);
+  void* args[] = {(void*)&a, (void*)&steps, (void*)&nsteps, (void*)&errp0, (void*)&errp_step,
+                  (void*)&flags, (void*)&flag};
+  const void* fn = xm == 1   ? reinterpret_cast<const void*>(&k_resident<N, TT, E, 1>)
+                   : xm == 2 ? reinterpret_cast<const void*>(&k_resident<N, TT, E, 2>)
+                             : reinterpret_cast<const void*>(&k_resident<N, TT, E, 0>);
+  if (g_launch_events[0]) {
+    (void)hipEventRecord(g_launch_events[0], st);
+  }
+  hipError_t e = hipLaunchCooperativeKernel(fn, dim3(N / TT), dim3((N / E) * TT), args, 0, st);
+  if (g_launch_events[0]) {
+    (void)hipEventRecord(g_launch_events[1], st);
+    g_launch_events[0] = g_launch_events[1] = nullptr;
+  }
+  return e;
+}
+
+// the resident form exists for one frame of 256^2 .. 1024^2 in tile 4 with
+// 8 values per thread (one workgroup of at most 512 threads per band, at
+// most 256 bands: one per CU)
+template <int N>
+static hipError_t resident_dispatch(int tile, int ept, int xm, const PassArgs& a,
+                                    const RStep* steps, int nsteps, double* errp0,
+                                    long long errp_step, unsigned* bar, int* flag,
+                                    hipStream_t st) {
+  if constexpr (N >= 256 && N <= 1024) {
+    if (tile == 4 && ept == 8)
+      return resident_nt<N, 4, 8>(xm, a, steps, nsteps, errp0, errp_step, bar, flag, st);
+  }
+  return hipErrorInvalidValue;
+}
+
 template <int N, int TT>
 static hipError_t shrinkwrap_nt(const c32* psi, float* sup, long long sstride, float* tmp,
                                 float* blur, float* maxp, const SwParam* tab, const SwParam& p0,
@@ -1942,7 +2434,10 @@ constexpr bool shape_ok() {
                          unsigned* bar, int* flag, int grid, hipStream_t st);               \
   hipError_t persist_grid_##n(int tile, int ept, int* out);                                 \
   hipError_t rowconv_##n(int tile, int ept, const c32* in, c32* out, const c32* ipsf,       \
-                         const c32* tw, const int* fmap, int frames, hipStream_t st);
+                         const c32* tw, const int* fmap, int frames, hipStream_t st);       \
+  hipError_t resident_##n(int tile, int ept, int xm, const PassArgs& a, const RStep* steps, \
+                          int nsteps, double* errp0, long long errp_step, unsigned* bar,    \
+                          int* flag, hipStream_t st);
 
 // a shape that is not built compiles to an error return, not a kernel
 #define SPK_CALL_COL(TT_, E_)                                                   \
@@ -2021,6 +2516,12 @@ constexpr bool shape_ok() {
                          const c32* tw, const int* fmap, int frames, hipStream_t st) {      \
     constexpr int SPK_N = n;                                                                \
     SPK_SHAPE_SWITCH(n, SPK_CALL_CONV)                                                      \
+  }                                                                                         \
+  hipError_t resident_##n(int tile, int ept, int xm, const PassArgs& a, const RStep* steps, \
+                          int nsteps, double* errp0, long long errp_step, unsigned* bar,    \
+                          int* flag, hipStream_t st) {                                      \
+    return resident_dispatch<n>(tile, ept, xm, a, steps, nsteps, errp0, errp_step, bar,     \
+                                flag, st);                                                  \
   }
 
 SPK_SIZES(SPK_SIZED_DECL)
@@ -2088,6 +2589,16 @@ hipError_t launch_colpass(int n, int tile, int ept, int mode, const PassArgs& a,
 hipError_t launch_rowpass(int n, int tile, int ept, bool use_t, bool emit_b, int sw,
                           bool noout, const PassArgs& a, int frames, hipStream_t st) {
   SPK_SIZE_SWITCH(rowpass, tile, ept, use_t, emit_b, sw, noout, a, frames, st)
+}
+
+bool resident_supported(int n, int tile, int ept) {
+  return n >= 256 && n <= 1024 && (n & (n - 1)) == 0 && tile == 4 && ept == 8;
+}
+
+hipError_t launch_resident(int n, int tile, int ept, int xm, const PassArgs& a,
+                           const RStep* steps, int nsteps, double* errp0, long long errp_step,
+                           unsigned* bar, int* flag, hipStream_t st) {
+  SPK_SIZE_SWITCH(resident, tile, ept, xm, a, steps, nsteps, errp0, errp_step, bar, flag, st)
 }
 
 hipError_t launch_rowfft(int n, int tile, int ept, int mode, const c32* in, c32* out,
@@ -2329,8 +2840,9 @@ hipError_t launch_roll_sum(const c32* data, int frames, int rows, int cols, cons
 
 hipError_t launch_copy(void* dst, const void* src, size_t bytes, hipStream_t st) {
   const size_t n16 = bytes / 16;
-  hipLaunchKernelGGL(k_copy16, dim3(2048), dim3(256), 0, st, (uint4*)dst, (const uint4*)src,
-                     n16);
+  if (n16 == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_copy16, dim3((unsigned)((n16 + 1023) / 1024)), dim3(256), 0, st,
+                     (uint4*)dst, (const uint4*)src, n16);
   return hipGetLastError();
 }
 

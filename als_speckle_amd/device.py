@@ -19,12 +19,15 @@ from ._capi import check
 
 
 _live_plans: "weakref.WeakSet[Plan]" = weakref.WeakSet()
+_live_groups: "weakref.WeakSet[Group]" = weakref.WeakSet()
 _live_ctxs: "weakref.WeakSet[Context]" = weakref.WeakSet()
 
 
 @atexit.register
 def _close_all() -> None:
     # release device objects while the HIP runtime is still up
+    for g in list(_live_groups):
+        g.close()
     for p in list(_live_plans):
         p.close()
     for c in list(_live_ctxs):
@@ -136,7 +139,7 @@ class Plan:
 
     def __init__(self, ctx: Context, n: int, batch: int = 1, shared: bool = True,
                  tile: int = 0, ept: int = 0, persist: bool = False,
-                 sw_separate: bool = False):
+                 sw_separate: bool = False, resident: int = 0, exchange: int = 0):
         self.ctx, self.lib = ctx, ctx.lib
         self.n, self.batch, self.shared = int(n), int(batch), bool(shared)
         self.persist = bool(persist)
@@ -145,6 +148,8 @@ class Plan:
         tune.ept = int(ept)
         tune.persist = int(self.persist)
         tune.sw_separate = int(bool(sw_separate))
+        tune.resident = int(resident)      # 1 on, 2 off, 0 library default
+        tune.exchange = int(exchange)
         h = c_void_p()
         check(self.lib.spk_plan_create(ctx.handle, self.n, self.batch, int(self.shared),
                                        byref(tune), byref(h)))
@@ -213,6 +218,12 @@ class Plan:
         fu, se = ctypes.c_longlong(), ctypes.c_longlong()
         check(self.lib.spk_plan_sw_counts(self.handle, byref(fu), byref(se)))
         return fu.value, se.value
+
+    def resident_counts(self) -> tuple[int, int]:
+        """(launches, steps) of the resident single-frame launches so far."""
+        la, st = ctypes.c_longlong(), ctypes.c_longlong()
+        check(self.lib.spk_plan_resident_counts(self.handle, byref(la), byref(st)))
+        return la.value, st.value
 
     def shrinkwrap(self, sigma: float, tau: float, frame: int = -1) -> None:
         check(self.lib.spk_shrinkwrap(self.handle, frame, float(sigma), float(tau)))
@@ -306,7 +317,8 @@ class Plan:
         check(self.lib.spk_plan_profile(self.handle, int(enable)))
 
     def profile_read(self, which: int):
-        """(launches, total_ms) of pass 0 (column) or 1 (row) since the last read."""
+        """(launches, total_ms) of pass 0 (column), 1 (row) or 2 (resident
+        launches) since the last read."""
         n = ctypes.c_int()
         ms = ctypes.c_double()
         check(self.lib.spk_plan_profile_read(self.handle, which, byref(n), byref(ms)))
@@ -366,6 +378,77 @@ class Graph:
     def close(self) -> None:
         if getattr(self, "handle", None):
             self.lib.spk_graph_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        if sys.is_finalizing():
+            return
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def group_shape(n: int) -> tuple[int, int]:
+    """(tile, ept) a plan of size n needs to join a mixed-size launch
+    (sizes 256, 512 and 1024); raises SpkError for other sizes."""
+    lib = _capi.load()
+    t, e = ctypes.c_int(), ctypes.c_int()
+    check(lib.spk_group_shape(int(n), byref(t), byref(e)))
+    return t.value, e.value
+
+
+class Group:
+    """Frames of different sizes in one launch per pass (spk_group_iterate):
+    ``iterate`` runs one spk_iterate call per plan with the plans'
+    single-projection passes merged into one column-pass and one row-pass
+    launch per step.  Plans come from ``group_plan`` (or any plan in the
+    ``group_shape`` tile) on this group's context."""
+
+    def __init__(self, ctx: Context):
+        self.ctx, self.lib = ctx, ctx.lib
+        h = c_void_p()
+        check(self.lib.spk_group_create(ctx.handle, byref(h)))
+        self.handle = h
+        _live_groups.add(self)
+
+    def plan(self, n: int, batch: int = 1, shared: bool = True, **kw) -> Plan:
+        """A plan of size n in the mixed-size shape, on the group's context."""
+        tile, ept = group_shape(n)
+        return Plan(self.ctx, n, batch=batch, shared=shared, tile=tile, ept=ept, **kw)
+
+    def iterate(self, plans, steps, per_frame=None, error_trace: bool = True):
+        """plans: list of Plan; steps: one step table per plan (as for
+        Plan.iterate), all with the same number of steps; per_frame: one flag
+        per plan (default all False).  Returns one (nsteps, batch) float64
+        error tensor per plan, or Nones."""
+        count = len(plans)
+        if len(steps) != count:
+            raise ValueError("one step table per plan")
+        per_frame = [False] * count if per_frame is None else [bool(x) for x in per_frame]
+        tabs = [_capi.step_table(s) for s in steps]
+        nsteps = [t.size // (p.batch if pf else 1) for t, p, pf in zip(tabs, plans, per_frame)]
+        if len(set(nsteps)) > 1:
+            raise ValueError(f"every plan takes the same number of steps, got {nsteps}")
+        ns = nsteps[0] if nsteps else 0
+        errs = [torch.empty((ns, p.batch), dtype=torch.float64, device=self.ctx.device)
+                if error_trace else None for p in plans]
+        hs = (c_void_p * count)(*[p.handle.value for p in plans])
+        ts = (c_void_p * count)(*[t.ctypes.data for t in tabs])
+        pf = (ctypes.c_int * count)(*[int(x) for x in per_frame])
+        es = (c_void_p * count)(*[(e.data_ptr() if e is not None else None) for e in errs])
+        check(self.lib.spk_group_iterate(self.handle, hs, count, ts, ns, pf, es))
+        return errs
+
+    def counts(self) -> tuple[int, int]:
+        """(merged launches, member passes they covered) so far."""
+        la, mp = ctypes.c_longlong(), ctypes.c_longlong()
+        check(self.lib.spk_group_counts(self.handle, byref(la), byref(mp)))
+        return la.value, mp.value
+
+    def close(self) -> None:
+        if getattr(self, "handle", None):
+            self.lib.spk_group_destroy(self.handle)
             self.handle = None
 
     def __del__(self):

@@ -327,6 +327,51 @@ class phasing:
         return torch.roll(g, shifts=(-dr, -dc), dims=(0, 1)).contiguous()
 
     # ------------------------------------------------------------------
+    def _debug_snapshots(self, iteration, alg, when):
+        """``iterate(debug_on=...)`` dumps of ``_iteration``
+        (``phasing.py:1193-1226``, ``1244-1246``), as FITS files in the
+        working directory with the reference's names: before the iteration
+        ``psi_fourier_1`` (fftshift of Psi = fft2(psi_in)), ``fourier_div``
+        (the divisor) and ``psi_fourier_2`` (Psi after the modulus
+        replacement), in polar components, and ``out`` (psi_out) in cartesian
+        components; after a HIO iteration ``hio update`` (the new psi_in).
+        The intermediates come from the library's stand-alone transform of
+        the same iterate (the iteration itself runs the fused passes, which
+        keep them on chip); with a spectral rescale only ``psi_fourier_1``
+        and the update are written.  For DM the reference's second
+        projection overwrites the same files; these hold the first."""
+        from . import io as sio
+        shift = np.fft.fftshift
+        tag = "%s i%s" % (self.compute_device, iteration)
+        if when == "after":
+            if alg == "hio":
+                sio.save("%s hio update.fits" % tag, self.get(self.psi_in), components="cartesian")
+            return
+        psi = self._plan.get_psi()
+        Psi_d = self._plan.fft2(psi)
+        Psi = Psi_d.cpu().numpy()
+        sio.save("%s psi_fourier_1.fits" % tag, shift(Psi), components="polar")
+        if getattr(self, "_spectrum_host", None) is not None:
+            warnings.warn("debug_on: with a spectrum only psi_fourier_1 and the update are written")
+            return
+        ipsf = getattr(self, "_ipsf_host", None)
+        if ipsf is not None:
+            # div = sqrt(|ifft2(fft2(|Psi|^2) * ipsf)|)  (phasing.py:1175-1183)
+            dev = self.ctx.device
+            I = torch.from_numpy((np.abs(Psi) ** 2).astype(np.complex64)).to(dev)
+            F = self._plan.fft2(I).cpu().numpy() * ipsf
+            conv = self._plan.fft2(torch.from_numpy(F.astype(np.complex64)).to(dev), inverse=True)
+            div = np.sqrt(np.abs(conv.cpu().numpy())).astype(np.float32)
+        else:
+            div = np.abs(Psi).astype(np.float32)
+        sio.save("%s fourier_div.fits" % tag, shift(div.astype(np.complex64)), components="polar")
+        m = self._modulus_dev.cpu().numpy()
+        with np.errstate(divide="ignore", invalid="ignore"):
+            Pp = np.where(div > 0, Psi * (m / np.where(div > 0, div, 1)), 0).astype(np.complex64)
+        sio.save("%s psi_fourier_2.fits" % tag, shift(Pp), components="polar")
+        out = self._plan.fft2(torch.from_numpy(Pp).to(self.ctx.device), inverse=True)
+        sio.save("%s out.fits" % tag, out.cpu().numpy(), components="cartesian")
+
     def iterate(self, iterations, order=None, silent=True, beta=0.8, debug_on=None,
                 shrinkwrap=None, semantics="numpy", error_trace=True):
         """Run iterations, ``phasing.py:181-289``.
@@ -343,10 +388,10 @@ class phasing:
             kernels) for ER and SF.
         error_trace: record the modulus error of every iteration
             (``get_error_trace``).
+        debug_on: iterations (0-based) at which to write the reference's
+            FITS debugging snapshots (see ``_debug_snapshots``).
         At the end the ROI of the iterate is copied to savebuffer[numtrial].
         """
-        if debug_on:
-            warnings.warn("debug_on snapshot dumps are not supported; ignored")
         if not (self.modulus_state == 2 and self.support_state == 2):
             print("cant iterate before loading support and modulus.")
             print("modulus state: " + str(self.modulus_state))
@@ -369,23 +414,32 @@ class phasing:
             sig, tau = sw.get(k, (0.0, 0.0))
             steps.append((_capi.ALG_CODES[a], sem, beta, sig, tau))
 
-        # progress printing needs host sync points; otherwise one call
+        # progress printing and debug snapshots need host sync points;
+        # otherwise one call
         every = None
         if silent is not True:
             every = 1 if (silent is False or not isinstance(silent, int)) else max(1, int(silent))
+        debug = sorted({int(k) for k in (debug_on or ()) if 0 <= int(k) < iterations})
+        cuts = set(debug) | {k + 1 for k in debug} | {0, iterations}
+        if every is not None:
+            cuts |= set(range(0, iterations, every))
+        cuts = sorted(c for c in cuts if c <= iterations)
         traces = []
-        if every is None or iterations == 0:
-            if iterations:
-                traces.append(self._plan.iterate(steps, error_trace=error_trace))
-        else:
-            for start in range(0, iterations, every):
-                chunk = steps[start:start + every]
-                traces.append(self._plan.iterate(chunk, error_trace=error_trace))
-                done = start + len(chunk)
-                if done % every == 0:
-                    self.ctx.sync()
-                    print("  iteration %s" % done)
-                    sys.stdout.flush()
+        resident_before = self._plan.resident_counts()[0]
+        for start, stop in zip(cuts[:-1], cuts[1:]):
+            if start in debug:
+                self._debug_snapshots(start, algs[start], "before")
+            traces.append(self._plan.iterate(steps[start:stop], error_trace=error_trace))
+            if start in debug:
+                self._debug_snapshots(start, algs[start], "after")
+            if every is not None and stop % every == 0:
+                self.ctx.sync()
+                print("  iteration %s" % stop)
+                sys.stdout.flush()
+        if self._plan.resident_counts()[0] != resident_before:
+            # resident launches wait at grid barriers: raise if one gave up
+            # (the reference's iterate blocks on its kernels, gpu.py:349)
+            self._plan.check()
         if error_trace and traces:
             self._trace = torch.cat([t[:, 0] for t in traces if t is not None])
         else:

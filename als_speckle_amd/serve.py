@@ -192,7 +192,10 @@ class _Chunk:
     def __init__(self, cls, cid, slots):
         self.cls, self.cid, self.slots = cls, cid, slots
         self.n = cls.n
-        self.plan = Plan(cls.ctx, cls.n, batch=slots, shared=False, tile=cls.tile, ept=cls.ept)
+        # per-pass launches: the size classes run on concurrent streams, and a
+        # resident launch (cooperative, the whole device) must not overlap another
+        self.plan = Plan(cls.ctx, cls.n, batch=slots, shared=False, tile=cls.tile, ept=cls.ept,
+                         resident=2)
         self.free = list(range(slots))
         self.active = {}          # slot -> [request, steps, pos, trace chunks, admitted]
         self.event = None

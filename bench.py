@@ -19,7 +19,9 @@ Reported beside it in the same JSON line:
   batched       64 trials of one 512^2 frame (RAAR beta 0.87 and DM beta 0.7,
                 500 iterations each, aligned and averaged on the GPU), and 16
                 trials of the headline frame and schedule in one plan
-                (recons/s and the ratio to the single-reconstruction form)
+                (recons/s and the ratio to the single-reconstruction form),
+                and uneven frame sizes (64 x 256^2, 16 x 512^2, 4 x 1024^2)
+                in one merged launch per pass against per-plan launches
   serving       Poisson arrivals of mixed sizes {256: 40 %, 512: 30 %,
                 1024: 20 %, 2048: 10 %}, 500 HIO/ER iterations with shrinkwrap
                 each, continuous batching under an HBM budget: p50/p99
@@ -64,8 +66,8 @@ def parse():
     ap.add_argument("--requests", type=int, default=2000)
     ap.add_argument("--budget-gb", type=float, default=48.0)
     ap.add_argument("--kernel-probe", action="store_true",
-                    help="only a 256 MiB device copy (HBM counter calibration) and 20 "
-                         "iterations of the headline frame, for rocprofv3 --pmc runs")
+                    help="only a 256 MiB device copy (HBM counter calibration) and the "
+                         "iterations of one workload (--probe-frame), for rocprofv3 --pmc runs")
     ap.add_argument("--probe-frame", default="single_1024",
                     choices=("single_1024", "large_4096", "trials_512", "trials_1024"),
                     help="the workload of --kernel-probe: one 1024^2 or 4096^2 frame (headline "
@@ -75,10 +77,12 @@ def parse():
 
 
 def kernel_probe(ctx, which="single_1024"):
-    """A copy of known size, then 20 iterations of one workload: the
-    headline schedule on one frame (the 1024^2 headline frame or the 4096^2
-    large frame) or on 16 trials of the 1024^2 frame, or RAAR (beta 0.87)
-    on 64 trials of the 512^2 frame sharing one modulus and support."""
+    """A copy of known size, then iterations of one workload: the whole
+    1000-iteration headline schedule on the 1024^2 headline frame (one
+    resident launch, the launch the single-frame bench times), 20 iterations
+    of that schedule on the 4096^2 large frame or on 16 trials of the 1024^2
+    frame, or 20 RAAR (beta 0.87) iterations on 64 trials of the 512^2 frame
+    sharing one modulus and support."""
     import torch
     from als_speckle_amd import _capi
     from als_speckle_amd.device import Plan
@@ -102,22 +106,27 @@ def kernel_probe(ctx, which="single_1024"):
         np.random.seed(1 + t)
         g[t] = (np.random.rand(n, n) + 1j * np.random.rand(n, n)).astype(np.complex64)
     plan.set_psi(torch.from_numpy(g if batch > 1 else g[0]).to(ctx.device))
+    iters = 1000 if which == "single_1024" else 20
     if which == "trials_512":
-        steps = [(_capi.RAAR, _capi.SEM_NUMPY, 0.87, 0.0, 0.0)] * 20
+        steps = [(_capi.RAAR, _capi.SEM_NUMPY, 0.87, 0.0, 0.0)] * iters
     else:
-        steps, _ = headline_steps(20)
+        steps, _ = headline_steps(iters)
     plan.iterate(steps)
     ctx.sync()
     print(json.dumps({"kernel_probe": True, "workload": which, "copy_bytes": nbytes, "copies": 3,
-                      "iterations": 20, "n": n, "frames": batch}), flush=True)
+                      "iterations": iters, "n": n, "frames": batch,
+                      "resident_launches": plan.resident_counts()[0]}), flush=True)
+
+
+KERNEL_NAMES = {"column": "colpass", "row": "rowpass", "resident": "resident"}
 
 
 def traffic_file(dom):
-    """The committed PMC summary of the dominant pass ("column" or "row") of
-    the 1024^2 headline frame: HBM bytes per dispatch from FETCH_SIZE and
-    WRITE_SIZE, scaled by the copy-kernel calibration."""
-    kname = "colpass" if dom == "column" else "rowpass"
-    return os.path.join(ROOT, "profiles", f"traffic_{kname}_1024.json")
+    """The committed PMC summary of the dominant kernel ("column" pass, "row"
+    pass or "resident" launch) of the 1024^2 headline frame: HBM bytes per
+    dispatch from FETCH_SIZE and WRITE_SIZE, scaled by the copy-kernel
+    calibration."""
+    return os.path.join(ROOT, "profiles", f"traffic_{KERNEL_NAMES[dom]}_1024.json")
 
 
 def measured_traffic(dom):
@@ -231,6 +240,26 @@ def bench_next_rows(ctx):
     return out
 
 
+def copy_bandwidth(ctx, nbytes=1 << 30, reps=10):
+    """A plain device copy of 1 GiB (k_copy16), read + write bytes per second:
+    what a streaming kernel reaches on this box, beside the 8 TB/s peak."""
+    import torch
+    src = torch.ones(nbytes // 4, dtype=torch.float32, device=ctx.device)
+    dst = torch.empty_like(src)
+    ctx.copy_bytes(dst, src)
+    ctx.sync()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(ctx.stream)
+    for _ in range(reps):
+        ctx.copy_bytes(dst, src)
+    e1.record(ctx.stream)
+    e1.synchronize()
+    dt = e0.elapsed_time(e1) / 1e3 / reps
+    assert torch.equal(dst[:1024], src[:1024]) and torch.equal(dst[-1024:], src[-1024:])
+    return {"bytes": nbytes, "tb_per_s": round(2 * nbytes / dt / 1e12, 3),
+            "frac_of_peak": round(2 * nbytes / dt / HBM_PEAK, 4)}
+
+
 def cpu_baseline(frame, iters):
     """Restated reference numpy path (oracle/ref_numpy.py) on a bounded sample."""
     from oracle.ref_numpy import RefNumpyPhasing
@@ -273,42 +302,60 @@ def bench_single(a):
     err = torch.empty((iters, 1), dtype=torch.float64, device=dev)
     torch.cuda.synchronize()
     steps, nsw = headline_steps(iters)
+    # the library's default for one 1024^2 frame (resident launches), and the
+    # one-launch-per-pass form, also replayed as a captured HIP graph (graph
+    # capture runs the per-pass form)
     plan = Plan(ctx, n, batch=1, shared=True)
-    plan.set_modulus(m_d)
+    plan_pp = Plan(ctx, n, batch=1, shared=True, resident=2)
+    for p in (plan, plan_pp):
+        p.set_modulus(m_d)
 
-    def reconstruction(slot):
-        plan.set_support(s_d)
-        plan.set_psi(g_d, mult_support=True)
-        plan.iterate(steps, err_out=err)
-        plan.copy_roi(0, savebuf, slot, r0, c0, r1 - r0, c1 - c0)
+    def reconstruction(slot, p=plan):
+        p.set_support(s_d)
+        p.set_psi(g_d, mult_support=True)
+        p.iterate(steps, err_out=err)
+        p.copy_roi(0, savebuf, slot, r0, c0, r1 - r0, c1 - c0)
+
+    def timed(fn):
+        ctx.sync()
+        t0 = time.perf_counter()
+        for k in range(a.steps):
+            fn(a.warmup + k)
+        ctx.sync()
+        return time.perf_counter() - t0, float(err[-1, 0].item())
 
     with torch.cuda.stream(stream):
         for w in range(a.warmup):
             reconstruction(w)
-        ctx.sync()
-        t0 = time.perf_counter()
-        for k in range(a.steps):
-            reconstruction(a.warmup + k)
-        ctx.sync()
-        dt_eager = time.perf_counter() - t0
-        final_err = float(err[-1, 0].item())
-
-        graph = plan.capture(lambda: reconstruction(a.steps + a.warmup + 1))
+            reconstruction(w, plan_pp)
+        dt_eager, final_err = timed(reconstruction)
+        dt_pp, pp_err = timed(lambda k: reconstruction(k, plan_pp))
+        graph = plan_pp.capture(lambda: reconstruction(a.steps + a.warmup + 1, plan_pp))
         graph.launch()
-        ctx.sync()
-        t0 = time.perf_counter()
-        for k in range(a.steps):
-            graph.launch()
-        ctx.sync()
-        dt_graph = time.perf_counter() - t0
-        graph_err = float(err[-1, 0].item())
+        dt_graph, graph_err = timed(lambda k: graph.launch())
         graph.close()
 
+        # per-launch timing of one more reconstruction of each form
+        la0, st0 = plan.resident_counts()
+        fu0, _ = plan.sw_counts()
         plan.profile(True)
         reconstruction(a.warmup + a.steps)
-        ncol, tcol = plan.profile_read(0)
-        nrow, trow = plan.profile_read(1)
+        nres, tres = plan.profile_read(2)
         plan.profile(False)
+        la1, st1 = plan.resident_counts()
+        fu1, _ = plan.sw_counts()
+        plan_pp.profile(True)
+        reconstruction(a.warmup + a.steps, plan_pp)
+        ncol, tcol = plan_pp.profile_read(0)
+        nrow, trow = plan_pp.profile_read(1)
+        plan_pp.profile(False)
+        res = None
+        if nres and la1 > la0:
+            # the resident launch's algorithmic bytes: the survey model's
+            # bytes of the steps it ran and of the shrinkwraps inside it
+            res_steps, res_sw = (st1 - st0) / (la1 - la0), (fu1 - fu0) / (la1 - la0)
+            res = {"avg": tres / nres * 1e-3, "steps": res_steps, "shrinkwraps": res_sw,
+                   "bytes": int(res_steps * (B_COL + B_ROW) * n * n + res_sw * B_SHRINK * n * n)}
 
         # equal final error: the same schedule from the oracle fixture's
         # random starts; the error of each final iterate is the error trace
@@ -328,13 +375,18 @@ def bench_single(a):
                                             "8 random starts, 1000-iteration headline schedule")
     avg = {"column": tcol / ncol * 1e-3, "row": trow / nrow * 1e-3}
     byts = {"column": B_COL * n * n, "row": B_ROW * n * n}
-    dom = max(avg, key=lambda k: avg[k])
+    forms = {"resident": dt_eager, "per_pass": dt_pp, "per_pass_graph": dt_graph}
+    best = min(forms, key=forms.get)
+    if best == "resident" and res is not None:
+        # one launch runs the whole reconstruction: it is the dominant kernel
+        avg["resident"], byts["resident"] = res["avg"], res["bytes"]
+        dom = "resident"
+    else:
+        dom = max(("column", "row"), key=lambda k: avg[k])
     model = iters * (B_COL + B_ROW) * n * n + nsw * B_SHRINK * n * n + B_ONEOFF * n * n
-    assert graph_err == final_err, "graph replay differs from the eager reconstruction"
-    dt = min(dt_eager, dt_graph)
-    return dict(frame=frame, n=n, iters=iters, dt=dt, dt_eager=dt_eager, dt_graph=dt_graph,
-                launch="hipGraph" if dt_graph < dt_eager else "eager",
-                final_err=final_err, avg=avg, byts=byts, dom=dom, model=model,
+    assert graph_err == final_err == pp_err, "the launch forms must give the same reconstruction"
+    return dict(frame=frame, n=n, iters=iters, dt=forms[best], forms=forms, launch=best,
+                final_err=final_err, avg=avg, byts=byts, dom=dom, model=model, res=res,
                 vs_oracle=vs_oracle)
 
 
@@ -414,6 +466,85 @@ def bench_trials_512(ctx):
     out["raar"].update({"column_pass_us": round(col_us, 2), "row_pass_us": round(row_us, 2),
                         "column_pass_hbm_frac": round(col_b / (col_us * 1e-6) / HBM_PEAK, 4),
                         "row_pass_hbm_frac": round(row_b / (row_us * 1e-6) / HBM_PEAK, 4)})
+    return out
+
+
+def bench_mixed_sizes(ctx, iters=100):
+    """Uneven frame sizes in one launch: 64 frames of 256^2, 16 of 512^2 and
+    4 of 1024^2 (the same pixel count per size), HIO beta 0.85 with the
+    error trace, as one merged column-pass and row-pass launch per step for
+    all three plans (spk_group_iterate), against the same three plans
+    iterated by their own calls one after another on one stream and on three
+    concurrent streams.  Same kernels' arithmetic in every form (the final
+    iterates are compared bitwise)."""
+    import torch
+    from als_speckle_amd import _capi, synth
+    from als_speckle_amd.device import Context, Group, Plan, group_shape
+    mix = [(256, 64), (512, 16), (1024, 4)]
+    dev = ctx.device
+    steps = [(_capi.HIO, _capi.SEM_NUMPY, 0.85, 0.0, 0.0)] * iters
+    data = {}
+    for n, b in mix:
+        f = synth.make_frame(n, 7, shape="polygon", fill=0.14, photons=1e7)
+        rng = np.random.default_rng(n)
+        g = (rng.random((b, n, n)) + 1j * rng.random((b, n, n))).astype(np.complex64)
+        data[n] = (torch.from_numpy(np.stack([f.modulus] * b)).to(dev),
+                   torch.from_numpy(np.stack([f.support] * b)).to(dev),
+                   torch.from_numpy(g).to(dev))
+
+    def make(c, n, b):
+        tile, ept = group_shape(n)
+        p = Plan(c, n, batch=b, shared=False, tile=tile, ept=ept, resident=2)
+        m, s, g = data[n]
+        p.set_modulus(m)
+        p.set_support(s)
+        p.set_psi(g)
+        return p
+
+    def reset(plans):
+        for p in plans:
+            p.set_support(data[p.n][1])
+            p.set_psi(data[p.n][2])
+
+    grp = Group(ctx)
+    gplans = [make(ctx, n, b) for n, b in mix]
+    streams = [torch.cuda.Stream(device=dev) for _ in mix]
+    sctx = [Context(ctx.index, stream=st) for st in streams]
+    splans = [make(c, n, b) for c, (n, b) in zip(sctx, mix)]
+    forms = {
+        "merged": lambda: grp.iterate(gplans, [steps] * len(mix)),
+        "one_stream": lambda: [p.iterate(steps) for p in gplans],
+        "three_streams": lambda: [p.iterate(steps) for p in splans],
+    }
+    out = {}
+    finals = {}
+    for name, fn in forms.items():
+        plans = splans if name == "three_streams" else gplans
+        best = None
+        for rep in range(3):
+            reset(plans)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            fn()
+            torch.cuda.synchronize()
+            dt = time.perf_counter() - t0
+            best = dt if best is None else min(best, dt)
+        finals[name] = [p.get_psi().cpu().numpy() for p in plans]
+        px = sum(n * n * b for n, b in mix)
+        out[name] = {"us_per_iteration": round(best / iters * 1e6, 2),
+                     "frame_iterations_per_s": round(sum(b for _, b in mix) * iters / best, 1),
+                     "model_hbm_frac": round((B_COL + B_ROW) * px * iters / best / HBM_PEAK, 4)}
+    out["bitwise_equal_forms"] = all(
+        np.array_equal(a, b) for name in ("one_stream", "three_streams")
+        for a, b in zip(finals["merged"], finals[name]))
+    out["frames"] = {str(n): b for n, b in mix}
+    out["iterations"] = iters
+    out["launches_merged"] = grp.counts()[0]
+    for p in gplans + splans:
+        p.close()
+    grp.close()
+    for c in sctx:
+        c.close()
     return out
 
 
@@ -632,8 +763,9 @@ def main():
     d_avg, d_bytes = s["avg"][dom], s["byts"][dom]
     achieved = d_bytes / d_avg
 
-    kname = "colpass" if dom == "column" else "rowpass"
+    kname = KERNEL_NAMES[dom]
     traffic = measured_traffic(dom)
+    col_bytes, col_us = B_COL * n * n, s["avg"]["column"]
 
     out = {
         "metric": "HIO iters/s at 1024^2 c64",
@@ -654,14 +786,19 @@ def main():
                         "tau 0.2), FP64 error trace, polygon support 15% fill, Poisson 1e8",
             "n": n, "iterations": iters, "recons_per_s": round(plain_rps, 3),
             "launch": s["launch"],
-            "eager_iterations_per_s": round(iters * a.steps / s["dt_eager"], 1),
-            "graph_iterations_per_s": round(iters * a.steps / s["dt_graph"], 1),
+            "iterations_per_s_by_form": {k: round(iters * a.steps / v, 1)
+                                         for k, v in s["forms"].items()},
             "final_modulus_error": s["final_err"],
             "final_error_vs_oracle": s["vs_oracle"],
             "model_bytes_per_recon": s["model"],
             "model_hbm_frac_whole_recon": round(s["model"] * a.steps / dt / HBM_PEAK, 4),
             "column_pass_us": round(s["avg"]["column"] * 1e6, 3),
             "row_pass_us": round(s["avg"]["row"] * 1e6, 3),
+            # the FFT + projection (column) pass of the one-launch-per-pass
+            # form: 20 B/px per launch
+            "fft_projection_pass": {"us": round(col_us * 1e6, 3),
+                                    "achieved_gb_s": round(col_bytes / col_us / 1e9, 1),
+                                    "frac": round(col_bytes / col_us / HBM_PEAK, 4)},
         },
         "roofline": {
             "bound": "hbm", "kernel": f"k_{kname}<1024>",
@@ -671,6 +808,11 @@ def main():
             "avg_launch_us": round(d_avg * 1e6, 3),
         },
     }
+    if s["res"] is not None:
+        out["roofline"]["resident_launch"] = {
+            "steps_per_launch": s["res"]["steps"], "shrinkwraps_per_launch": s["res"]["shrinkwraps"],
+            "us_per_step": round(s["res"]["avg"] * 1e6 / s["res"]["steps"], 3)}
+    out["hbm_copy_reference"] = copy_bandwidth(ctx)
     if not a.no_batched:
         log("batched trials 64 x 512^2 (RAAR, DM)")
         out["batched"] = {"trials_512": bench_trials_512(ctx)}
@@ -678,6 +820,8 @@ def main():
         b16 = bench_batched_1024(ctx, s["frame"])
         b16["speedup_vs_single"] = round(b16["recons_per_s"] / plain_rps, 3)
         out["batched"]["trials_1024"] = b16
+        log("uneven frame sizes in one launch (256^2, 512^2, 1024^2)")
+        out["batched"]["mixed_sizes"] = bench_mixed_sizes(ctx)
     if not a.no_large:
         log("large 4096^2 reconstruction")
         out["large_4096"] = bench_large_4096(ctx)

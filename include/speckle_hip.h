@@ -68,7 +68,20 @@ typedef struct spk_tune {
   int32_t sw_separate;/* 1: run each shrinkwrap as its own kernels (default:
                          fused into the passes of its step and the next
                          where the blur radius is <= 24 px, i.e. sigma <= 4) */
-  int32_t reserved[4];
+  int32_t resident;   /* one frame of 256^2..1024^2 in tile 4, 8 values per
+                         thread: runs of non-DM steps run as one cooperative
+                         resident launch each (the frame's modulus, support
+                         and iterate held on chip, Phi exchanged through a
+                         grid barrier, shrinkwraps inside the launch); 0 or
+                         1: where built (the default), 2: never.  A resident
+                         launch needs the whole device: no other cooperative
+                         launch may run beside it (spk_plan_check reports a
+                         barrier that gave up waiting)                       */
+  int32_t exchange;   /* resident launches: how Phi crosses the barrier,
+                         1 plain + device-scope fences, 2 device-scope
+                         accesses + fences, 3 device-scope accesses without
+                         fences (the default); 0 = default                   */
+  int32_t reserved[2];
 } spk_tune;
 
 /* -------------------------------------------------------------------------
@@ -98,9 +111,9 @@ int spk_plan_info(const spk_plan* plan, int* tile, int* ept, int* persist);
 int spk_plan_sw_counts(const spk_plan* plan, long long* fused, long long* separate);
 
 /* Waits for the plan's queued work and reports whether a grid barrier of a
- * persistent iteration block gave up waiting (SPK_EHIP; the flag is then
- * cleared).  No counterpart in the reference, whose kernels each end with a
- * blocking wait (gpu.py:349). */
+ * resident launch or a persistent iteration block gave up waiting (SPK_EHIP;
+ * the flag is then cleared).  No counterpart in the reference, whose
+ * kernels each end with a blocking wait (gpu.py:349). */
 int spk_plan_check(spk_plan* plan);
 
 /* -------------------------------------------------------------------------
@@ -119,7 +132,9 @@ int spk_plan_destroy(spk_plan* plan);
  * allocated their buffers (for HBM-budget admission).  options: bitwise OR of
  * SPK_PB_* (1 keeps the meaning of the former with_dm argument). */
 enum {
-  SPK_PB_DM = 1,         /* DM steps: the modified estimate phib            */
+  SPK_PB_DM = 1,         /* DM steps: the modified estimate phib (with an
+                            ipsf or spectrum also the work buffers of DM's
+                            second projection)                              */
   SPK_PB_SHRINKWRAP = 2, /* in-loop shrinkwrap: blur work maps, band maxima */
   SPK_PB_IPSF = 4,       /* spk_set_ipsf: the ipsf and per-frame work frames */
   SPK_PB_SPECTRUM = 8,   /* spk_set_spectrum: per-frame intensity maps       */
@@ -152,7 +167,8 @@ int spk_get_psi(spk_plan* plan, int frame, float* psi_c64);
  * of the point-spread function, corner-centred), shared by the plan's
  * frames.  While set, every iteration's modulus replacement divides by
  * div = sqrt(|ifft2(fft2(|Psi|^2) * ipsf)|) instead of |Psi| and the error
- * trace compares div with m.  NULL switches it off.  Not combined with DM. */
+ * trace compares div with m.  NULL switches it off.  DM steps project their
+ * modified estimate with its own partially coherent divisor. */
 int spk_set_ipsf(spk_plan* plan, const float* ipsf_c64);
 
 /* Longitudinal (spectral) partial coherence (phasing.py:430-448
@@ -162,7 +178,7 @@ int spk_set_ipsf(spk_plan* plan, const float* ipsf_c64);
  * divisor (|Psi|^2, or the ipsf-blurred intensity when an ipsf is also set)
  * is replaced by sum_k w_k * (its bilinear interpolation at coordinates /
  * r_k) before the square root.  rescales = NULL or K = 0 switches it off.
- * Not combined with DM. */
+ * DM steps project their modified estimate with its own rescaled divisor. */
 int spk_set_spectrum(spk_plan* plan, const float* rescales, const float* weights, int K);
 
 /* -------------------------------------------------------------------------
@@ -178,6 +194,32 @@ int spk_set_spectrum(spk_plan* plan, const float* rescales, const float* weights
  * ---------------------------------------------------------------------- */
 int spk_iterate(spk_plan* plan, const spk_step* steps, int nsteps,
                 int per_frame, double* err_trace);
+
+/* -------------------------------------------------------------------------
+ * Frames of different sizes in one launch.  spk_group_iterate runs, for each
+ * of `count` plans (at most 16, one context), the spk_iterate call
+ * (steps[m], nsteps, per_frame[m], err_trace[m]; per_frame and err_trace
+ * may be NULL), with the single-projection passes of every plan merged per
+ * step into one column-pass and one row-pass launch over all their active
+ * frames.  Results equal each plan's own spk_iterate call bitwise.  Steps
+ * with DM work or a partially coherent projection run with the plan's own
+ * launches in the same call; shrinkwraps as in spk_iterate.  Member plans
+ * hold frames of 256^2, 512^2 or 1024^2 in the shape spk_group_shape gives
+ * (the layout tile that makes every size's workgroup 256 threads), are
+ * not persistent and are not being captured.
+ * Pattern: the reference runs one frame size per phasing object and its
+ * trials one after another (phasing.py:353, examples/basic_phasing_example.py:27-30);
+ * no reference counterpart.
+ * ---------------------------------------------------------------------- */
+typedef struct spk_group spk_group;
+int spk_group_shape(int n, int* tile, int* ept);
+int spk_group_create(spk_ctx* ctx, spk_group** out);
+int spk_group_destroy(spk_group* group);
+int spk_group_iterate(spk_group* group, spk_plan* const* plans, int count,
+                      const spk_step* const* steps, int nsteps, const int* per_frame,
+                      double* const* err_trace);
+/* Merged launches made so far and the member passes they covered. */
+int spk_group_counts(const spk_group* group, long long* launches, long long* member_passes);
 
 /* Shrinkwrap now, on every frame: support *= [G_sigma (*) |psi| > tau*max].
  * Build-defined in-loop form of refine_support (phasing.py:1747-1815). */
@@ -308,12 +350,17 @@ int spk_copy_bytes(spk_ctx* ctx, void* dst, const void* src, size_t bytes);
 /* Per-pass timing inside spk_iterate: with enable != 0 every column-pass and
  * row-pass launch of this plan is made with hipExtLaunchKernelGGL and a
  * start/stop HIP event pair recorded by the dispatch itself (the kernel's own
- * execution interval).  spk_plan_profile_read synchronises, returns the number
- * of launches and their summed milliseconds for pass 0 (column) or 1 (row),
- * and clears that pass's record. */
+ * execution interval); resident launches get an event pair around their
+ * launch on the stream.  spk_plan_profile_read synchronises, returns the
+ * number of launches and their summed milliseconds for pass 0 (column), 1
+ * (row) or 2 (resident launches), and clears that record. */
 int spk_plan_profile(spk_plan* plan, int enable);
 int spk_plan_profile_read(spk_plan* plan, int pass, int* launches,
                           double* total_ms);
+
+/* The plan's resident launches (spk_tune.resident) so far: launches and the
+ * steps they ran.  No reference counterpart. */
+int spk_plan_resident_counts(const spk_plan* plan, long long* launches, long long* steps);
 
 /* Launch `reps` column passes (which = 0), row passes (1) or both (2) of a
  * single-projection step on the plan's current state, timed with HIP events
