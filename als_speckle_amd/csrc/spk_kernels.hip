@@ -41,12 +41,17 @@ namespace spk {
 // ---------------------------------------------------------------------------
 constexpr int kLdsBytes = 160 * 1024;
 
+// the count of workgroups per CU up to which double buffering must keep the
+// single-buffer occupancy (a build define for layout experiments)
+#ifndef SPK_DB_WGS
+#define SPK_DB_WGS 4
+#endif
 template <int N, int TT>
 constexpr bool db_pays() {
   constexpr int buf = lds_elems<N, TT>();
   constexpr int one = kLdsBytes / ((buf + N) * 8);
   constexpr int two = kLdsBytes / ((2 * buf + N) * 8);
-  return two >= 1 && two >= (one < 4 ? one : 4);
+  return two >= 1 && two >= (one < SPK_DB_WGS ? one : SPK_DB_WGS);
 }
 
 template <int N, int TT>

@@ -32,7 +32,7 @@ import numpy as np
 import torch
 
 from . import _capi
-from .device import Context, Plan, plan_bytes, require_gpu
+from .device import Context, Group, Plan, group_shape, plan_bytes, require_gpu
 from .phasing import expand_order, shrinkwrap_schedule
 
 __all__ = ["Request", "Result", "Server", "block_table", "plan_admission", "chunk_shapes"]
@@ -40,6 +40,13 @@ __all__ = ["Request", "Result", "Server", "block_table", "plan_admission", "chun
 # plan options a serving chunk is sized for: in-loop shrinkwrap work maps and
 # the error partials of one launch round (DM adds its modified estimate)
 CHUNK_OPTIONS = _capi.PB_SHRINKWRAP | _capi.PB_TRACE
+
+# frame sizes the mixed-size launch takes (spk_group_shape)
+MERGED_SIZES = (256, 512, 1024)
+
+
+def group_tile_ok(n: int) -> bool:
+    return int(n) in MERGED_SIZES
 
 
 @dataclasses.dataclass
@@ -214,19 +221,28 @@ class _Chunk:
             self.plan.set_psi(_as_dev(req.guess, dev, torch.complex64), slot, mult_support=True)
         self.active[slot] = [req, _steps_for(req), 0, [], now]
 
+    def table(self, block: int):
+        """(per-frame step table, steps each slot takes) of the next block."""
+        return block_table({f: (st[1], st[2]) for f, st in self.active.items()}, block,
+                           self.slots)
+
+    def launched(self, err, takes, event):
+        """Account a queued block: its error trace and the steps each slot took."""
+        for f, st in self.active.items():
+            st[3].append((err, f, takes[f]))
+            st[2] += takes[f]
+        self.event = event
+
     def launch(self, block: int):
         """Queue the next `block` iterations of every active slot."""
         if not self.active:
             return False
-        table, takes = block_table({f: (st[1], st[2]) for f, st in self.active.items()},
-                                   block, self.slots)
+        table, takes = self.table(block)
         with torch.cuda.stream(self.cls.stream):
             err = self.plan.iterate(table, per_frame=True)
-        for f, st in self.active.items():
-            st[3].append((err, f, takes[f]))
-            st[2] += takes[f]
-        self.event = torch.cuda.Event()
-        self.event.record(self.cls.stream)
+        event = torch.cuda.Event()
+        event.record(self.cls.stream)
+        self.launched(err, takes, event)
         return True
 
     def retire(self, now: float):
@@ -256,13 +272,22 @@ class _Chunk:
 
 
 class _Class:
-    """One frame size: its stream, context, kernel shape and chunks."""
+    """One frame size: its stream, context, kernel shape and chunks.  A class
+    of the merged set shares the merged stream and context and takes the
+    mixed-size launch shape."""
 
-    def __init__(self, device_index, n, chunk_slots, autotune=False):
+    def __init__(self, device_index, n, chunk_slots, autotune=False, merged=None):
         self.n, self.chunk_slots = n, chunk_slots
+        self.merged = merged is not None
+        self.tile = self.ept = 0
+        if self.merged:
+            self.stream, self.ctx = merged
+            self.tile, self.ept = group_shape(n)
+            self.chunks = []
+            self.next_id = 0
+            return
         self.stream = torch.cuda.Stream(device=device_index)
         self.ctx = Context(device_index, stream=self.stream)
-        self.tile = self.ept = 0
         if autotune:
             from .tune import autotune as tune_fn
             with torch.cuda.stream(self.stream):
@@ -299,11 +324,14 @@ class Server:
     shrinkwraps mostly fall inside a launch, where they run fused into the
     passes (a shrinkwrap on the last step of a launch runs as stand-alone
     kernels).  autotune: pick each size's FFT layout and radix schedule by
-    measurement (tune.autotune).
+    measurement (tune.autotune).  merge_sizes: the classes of 256^2, 512^2
+    and 1024^2 share one stream and run each block of all their chunks as
+    merged launches (spk_group_iterate: one launch per pass for every size);
+    other sizes keep their own streams.  Results are the same either way.
     """
 
     def __init__(self, device=0, slots=None, budget_bytes=None, mix=None, sizes=None,
-                 block=21, autotune=False, chunk_slots=None):
+                 block=21, autotune=False, chunk_slots=None, merge_sizes=False):
         require_gpu()
         self.device_index = int(device)
         self.block = int(block)
@@ -323,7 +351,18 @@ class Server:
                 cb = {n: plan_bytes(n, cs[n], False, CHUNK_OPTIONS) for n in cs}
             self.chunk_slots, self.chunk_bytes = cs, cb
             self.growable = True
-        self.classes = {n: _Class(self.device_index, n, k, autotune)
+        # merge_sizes: the classes of the mixed-size launch sizes share one
+        # stream, and every block of their chunks runs as one
+        # spk_group_iterate call (one launch per pass for all of them)
+        self.group = None
+        merged = None
+        if merge_sizes and sum(group_tile_ok(n) for n in self.chunk_slots) >= 2:
+            st = torch.cuda.Stream(device=self.device_index)
+            self._mctx = Context(self.device_index, stream=st)
+            self.group = Group(self._mctx)
+            merged = (st, self._mctx)
+        self.classes = {n: _Class(self.device_index, n, k, autotune,
+                                  merged if (merged and group_tile_ok(n)) else None)
                         for n, k in sorted(self.chunk_slots.items())}
         if not self.growable:
             for c in self.classes.values():
@@ -422,11 +461,39 @@ class Server:
             ch.admit(r, now)
 
     # -- the loop ------------------------------------------------------------
+    def _merged_chunks(self):
+        return [ch for c in self.classes.values() if c.merged for ch in c.chunks]
+
+    def _own_chunks(self):
+        return [ch for c in self.classes.values() if not c.merged for ch in c.chunks]
+
+    def _launch_merged(self):
+        """One block of every busy chunk of the merged classes, as
+        spk_group_iterate calls of at most 16 plans; the chunks share the
+        completion event."""
+        chunks = [ch for ch in self._merged_chunks() if ch.active]
+        if not chunks:
+            return []
+        tabs = [ch.table(self.block) for ch in chunks]
+        errs = []
+        with torch.cuda.stream(self._mctx.stream):
+            for i in range(0, len(chunks), 16):
+                part = chunks[i:i + 16]
+                errs += self.group.iterate([ch.plan for ch in part],
+                                           [t[0] for t in tabs[i:i + 16]],
+                                           per_frame=[True] * len(part))
+        event = torch.cuda.Event()
+        event.record(self._mctx.stream)
+        for ch, (_, takes), err in zip(chunks, tabs, errs):
+            ch.launched(err, takes, event)
+        return chunks
+
     def tick(self):
         """Admit, launch one block on every busy chunk, wait, retire (lockstep)."""
         now = self.now()
         self._admit(now)
-        launched = [ch for ch in self.chunks() if ch.launch(self.block)]
+        launched = [ch for ch in self._own_chunks() if ch.launch(self.block)]
+        launched += self._launch_merged()
         for ch in launched:
             ch.event.synchronize()
         now = self.now()
@@ -446,7 +513,7 @@ class Server:
         another.  Returns True if anything progressed."""
         progressed = False
         self._admit(self.now())
-        for ch in self.chunks():
+        for ch in self._own_chunks():
             if ch.event is not None:
                 if not ch.event.query():
                     continue
@@ -456,6 +523,20 @@ class Server:
                 self._admit(now)
                 progressed = True
             if ch.active and ch.launch(self.block):
+                progressed = True
+        if self.group is not None:
+            merged = self._merged_chunks()
+            pending = [ch for ch in merged if ch.event is not None]
+            if pending and not pending[0].event.query():
+                return progressed
+            if pending:
+                now = self.now()
+                for ch in pending:
+                    self.results.extend(ch.retire(now))
+                    ch.event = None
+                self._admit(now)
+                progressed = True
+            if self._launch_merged():
                 progressed = True
         return progressed
 
@@ -487,3 +568,7 @@ class Server:
             ch.close()
         for c in self.classes.values():
             c.chunks = []
+        if self.group is not None:
+            self.group.close()
+            self._mctx.close()
+            self.group = None

@@ -529,14 +529,16 @@ def bench_mixed_sizes(ctx, iters=100):
             torch.cuda.synchronize()
             dt = time.perf_counter() - t0
             best = dt if best is None else min(best, dt)
-        finals[name] = [p.get_psi().cpu().numpy() for p in plans]
+        outs = [p.get_psi() for p in plans]
+        torch.cuda.synchronize()          # the plans' streams wrote them
+        finals[name] = [o.cpu().numpy() for o in outs]
         px = sum(n * n * b for n, b in mix)
         out[name] = {"us_per_iteration": round(best / iters * 1e6, 2),
                      "frame_iterations_per_s": round(sum(b for _, b in mix) * iters / best, 1),
                      "model_hbm_frac": round((B_COL + B_ROW) * px * iters / best / HBM_PEAK, 4)}
-    out["bitwise_equal_forms"] = all(
-        np.array_equal(a, b) for name in ("one_stream", "three_streams")
-        for a, b in zip(finals["merged"], finals[name]))
+    out["bitwise_equal_to_merged"] = {
+        name: all(np.array_equal(a, b) for a, b in zip(finals["merged"], finals[name]))
+        for name in ("one_stream", "three_streams")}
     out["frames"] = {str(n): b for n, b in mix}
     out["iterations"] = iters
     out["launches_merged"] = grp.counts()[0]
@@ -717,11 +719,14 @@ def bench_serving(a):
     reqs = make_requests(a.requests, arrivals)
     out = {"arrival_rate_per_s": round(lam, 2), "capacity_req_per_s": round(cap, 2),
            "hbm_budget_gb": a.budget_gb}
-    # the same arrivals through the budget-shared form and the static form
-    # (one fixed pool per size, sized from the budget by expected work)
-    for form in ("shared_budget", "static_pools"):
+    # the same arrivals through the budget-shared form, the same with the
+    # 256/512/1024 classes in merged launches, and the static form (one
+    # fixed pool per size, sized from the budget by expected work)
+    for form in ("shared_budget", "merged_sizes", "static_pools"):
         if form == "shared_budget":
             srv = Server(device=0, budget_bytes=budget, mix=mix)
+        elif form == "merged_sizes":
+            srv = Server(device=0, budget_bytes=budget, mix=mix, merge_sizes=True)
         else:
             srv = Server(device=0, slots=Server.size_slots(budget, mix))
         torch.cuda.synchronize()

@@ -224,3 +224,26 @@ def test_serving_mix_sizes_lockstep_against_the_oracle():
     for r in reqs:
         assert srv.cancel(r.rid)
     srv.close()
+
+
+def test_merged_size_classes_same_alone_and_in_mix():
+    """merge_sizes: the 256^2, 512^2 and 1024^2 classes run every block as
+    merged launches (spk_group_iterate); a request's result does not depend
+    on what else is resident, and the merged launches are used."""
+    print("[serve merged sizes]", flush=True)
+    specs = [(256, 21, 0.0), (512, 22, 0.0), (256, 23, 0.0), (1024, 24, 0.0), (512, 25, 0.0),
+             (256, 26, 0.0)]
+    slots = {256: 2, 512: 1, 1024: 1}
+    mix = Server(slots=slots, block=21, merge_sizes=True)
+    assert mix.group is not None
+    res = {r.rid: r for r in mix.run(_requests(specs))}
+    assert sorted(res) == list(range(len(specs)))
+    assert mix.group.counts()[0] > 0 and mix.group.counts()[1] > mix.group.counts()[0]
+    mix.close()
+    for rid in (1, 3, 5):
+        alone = Server(slots=slots, block=21, merge_sizes=True)
+        one = alone.run([_requests(specs)[rid]])[0]
+        assert np.array_equal(one.psi.cpu().numpy(), res[rid].psi.cpu().numpy())
+        assert np.array_equal(one.support.cpu().numpy(), res[rid].support.cpu().numpy())
+        assert np.array_equal(one.trace, res[rid].trace)
+        alone.close()
