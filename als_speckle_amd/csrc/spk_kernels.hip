@@ -41,23 +41,25 @@ namespace spk {
 // ---------------------------------------------------------------------------
 constexpr int kLdsBytes = 160 * 1024;
 
-// the count of workgroups per CU up to which double buffering must keep the
-// single-buffer occupancy (a build define for layout experiments)
-#ifndef SPK_DB_WGS
-#define SPK_DB_WGS 4
-#endif
-template <int N, int TT>
+// DW: the count of workgroups per CU up to which double buffering must keep
+// the single-buffer occupancy.  The row-direction kernels use 4; the column
+// passes use 8: at 512^2 x 64 frames (tile 4) one buffer admits 5 column
+// workgroups per CU instead of 4 and the pass takes 54.0 instead of 58.9 us,
+// while the row pass is 4.5 % faster with two buffers (measured on MI355X).
+constexpr int kRowDbWgs = 4;
+constexpr int kColDbWgs = 8;
+template <int N, int TT, int DW>
 constexpr bool db_pays() {
   constexpr int buf = lds_elems<N, TT>();
   constexpr int one = kLdsBytes / ((buf + N) * 8);
   constexpr int two = kLdsBytes / ((2 * buf + N) * 8);
-  return two >= 1 && two >= (one < SPK_DB_WGS ? one : SPK_DB_WGS);
+  return two >= 1 && two >= (one < DW ? one : DW);
 }
 
-template <int N, int TT>
+template <int N, int TT, int DW = kRowDbWgs>
 struct LdsPlan {
   static constexpr int BUF = lds_elems<N, TT>();
-  static constexpr bool DB = db_pays<N, TT>();
+  static constexpr bool DB = db_pays<N, TT, DW>();
   static constexpr int NBUF = DB ? 2 : 1;
   static constexpr bool FULL_TW = (NBUF * BUF + N) * 8 <= kLdsBytes;
   static constexpr int TW_ELEMS = FULL_TW ? N : (N >> 6) + 64;
@@ -70,9 +72,9 @@ struct LdsPlan {
 // its share before the data loads are issued (fetch), and writes it to LDS
 // once the data loads are in flight (put), so the table's latency overlaps
 // the data's.  No barrier here: stage 0 uses no twiddles and ends with one.
-template <int N, int TT, int NT>
+template <int N, int TT, int NT, int DW = kRowDbWgs>
 struct TwStage {
-  using L = LdsPlan<N, TT>;
+  using L = LdsPlan<N, TT, DW>;
   static constexpr int CNT = L::TW_ELEMS;
   static constexpr int PER = (CNT + NT - 1) / NT;
   c32 v[PER];
@@ -136,9 +138,9 @@ __device__ __forceinline__ float gld_nt(const float* base, unsigned i) {
       reinterpret_cast<const float*>(reinterpret_cast<const char*>(base) + i * 4u));
 }
 
-template <int N, int TT>
+template <int N, int TT, int DW = kRowDbWgs>
 __device__ __forceinline__ auto twiddle_view(const c32* twl) {
-  if constexpr (LdsPlan<N, TT>::FULL_TW) return TwStaged{twl};
+  if constexpr (LdsPlan<N, TT, DW>::FULL_TW) return TwStaged{twl};
   else return TwTwoLevel<N>{twl};
 }
 
@@ -147,18 +149,18 @@ __device__ __forceinline__ auto twiddle_view(const c32* twl) {
 // logic and every address computation live in spk_fft_core.hpp, shared with
 // the CPU emulation in the test suite; kernels add the barriers.
 // ---------------------------------------------------------------------------
-template <int N, int E, int TT, bool INV, int S, class TW>
+template <int N, int E, int TT, bool INV, int S, int DW, class TW>
 __device__ __forceinline__ void fft_stage(c32 (&x)[E], const int t, const int b,
                                           c32* __restrict__ lds, const TW& tw) {
   using P = RadixPlan<N, E>;
-  using L = LdsPlan<N, TT>;
+  using L = LdsPlan<N, TT, DW>;
   c32* buf = lds + (L::DB ? (S & 1) * L::BUF : 0);
   stage_butterflies<N, E, TT, INV, S>(x, t, b, buf, tw);
   if constexpr (S != P::kStages - 1) {
     __syncthreads();
     stage_gather<N, E, TT>(x, t, b, buf);
     if constexpr (!L::DB) __syncthreads();
-    fft_stage<N, E, TT, INV, S + 1>(x, t, b, lds, tw);
+    fft_stage<N, E, TT, INV, S + 1, DW>(x, t, b, lds, tw);
   }
 }
 
@@ -166,13 +168,13 @@ __device__ __forceinline__ void fft_stage(c32 (&x)[E], const int t, const int b,
 // buffering, an earlier transform's last exchange may still be read by
 // slower threads; a barrier is needed when that exchange used buffer 0,
 // which this transform's first stage writes.
-template <int N, int E, int TT, bool INV, bool FIRST, class TW>
+template <int N, int E, int TT, bool INV, bool FIRST, int DW = kRowDbWgs, class TW>
 __device__ __forceinline__ void fft_regs(c32 (&x)[E], int t, int b, c32* lds, const TW& tw) {
   using P = RadixPlan<N, E>;
-  using L = LdsPlan<N, TT>;
+  using L = LdsPlan<N, TT, DW>;
   if constexpr (!FIRST && L::DB && P::kStages >= 2 && ((P::kStages - 2) & 1) == 0)
     __syncthreads();
-  fft_stage<N, E, TT, INV, 0>(x, t, b, lds, tw);
+  fft_stage<N, E, TT, INV, 0, DW>(x, t, b, lds, tw);
 }
 
 // deterministic workgroup sum, result valid in thread 0.  Full wavefronts
@@ -271,9 +273,9 @@ template <int N, int TT>
 constexpr int sw_lds_floats() { return sw_stage_off<N, TT>() + N * TT; }
 // the pass kernels' LDS holds the lines and the staged outputs (else the
 // outputs go straight to global memory)
-template <int N, int TT>
+template <int N, int TT, int DW = kRowDbWgs>
 constexpr bool sw_stage_fits() {
-  return sw_lds_floats<N, TT>() * 4 <= LdsPlan<N, TT>::TOTAL * 8;
+  return sw_lds_floats<N, TT>() * 4 <= LdsPlan<N, TT, DW>::TOTAL * 8;
 }
 
 // Store the staged band (N x TT floats) as float4s: group v of four staged
@@ -399,12 +401,12 @@ __device__ __forceinline__ c32 update_value(const Coef& c, c32 pout, c32 pin, fl
 // (a one-item kernel); otherwise the table is already there (persistent
 // kernel).  errp: partial error sums [frame][band] (MODE 0, z = 0).
 // ---------------------------------------------------------------------------
-template <int N, int TT, int E, int MODE, bool OWN_TW>
+template <int N, int TT, int E, int MODE, bool OWN_TW, int DW = kColDbWgs>
 __device__ __forceinline__ void colpass_item(const PassArgs& a, int f, int band, int z,
                                              double* errp, c32* lds, bool swf = false) {
   constexpr int NB = N / TT;
   constexpr bool PROJ = MODE == 0 || MODE == 9 || MODE == 10;
-  using L = LdsPlan<N, TT>;
+  using L = LdsPlan<N, TT, DW>;
   const size_t fo = (size_t)f * N * N;
   // dual layout: read Phi column-band-major (contiguous), write it tiled for
   // the row pass; single layout: tiles in place
@@ -415,7 +417,7 @@ __device__ __forceinline__ void colpass_item(const PassArgs& a, int f, int band,
   constexpr int NT = (N / E) * TT;
 
   // every global load up front: twiddle table, data, modulus
-  TwStage<N, TT, NT> tws;
+  TwStage<N, TT, NT, DW> tws;
   if constexpr (OWN_TW) tws.fetch(a.tw);
   c32 x[E];
 #pragma unroll
@@ -438,13 +440,13 @@ __device__ __forceinline__ void colpass_item(const PassArgs& a, int f, int band,
   float rb[MODE == 9 ? E : 1];
   c32* twl = lds + L::NBUF * L::BUF;
   if constexpr (OWN_TW) tws.put(twl);
-  const auto tw = twiddle_view<N, TT>(twl);
+  const auto tw = twiddle_view<N, TT, DW>(twl);
 
   double err = 0.0;
   if constexpr (MODE == 3) {
-    fft_regs<N, E, TT, true, true>(x, t, b, lds, tw);
+    fft_regs<N, E, TT, true, true, DW>(x, t, b, lds, tw);
   } else {
-    fft_regs<N, E, TT, false, true>(x, t, b, lds, tw);
+    fft_regs<N, E, TT, false, true, DW>(x, t, b, lds, tw);
   }
   if constexpr (PROJ) {
     if constexpr (!PREFETCH) {
@@ -471,7 +473,7 @@ __device__ __forceinline__ void colpass_item(const PassArgs& a, int f, int band,
         for (int i = 0; i < E; ++i) rb[i] = gld(a.sw_tmp + fo, col_pos<N, TT, E>(band, b, t, i));
       }
     }
-    fft_regs<N, E, TT, true, false>(x, t, b, lds, tw);
+    fft_regs<N, E, TT, true, false, DW>(x, t, b, lds, tw);
   }
 #pragma unroll
   for (int i = 0; i < E; ++i) gst(dst, col_pos<N, TT, E>(band, b, t, i), x[i]);
@@ -495,7 +497,7 @@ __device__ __forceinline__ void colpass_item(const PassArgs& a, int f, int band,
       sw_weights(a.swf_tab[a.sw_pf ? f : 0].w, w);
       __syncthreads();
       float mx;
-      if constexpr (sw_stage_fits<N, TT>()) {
+      if constexpr (sw_stage_fits<N, TT, DW>()) {
         // staged in tile order: tile q (rows qTT..qTT+TT-1 of the band) at q TT^2
         float* stage = line + sw_stage_off<N, TT>();
         mx = sw_blur_lines<N, TT>(line, w, [&](int l, int row, float v) {
@@ -524,13 +526,13 @@ __device__ __forceinline__ void colpass_item(const PassArgs& a, int f, int band,
 template <int N, int TT, int E>
 __device__ __forceinline__ void colpass_ipsf_a(const PassArgs& a, int f, int band, int z,
                                                c32* lds) {
-  using L = LdsPlan<N, TT>;
+  using L = LdsPlan<N, TT, kColDbWgs>;
   constexpr int NT = (N / E) * TT;
   constexpr bool DUAL = dual_layout<N>();
   const size_t fo = (size_t)f * N * N;
   const c32* src = (z ? (DUAL ? a.phibc : a.phib) : (DUAL ? a.phic : a.phi)) + fo;
   const int tid = threadIdx.x, b = tid % TT, t = tid / TT;
-  TwStage<N, TT, NT> tws;
+  TwStage<N, TT, NT, kColDbWgs> tws;
   tws.fetch(a.tw);
   c32 x[E];
 #pragma unroll
@@ -538,11 +540,11 @@ __device__ __forceinline__ void colpass_ipsf_a(const PassArgs& a, int f, int ban
     x[i] = gld(src, DUAL ? mcol_pos<N, TT, E>(band, b, t, i) : col_pos<N, TT, E>(band, b, t, i));
   c32* twl = lds + L::NBUF * L::BUF;
   tws.put(twl);
-  const auto tw = twiddle_view<N, TT>(twl);
-  fft_regs<N, E, TT, false, true>(x, t, b, lds, tw);
+  const auto tw = twiddle_view<N, TT, kColDbWgs>(twl);
+  fft_regs<N, E, TT, false, true, kColDbWgs>(x, t, b, lds, tw);
 #pragma unroll
   for (int i = 0; i < E; ++i) x[i] = c32{fmaf(x[i].x, x[i].x, x[i].y * x[i].y), 0.0f};
-  fft_regs<N, E, TT, false, false>(x, t, b, lds, tw);
+  fft_regs<N, E, TT, false, false, kColDbWgs>(x, t, b, lds, tw);
 #pragma unroll
   for (int i = 0; i < E; ++i) gst((z ? a.dbuf2 : a.dbuf) + fo, col_pos<N, TT, E>(band, b, t, i), x[i]);
 }
@@ -554,7 +556,7 @@ __device__ __forceinline__ void colpass_ipsf_a(const PassArgs& a, int f, int ban
 template <int N, int TT, int E, int MODE>
 __device__ __forceinline__ void colpass_intensity(const PassArgs& a, int f, int band, int z,
                                                   c32* lds) {
-  using L = LdsPlan<N, TT>;
+  using L = LdsPlan<N, TT, kColDbWgs>;
   constexpr int NT = (N / E) * TT;
   constexpr bool DUAL = dual_layout<N>();
   const size_t fo = (size_t)f * N * N;
@@ -562,7 +564,7 @@ __device__ __forceinline__ void colpass_intensity(const PassArgs& a, int f, int 
   const c32* src = MODE == 6 ? (z ? (DUAL ? a.phibc : a.phib) : (DUAL ? a.phic : a.phi)) + fo
                              : (z ? (DUAL ? a.dbufc2 : a.dbuf2) : (DUAL ? a.dbufc : a.dbuf)) + fo;
   const int tid = threadIdx.x, b = tid % TT, t = tid / TT;
-  TwStage<N, TT, NT> tws;
+  TwStage<N, TT, NT, kColDbWgs> tws;
   tws.fetch(a.tw);
   c32 x[E];
 #pragma unroll
@@ -570,7 +572,7 @@ __device__ __forceinline__ void colpass_intensity(const PassArgs& a, int f, int 
     x[i] = gld(src, DUAL ? mcol_pos<N, TT, E>(band, b, t, i) : col_pos<N, TT, E>(band, b, t, i));
   c32* twl = lds + L::NBUF * L::BUF;
   tws.put(twl);
-  fft_regs<N, E, TT, MODE == 7, true>(x, t, b, lds, twiddle_view<N, TT>(twl));
+  fft_regs<N, E, TT, MODE == 7, true, kColDbWgs>(x, t, b, lds, twiddle_view<N, TT, kColDbWgs>(twl));
   float* dst = (z ? a.imap2 : a.imap) + fo;
 #pragma unroll
   for (int i = 0; i < E; ++i) {
@@ -590,7 +592,7 @@ __device__ __forceinline__ void colpass_intensity(const PassArgs& a, int f, int 
 template <int N, int TT, int E, int SRC>
 __device__ __forceinline__ void colpass_div_b(const PassArgs& a, int f, int band, int z,
                                               double* errp, c32* lds) {
-  using L = LdsPlan<N, TT>;
+  using L = LdsPlan<N, TT, kColDbWgs>;
   constexpr int NB = N / TT;
   constexpr int NT = (N / E) * TT;
   constexpr bool DUAL = dual_layout<N>();
@@ -598,7 +600,7 @@ __device__ __forceinline__ void colpass_div_b(const PassArgs& a, int f, int band
   // z = 1: the projection of DM's estimate phib with its own divisor (no error)
   const c32* src = (z ? (DUAL ? a.phibc : a.phib) : (DUAL ? a.phic : a.phi)) + fo;
   const int tid = threadIdx.x, b = tid % TT, t = tid / TT;
-  TwStage<N, TT, NT> tws;
+  TwStage<N, TT, NT, kColDbWgs> tws;
   tws.fetch(a.tw);
   c32 d[SRC == 0 ? E : 1];
   float div[E];
@@ -621,9 +623,9 @@ __device__ __forceinline__ void colpass_div_b(const PassArgs& a, int f, int band
   for (int i = 0; i < E; ++i) m[i] = gld(mod, mcol_pos<N, TT, E>(band, b, t, i));
   c32* twl = lds + L::NBUF * L::BUF;
   tws.put(twl);
-  const auto tw = twiddle_view<N, TT>(twl);
+  const auto tw = twiddle_view<N, TT, kColDbWgs>(twl);
   if constexpr (SRC == 0) {
-    fft_regs<N, E, TT, true, true>(d, t, b, lds, tw);
+    fft_regs<N, E, TT, true, true, kColDbWgs>(d, t, b, lds, tw);
 #pragma unroll
     for (int i = 0; i < E; ++i)
       div[i] = sqrtf(sqrtf(fmaf(d[i].x, d[i].x, d[i].y * d[i].y)) * a.inv_nn);
@@ -631,7 +633,7 @@ __device__ __forceinline__ void colpass_div_b(const PassArgs& a, int f, int band
 #pragma unroll
     for (int i = 0; i < E; ++i) div[i] = sqrtf(div[i]);
   }
-  fft_regs<N, E, TT, false, SRC != 0>(x, t, b, lds, tw);
+  fft_regs<N, E, TT, false, SRC != 0, kColDbWgs>(x, t, b, lds, tw);
   double err = 0.0;
 #pragma unroll
   for (int i = 0; i < E; ++i) {
@@ -639,7 +641,7 @@ __device__ __forceinline__ void colpass_div_b(const PassArgs& a, int f, int band
     err = fma(dd, dd, err);
     x[i] = scale(x[i], div[i] > 0.0f ? m[i] / div[i] : 0.0f);
   }
-  fft_regs<N, E, TT, true, false>(x, t, b, lds, tw);
+  fft_regs<N, E, TT, true, false, kColDbWgs>(x, t, b, lds, tw);
 #pragma unroll
   for (int i = 0; i < E; ++i) gst((z ? a.phib : a.phi) + fo, col_pos<N, TT, E>(band, b, t, i), x[i]);
   if (errp && z == 0) {
@@ -652,7 +654,7 @@ __device__ __forceinline__ void colpass_div_b(const PassArgs& a, int f, int band
 template <int N, int TT, int E, int MODE>
 __global__ void __launch_bounds__((N / E) * TT) k_colpass(PassArgs a) {
   constexpr int NB = N / TT;
-  __shared__ c32 lds[LdsPlan<N, TT>::TOTAL];
+  __shared__ c32 lds[LdsPlan<N, TT, kColDbWgs>::TOTAL];
   static_assert(MODE != 9 || TT * sw_line<N>() * sizeof(float) <= sizeof(lds),
                 "fused blur lines exceed the pass's LDS");
   const int band = xcd_band<NB>(blockIdx.x), z = blockIdx.z;
@@ -998,7 +1000,8 @@ __global__ void __launch_bounds__((N / E) * TT)
       const int f = fmap ? fmap[fi] : fi;
       const Coef c = tab ? tab[f] : ps.c;
       __syncthreads();  // twiddles staged / previous item done with LDS
-      if (c.flags & kActive) colpass_item<N, TT, E, 0, false>(a, f, band, 0, errp, lds);
+      if (c.flags & kActive)
+        colpass_item<N, TT, E, 0, false, kRowDbWgs>(a, f, band, 0, errp, lds);
     }
     grid_sync(bar, ++epoch * G, flag);
     for (int it = blockIdx.x; it < items; it += G) {

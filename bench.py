@@ -272,10 +272,22 @@ def cpu_baseline(frame, iters):
     r.iterate(algs)
     dt = time.perf_counter() - t0
     cores = len(os.sched_getaffinity(0))
+    # the complex128 oracle on the same frame, a shorter sample
+    from oracle import phasing_c128 as O
+    psi = r.psi_in.astype(np.complex128)
+    m, s = frame.modulus.astype(np.float64), r.support.astype(np.float64)
+    n_or = max(5, iters // 15)
+    t0 = time.perf_counter()
+    for a in algs[:n_or]:
+        psi, _, _ = O.iteration(psi, m, s, a, beta=0.8)
+    dt_or = time.perf_counter() - t0
     return {"value": round(iters / dt, 3), "unit": "iterations/s", "cores": 1, "kind": "port",
             "sample": f"{iters} iterations (90 HIO/10 ER, beta 0.8) of the 1024x1024 frame, "
                       f"numpy {np.__version__} complex64 path with the reference's redundant "
-                      f"FFT per _fft2 call, single-threaded ({cores} host cores visible)"}
+                      f"FFT per _fft2 call, single-threaded ({cores} host cores visible)",
+            "oracle_c128": {"value": round(n_or / dt_or, 3), "unit": "iterations/s", "cores": 1,
+                            "sample": f"{n_or} iterations of the complex128 oracle (with its "
+                                      f"FP64 modulus error) on the same frame"}}
 
 
 def bench_single(a):

@@ -108,6 +108,21 @@ def test_library_rejects_bad_arguments_without_gpu():
     assert lib.spk_plan_bytes(1024, 1, 1, 64, ctypes.byref(out)) == -1   # unknown option bit
 
 
+def test_mixed_size_shapes_give_one_workgroup_shape():
+    """spk_group_shape: every member size runs 256-thread workgroups of eight
+    values per thread ((n / 8) * tile == 256); other sizes are refused."""
+    from als_speckle_amd.device import group_shape
+    for n, tile in ((256, 8), (512, 4), (1024, 2)):
+        assert group_shape(n) == (tile, 8)
+        assert (n // 8) * tile == 256
+    for n in (64, 128, 2048, 4096, 1000):
+        with pytest.raises(_capi.SpkError, match="256, 512 and 1024"):
+            group_shape(n)
+    lib = _capi.load()
+    assert lib.spk_group_create(None, None) == -1            # SPK_EINVAL, no GPU call
+    assert lib.spk_group_iterate(None, None, 1, None, 1, None, None) == -1
+
+
 def test_plan_bytes_counts_the_buffers_of_each_option():
     """spk_plan_bytes grows by each option's buffers (per frame: phib 8 B/px,
     shrinkwrap maps 8 B/px, ipsf work 8 B/px, spectrum maps 8 B/px; two Phi
