@@ -315,12 +315,15 @@ __device__ __forceinline__ float sw_blur_lines(const float* line, const float (&
 }
 
 // sw_blur_lines with every tap and weight read from LDS instead of a
-// register window: the same terms summed in the same order (w[0] x[j], then
-// d = 1 .. 24 with zero weights past the radius), so the same results, with
-// few live registers (for the resident kernel, whose iterate, support and
-// modulus stay in registers).  wl: the kSwFuseRmax + 1 weights in LDS.
+// register window, with few live registers (for the resident kernel, whose
+// iterate, support and modulus stay in registers).  The terms are summed in
+// the same order (w[0] x[j], then d = 1 .. R); the register form goes on to
+// d = 24 with zero weights, whose terms add exactly zero to these sums of
+// non-negative |psi| (finite inputs), so the results are the same.
+// wl: the kSwFuseRmax + 1 weights in LDS; R <= kSwFuseRmax: the radius.
 template <int N, int TT, class OUT>
-__device__ __forceinline__ float sw_blur_lines_lds(const float* line, const float* wl, OUT out) {
+__device__ __forceinline__ float sw_blur_lines_lds(const float* line, const float* wl, int R,
+                                                   OUT out) {
   constexpr int K = 8;
   const int nt = blockDim.x;
   float mx = -INFINITY;
@@ -333,7 +336,7 @@ __device__ __forceinline__ float sw_blur_lines_lds(const float* line, const floa
 #pragma unroll
     for (int k = 0; k < K; ++k) acc[k] = w0 * ln[blur_slot((j0 + k) & (N - 1))];
 #pragma unroll 1
-    for (int d = 1; d <= kSwFuseRmax; ++d) {
+    for (int d = 1; d <= R; ++d) {
       const float wd = wl[d];
 #pragma unroll
       for (int k = 0; k < K; ++k)
@@ -349,10 +352,14 @@ __device__ __forceinline__ float sw_blur_lines_lds(const float* line, const floa
   return mx;
 }
 
-// the fused window's weights of one shrinkwrap into LDS (zero past the radius)
-__device__ __forceinline__ void sw_weights_lds(const BlurW& bw, float* wl) {
-  if (threadIdx.x <= kSwFuseRmax)
-    wl[threadIdx.x] = (int)threadIdx.x <= bw.radius ? bw.w[threadIdx.x] : 0.0f;
+// the fused window's weight of tap threadIdx.x (zero past the radius),
+// fetched early; sw_weight_put stores it in the LDS weight table
+__device__ __forceinline__ float sw_weight_fetch(const BlurW& bw) {
+  return (threadIdx.x <= kSwFuseRmax && (int)threadIdx.x <= bw.radius) ? bw.w[threadIdx.x]
+                                                                         : 0.0f;
+}
+__device__ __forceinline__ void sw_weight_put(float w, float* wl) {
+  if (threadIdx.x <= kSwFuseRmax) wl[threadIdx.x] = w;
 }
 
 // Modulus replacement m * Psi / |Psi| (phasing_fourier.cl:12-15,
@@ -1187,10 +1194,24 @@ __global__ void __launch_bounds__((N / E) * TT)
       __syncthreads();   // the LDS of the previous phase is free
 #pragma unroll
       for (int i = 0; i < E; ++i) x[i] = xld<XM>(phi, col_pos<N, TT, E>(cband, b, t, i));
+      // a fused shrinkwrap's column half: its weights, fetched with Phi
+      float wcol = 0.0f;
+      int rcol = 0;
+      if (swfin) {
+        wcol = sw_weight_fetch(a.swr_tab[st.swf].w);
+        rcol = a.swr_tab[st.swf].w.radius;
+      }
       double err = 0.0;
       fft_regs<N, E, TT, false, true>(x, t, b, lds, tw);
 #pragma unroll
       for (int i = 0; i < E; ++i) x[i] = project_modulus(x[i], m[i], err);
+      // the row-blurred |psi| of the band's columns, loaded under the inverse
+      // transform (as colpass_item MODE 9)
+      float rb[E];
+      if (swfin) {
+#pragma unroll
+        for (int i = 0; i < E; ++i) rb[i] = xldf<XM>(a.sw_tmp, col_pos<N, TT, E>(cband, b, t, i));
+      }
       if (!noout) {
         fft_regs<N, E, TT, true, false>(x, t, b, lds, tw);
 #pragma unroll
@@ -1204,16 +1225,13 @@ __global__ void __launch_bounds__((N / E) * TT)
         // the column half of the previous step's shrinkwrap blur (as
         // colpass_item MODE 9): the band's columns of the row-blurred |psi|
         // -> sw_blur (tiled) and the band maximum
-        float rb[E];
-#pragma unroll
-        for (int i = 0; i < E; ++i) rb[i] = xldf<XM>(a.sw_tmp, col_pos<N, TT, E>(cband, b, t, i));
         __syncthreads();
 #pragma unroll
         for (int i = 0; i < E; ++i) swl[b * sw_line<N>() + blur_slot(t + i * T)] = rb[i];
-        sw_weights_lds(a.swr_tab[st.swf].w, swwl);
+        sw_weight_put(wcol, swwl);
         __syncthreads();
         float* stage = swl + sw_stage_off<N, TT>();
-        const float mx = sw_blur_lines_lds<N, TT>(swl, swwl, [&](int l, int row, float v) {
+        const float mx = sw_blur_lines_lds<N, TT>(swl, swwl, rcol, [&](int l, int row, float v) {
           stage[(row / TT) * (TT * TT) + (row % TT) * TT + l] = v;
         });
         __syncthreads();
@@ -1226,32 +1244,41 @@ __global__ void __launch_bounds__((N / E) * TT)
       }
       resident_sync<XM>(flags, ++epoch, flag);
     }
-    // row phase (rowpass_item; kNoOut skips Phi)
+    // row phase (rowpass_item; kNoOut skips Phi).  Every load is issued
+    // first: Phi, the band maxima and blur of a fused shrinkwrap's threshold,
+    // the weights of this step's shrinkwrap.
     __syncthreads();
+    if (!noout) {
+#pragma unroll
+      for (int i = 0; i < E; ++i) x[i] = xld<XM>(phi, row_pos<N, TT, E>(rband, b, t, i));
+    }
+    float wrow = 0.0f;
+    int rrow = 0;
+    if (swrow) {
+      wrow = sw_weight_fetch(a.swr_tab[st.swr].w);
+      rrow = a.swr_tab[st.swr].w.radius;
+    }
     if (swfin) {
       // the previous step's threshold: s = 0 where blur <= tau * max, with
       // the frame maximum from the column bands' maxima (FP64 threshold)
       constexpr int MPT = (NB + NT - 1) / NT;
       float mx = -INFINITY;
+      float bl[E];
 #pragma unroll
       for (int j = 0; j < MPT; ++j) {
         const int q = tid + j * NT;
         if (q < NB) mx = fmaxf(mx, xldf<XM>(a.sw_maxp, q));
       }
+#pragma unroll
+      for (int i = 0; i < E; ++i) bl[i] = xldf<XM>(a.sw_blur, row_pos<N, TT, E>(rband, b, t, i));
       const double thr = (double)a.swr_tab[st.swf].tau * (double)block_max(mx, swl);
 #pragma unroll
-      for (int i = 0; i < E; ++i) {
-        const float bl = xldf<XM>(a.sw_blur, row_pos<N, TT, E>(rband, b, t, i));
-        if (!((double)bl > thr)) sv[i] = 0.0f;
-      }
+      for (int i = 0; i < E; ++i)
+        if (!((double)bl[i] > thr)) sv[i] = 0.0f;
       sup_changed = true;
       __syncthreads();
     }
-    if (!noout) {
-#pragma unroll
-      for (int i = 0; i < E; ++i) x[i] = xld<XM>(phi, row_pos<N, TT, E>(rband, b, t, i));
-      fft_regs<N, E, TT, true, true>(x, t, b, lds, tw);
-    }
+    if (!noout) fft_regs<N, E, TT, true, true>(x, t, b, lds, tw);
     float av[E];
 #pragma unroll
     for (int i = 0; i < E; ++i) {
@@ -1270,10 +1297,10 @@ __global__ void __launch_bounds__((N / E) * TT)
       __syncthreads();
 #pragma unroll
       for (int i = 0; i < E; ++i) swl[b * sw_line<N>() + blur_slot(t + i * T)] = av[i];
-      sw_weights_lds(a.swr_tab[st.swr].w, swwl);
+      sw_weight_put(wrow, swwl);
       __syncthreads();
       float* stage = swl + sw_stage_off<N, TT>();
-      sw_blur_lines_lds<N, TT>(swl, swwl, [&](int l, int c, float v) {
+      sw_blur_lines_lds<N, TT>(swl, swwl, rrow, [&](int l, int c, float v) {
         stage[(c / TT) * (TT * TT) + l * TT + (c % TT)] = v;
       });
       __syncthreads();
