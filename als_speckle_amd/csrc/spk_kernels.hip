@@ -314,17 +314,18 @@ __device__ __forceinline__ float sw_blur_lines(const float* line, const float (&
   return mx;
 }
 
-// sw_blur_lines with every tap and weight read from LDS instead of a
-// register window, with few live registers (for the resident kernel, whose
-// iterate, support and modulus stay in registers).  The terms are summed in
-// the same order (w[0] x[j], then d = 1 .. R); the register form goes on to
-// d = 24 with zero weights, whose terms add exactly zero to these sums of
-// non-negative |psi| (finite inputs), so the results are the same.
-// wl: the kSwFuseRmax + 1 weights in LDS; R <= kSwFuseRmax: the radius.
+// sw_blur_lines for the resident kernel, whose iterate, support and modulus
+// stay in registers: each segment's taps are loaded in three chunks of tap
+// distance (d in 1..8, 9..16, 17..24: 15 values left and 15 right of the
+// segment per chunk, one LDS read each) and the weights are read from LDS
+// (one broadcast read per distance), so few registers are live.  The terms
+// are blur_window's, in blur_window's order (w0 x[j], then d ascending), so
+// the results are sw_blur_lines'.  wl: the kSwFuseRmax + 1 weights in LDS
+// (zero past the radius).
 template <int N, int TT, class OUT>
-__device__ __forceinline__ float sw_blur_lines_lds(const float* line, const float* wl, int R,
-                                                   OUT out) {
-  constexpr int K = 8;
+__device__ __forceinline__ float sw_blur_lines_win(const float* line, const float* wl, OUT out) {
+  constexpr int K = 8, C = 8;   // outputs per segment, distances per chunk
+  static_assert(kSwFuseRmax % C == 0, "whole chunks of tap distance");
   const int nt = blockDim.x;
   float mx = -INFINITY;
 #pragma unroll 1
@@ -335,13 +336,22 @@ __device__ __forceinline__ float sw_blur_lines_lds(const float* line, const floa
     const float w0 = wl[0];
 #pragma unroll
     for (int k = 0; k < K; ++k) acc[k] = w0 * ln[blur_slot((j0 + k) & (N - 1))];
-#pragma unroll 1
-    for (int d = 1; d <= R; ++d) {
-      const float wd = wl[d];
 #pragma unroll
-      for (int k = 0; k < K; ++k)
-        acc[k] = fmaf(wd, ln[blur_slot((j0 + k + d) & (N - 1))] + ln[blur_slot((j0 + k - d) & (N - 1))],
-                      acc[k]);
+    for (int a = 1; a <= kSwFuseRmax; a += C) {
+      // x[j0 + k - d] = lo[k - d + a + C - 1], x[j0 + k + d] = hi[k + d - a]
+      float lo[K + C - 1], hi[K + C - 1];
+#pragma unroll
+      for (int i = 0; i < K + C - 1; ++i) {
+        lo[i] = ln[blur_slot((j0 - a - (C - 1) + i) & (N - 1))];
+        hi[i] = ln[blur_slot((j0 + a + i) & (N - 1))];
+      }
+#pragma unroll
+      for (int d = a; d < a + C; ++d) {
+        const float wd = wl[d];
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+          acc[k] = fmaf(wd, hi[k + d - a] + lo[k - d + a + C - 1], acc[k]);
+      }
     }
 #pragma unroll
     for (int k = 0; k < K; ++k) {
@@ -1231,7 +1241,7 @@ __global__ void __launch_bounds__((N / E) * TT)
         sw_weight_put(wcol, swwl);
         __syncthreads();
         float* stage = swl + sw_stage_off<N, TT>();
-        const float mx = sw_blur_lines_lds<N, TT>(swl, swwl, rcol, [&](int l, int row, float v) {
+        const float mx = sw_blur_lines_win<N, TT>(swl, swwl, [&](int l, int row, float v) {
           stage[(row / TT) * (TT * TT) + (row % TT) * TT + l] = v;
         });
         __syncthreads();
@@ -1300,7 +1310,7 @@ __global__ void __launch_bounds__((N / E) * TT)
       sw_weight_put(wrow, swwl);
       __syncthreads();
       float* stage = swl + sw_stage_off<N, TT>();
-      sw_blur_lines_lds<N, TT>(swl, swwl, rrow, [&](int l, int c, float v) {
+      sw_blur_lines_win<N, TT>(swl, swwl, [&](int l, int c, float v) {
         stage[(c / TT) * (TT * TT) + l * TT + (c % TT)] = v;
       });
       __syncthreads();
