@@ -128,3 +128,22 @@ def test_resident_shrinkwrap_cases(gpu_ctx):
         got = run(gpu_ctx, f, g, calls, resident=True, trace=trace)
         same(got, ref)
         assert got[3][0] == 3          # sigma 5 splits the first call; the second is one block
+
+
+def test_resident_error_only_steps_and_calls_between_blocks(gpu_ctx):
+    """ER steps (numpy semantics) with an error trace, whose column phase only
+    forms the error partials: odd and even runs of them inside a block and at
+    its end, and a one-step call (per-pass launches reading the Phi the block
+    left) after each block: bitwise the per-pass results."""
+    print("[resident error-only steps]", flush=True)
+    f, g = data(512, 14)
+    calls = [[step("hio")] * 3 + [step("er")] * 3 + [step("hio")] * 2,
+             [step("er")],
+             [step("hio")] * 2 + [step("er")] * 2 + [step("raar", 0.87)] + [step("er")],
+             [step("hio")],
+             [step("er")] * 5,
+             [step("sf")]]
+    ref = run(gpu_ctx, f, g, calls, resident=False)
+    got = run(gpu_ctx, f, g, calls, resident=True)
+    same(got, ref)
+    assert got[3][0] == 3          # three blocks ran resident
