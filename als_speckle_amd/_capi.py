@@ -32,7 +32,7 @@ class SpkStep(ctypes.Structure):
 class SpkTune(ctypes.Structure):
     _fields_ = [("tile", c_int32), ("ept", c_int32), ("persist", c_int32),
                 ("sw_separate", c_int32), ("resident", c_int32), ("exchange", c_int32),
-                ("reserved", c_int32 * 2)]
+                ("barrier_polls", c_int32), ("reserved", c_int32)]
 
 
 class SpkError(RuntimeError):

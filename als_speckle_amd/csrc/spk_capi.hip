@@ -149,6 +149,7 @@ struct spk_plan {
   int resident = 0;
   int xm = 0;
   unsigned* rsync = nullptr;   // barrier flags of the resident launches (a line per band)
+  unsigned polls = 1u << 18;   // polls of a grid-barrier wait before it gives up
   long long res_launches = 0, res_steps = 0;
   long long sw_fused_count = 0, sw_separate_count = 0;
   unsigned* pbar = nullptr;
@@ -281,6 +282,7 @@ PassArgs base_args(spk_plan* p) {
   a.sw_tmp = p->sw_tmp;
   a.sw_blur = p->sw_blur;
   a.sw_maxp = p->sw_maxp;
+  a.polls = p->polls;
   return a;
 }
 
@@ -571,6 +573,7 @@ int spk_plan_create(spk_ctx* ctx, int n, int batch, int shared, const spk_tune* 
   const int res_mode = tune ? tune->resident : 0;
   p->resident = (res_mode != 2 && batch == 1 && resident_supported(n, tile, ept)) ? 1 : 0;
   p->xm = (tune && tune->exchange >= 1 && tune->exchange <= 3) ? tune->exchange - 1 : 2;
+  p->polls = (tune && tune->barrier_polls > 0) ? (unsigned)tune->barrier_polls : (1u << 18);
   p->sw_fuse = !(tune && tune->sw_separate > 0);
   const size_t nn = (size_t)n * n;
   const size_t fms = shared ? 1 : (size_t)batch;
@@ -646,8 +649,8 @@ int spk_plan_check(spk_plan* p) {
   SPK_HIP(hipMemcpy(h, p->pbar, sizeof(h), hipMemcpyDeviceToHost));
   if (h[2]) {
     SPK_HIP(hipMemset(p->pbar + 2, 0, sizeof(unsigned)));
-    return fail(SPK_EHIP, "a grid barrier of a persistent iteration block gave up waiting; "
-                          "results of that block are invalid");
+    return fail(SPK_EHIP, "a grid barrier of a resident or persistent launch gave up "
+                          "waiting; results of that launch are invalid");
   }
   return SPK_OK;
 }

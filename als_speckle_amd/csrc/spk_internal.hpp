@@ -88,6 +88,7 @@ struct PassArgs {
   const SwParam* swf_tab;   // for kSwFin (the previous step's)
   int sw_pf;                // 1: the tables hold one entry per frame, 0: one entry
   int nt_store;             // 1: the row pass stores psi and Phi non-temporally
+  unsigned polls;           // grid barriers: polls of a wait before it gives up
 };
 
 // One step of a persistent iteration block (k_persist).  Offsets are bytes

@@ -956,15 +956,17 @@ __global__ void __launch_bounds__((N / E) * TT)
 // workgroup barrier, thread 0 fences at device scope before its arrival
 // (release: the workgroup's stores leave the XCD's L2) and after the wait
 // (acquire: stale lines of other XCDs' data are dropped), then the workgroup
-// barrier releases the other waves.  The wait is bounded: after 2^18
-// polls (a quarter second or more, far above any legitimate wait; the launch
+// barrier releases the other waves.  The wait is bounded: after `limit`
+// polls (2^18 by default, a quarter second or more, far above any
+// legitimate wait; spk_tune.barrier_polls sets it; the launch
 // is cooperative, so every workgroup is resident) flag[0] (this launch's,
 // cleared with the counter before each launch) and flag[1] (sticky) are set
 // and the workgroup goes on, later barriers of the launch pass at once, and
 // every wave ends; the host reports the sticky flag as an error
 // (spk_plan_check).
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void grid_sync(unsigned* bar, unsigned target, int* flag) {
+__device__ __forceinline__ void grid_sync(unsigned* bar, unsigned target, int* flag,
+                                          unsigned limit) {
   // every wave waits for its own stores to reach L2 (s_waitcnt vmcnt(0):
   // gfx9 encoding, expcnt and lgkmcnt left at their maxima), then thread 0
   // publishes the workgroup's stores device-wide: one L2 write-back and one
@@ -979,7 +981,7 @@ __device__ __forceinline__ void grid_sync(unsigned* bar, unsigned target, int* f
       unsigned polls = 0;
       while (__hip_atomic_load(bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
         __builtin_amdgcn_s_sleep(1);
-        if (++polls > (1u << 18)) {
+        if (++polls > limit) {
           __hip_atomic_store(flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
           __hip_atomic_store(flag + 1, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
           break;
@@ -1020,7 +1022,7 @@ __global__ void __launch_bounds__((N / E) * TT)
       if (c.flags & kActive)
         colpass_item<N, TT, E, 0, false, kRowDbWgs>(a, f, band, 0, errp, lds);
     }
-    grid_sync(bar, ++epoch * G, flag);
+    grid_sync(bar, ++epoch * G, flag, a.polls);
     for (int it = blockIdx.x; it < items; it += G) {
       const int fi = it / NB, band = it % NB;
       const int f = fmap ? fmap[fi] : fi;
@@ -1028,7 +1030,7 @@ __global__ void __launch_bounds__((N / E) * TT)
       __syncthreads();
       if (c.flags & kActive) rowpass_item<N, TT, E, false, false, false>(a, f, band, c, lds);
     }
-    if (k + 1 < nsteps) grid_sync(bar, ++epoch * G, flag);
+    if (k + 1 < nsteps) grid_sync(bar, ++epoch * G, flag, a.polls);
   }
 }
 
@@ -1113,10 +1115,11 @@ __device__ __forceinline__ void sw_store_staged_x(const float* stage, float* dst
 // MI355X for 256 workgroups: 2.2 us per barrier, against 8 us with one
 // shared arrival counter.  Device-scope fences as the exchange mode needs
 // them (release before the flag, acquire after the wait).  Waits are
-// bounded as in grid_sync: after 2^18 polls flag[0] (this launch's) and
+// bounded as in grid_sync: after a.polls polls flag[0] (this launch's) and
 // flag[1] (sticky) are set and every later wait of the launch passes.
 template <int XM>
-__device__ __forceinline__ void resident_sync(unsigned* flags, unsigned epoch, int* flag) {
+__device__ __forceinline__ void resident_sync(unsigned* flags, unsigned epoch, int* flag,
+                                              unsigned limit) {
   constexpr int LINE = 32;   // unsigned words per 128-byte line
   constexpr bool fence = XM != 2;
   __builtin_amdgcn_s_waitcnt(0x0F70);   // this wave's stores have completed
@@ -1133,7 +1136,7 @@ __device__ __forceinline__ void resident_sync(unsigned* flags, unsigned epoch, i
       while (!gave && __hip_atomic_load(flags + j * LINE, __ATOMIC_RELAXED,
                                         __HIP_MEMORY_SCOPE_AGENT) < epoch) {
         __builtin_amdgcn_s_sleep(1);
-        if (++polls > (1u << 18)) {
+        if (++polls > limit) {
           __hip_atomic_store(flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
           __hip_atomic_store(flag + 1, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
           gave = true;
@@ -1252,7 +1255,7 @@ __global__ void __launch_bounds__((N / E) * TT)
         const float bm = block_max(mx, swl);
         if (tid == 0) xstf<XM>(a.sw_maxp, cband, bm);
       }
-      resident_sync<XM>(flags, ++epoch, flag);
+      resident_sync<XM>(flags, ++epoch, flag, a.polls);
     }
     // row phase (rowpass_item; kNoOut skips Phi).  Every load is issued
     // first: Phi, the band maxima and blur of a fused shrinkwrap's threshold,
@@ -1317,7 +1320,7 @@ __global__ void __launch_bounds__((N / E) * TT)
       sw_store_staged_x<N, TT, XM>(stage, a.sw_tmp + (size_t)rband * N * TT,
                                    [](int v) { return 4 * v; });
     }
-    if (k + 1 < nsteps) resident_sync<XM>(flags, ++epoch, flag);
+    if (k + 1 < nsteps) resident_sync<XM>(flags, ++epoch, flag, a.polls);
   }
 #pragma unroll
   for (int i = 0; i < E; ++i) gst(a.psi, row_pos<N, TT, E>(rband, b, t, i), pin[i]);

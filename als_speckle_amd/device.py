@@ -139,7 +139,8 @@ class Plan:
 
     def __init__(self, ctx: Context, n: int, batch: int = 1, shared: bool = True,
                  tile: int = 0, ept: int = 0, persist: bool = False,
-                 sw_separate: bool = False, resident: int = 0, exchange: int = 0):
+                 sw_separate: bool = False, resident: int = 0, exchange: int = 0,
+                 barrier_polls: int = 0):
         self.ctx, self.lib = ctx, ctx.lib
         self.n, self.batch, self.shared = int(n), int(batch), bool(shared)
         self.persist = bool(persist)
@@ -150,6 +151,7 @@ class Plan:
         tune.sw_separate = int(bool(sw_separate))
         tune.resident = int(resident)      # 1 on, 2 off, 0 library default
         tune.exchange = int(exchange)
+        tune.barrier_polls = int(barrier_polls)   # 0: library default
         h = c_void_p()
         check(self.lib.spk_plan_create(ctx.handle, self.n, self.batch, int(self.shared),
                                        byref(tune), byref(h)))
@@ -349,7 +351,8 @@ class Plan:
         check(self.lib.spk_set_spectrum(self.handle, r.ctypes.data, w.ctypes.data, r.size))
 
     def check(self) -> None:
-        """Wait for queued work; raise if a persistent block's barrier gave up."""
+        """Wait for queued work; raise if a grid barrier of a resident or
+        persistent launch gave up waiting (SpkError, the flag is cleared)."""
         check(self.lib.spk_plan_check(self.handle))
 
     def close(self) -> None:

@@ -147,3 +147,35 @@ def test_resident_error_only_steps_and_calls_between_blocks(gpu_ctx):
     got = run(gpu_ctx, f, g, calls, resident=True)
     same(got, ref)
     assert got[3][0] == 3          # three blocks ran resident
+
+
+@pytest.mark.parametrize("kind", ["resident", "persist"])
+def test_barrier_give_up_is_reported_and_cleared(gpu_ctx, kind):
+    """A grid-barrier wait that gives up, forced from the host with a poll
+    limit of 1 (the first workgroups to arrive stop waiting), lets the launch
+    drain: the call returns, spk_plan_check reports SPK_EHIP once and clears
+    the flag, and the next launch of a plan with the default limit gives the
+    per-pass results.  Every access of the kernels stays in bounds whatever
+    the values, so a launch past a given-up barrier computes garbage, never
+    out of range."""
+    print(f"[barrier give-up {kind}]", flush=True)
+    from als_speckle_amd._capi import SpkError
+    n = 1024
+    f, g = data(n, 16)
+    kw = dict(resident=1) if kind == "resident" else dict(resident=2, persist=True)
+    p = Plan(gpu_ctx, n, barrier_polls=1, **kw)
+    p.set_modulus(torch.from_numpy(f.modulus).to(gpu_ctx.device))
+    p.set_support(torch.from_numpy(f.support).to(gpu_ctx.device))
+    p.set_psi(torch.from_numpy(g).to(gpu_ctx.device))
+    p.iterate([step("hio")] * 12, error_trace=True)
+    with pytest.raises(SpkError) as e:
+        p.check()
+    assert e.value.code == -2
+    p.check()                                   # the flag was cleared
+    if kind == "resident":
+        assert p.resident_counts()[0] >= 1
+    p.close()
+    calls = [[step("hio")] * 6 + [step("er")] * 2]
+    ref = run(gpu_ctx, f, g, calls, resident=False)
+    got = run(gpu_ctx, f, g, calls, resident=True)
+    same(got, ref)
