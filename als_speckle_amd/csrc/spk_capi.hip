@@ -45,12 +45,13 @@ bool is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
 //  - 8 values per thread (more threads, shorter dependency chains) wherever
 //    the workgroup fits, except: 1024^2 with 4 or more frames, where 16
 //    values per thread (radix-16 stages) are 9 % faster per batched
-//    iteration, and 2048^2 (tile 2), where they are 10 % faster for one
-//    frame and 8 % for four.
+//    iteration, and tile 2 from 2048^2 up, where they are 10 % faster for
+//    one 2048^2 frame and 8 % for four, and 2.5-3.5 % faster per 4096^2
+//    iteration (row pass 171-175 instead of 179-181 us, column pass equal).
 int default_tile(int n) { return n >= 2048 ? 2 : 4; }
 int default_ept(int n, int tile, int batch) {
   if (n == 1024 && batch >= 4 && (n / 16) * tile <= 1024) return 16;
-  if (n == 2048 && tile == 2) return 16;
+  if (n >= 2048 && tile == 2) return 16;
   return (n / 8) * tile <= 1024 ? 8 : 16;
 }
 
