@@ -1111,15 +1111,21 @@ __device__ __forceinline__ void sw_store_staged_x(const float* stage, float* dst
 // Grid barrier of the resident kernel.  Every workgroup publishes the
 // barrier's epoch in its own flag (one 128-byte line per workgroup, so the
 // arrivals do not contend for one counter), and the first wavefront of every
-// workgroup reads all the flags until each holds the epoch.  Measured on
-// MI355X for 256 workgroups: 2.2 us per barrier, against 8 us with one
-// shared arrival counter.  Device-scope fences as the exchange mode needs
-// them (release before the flag, acquire after the wait).  Waits are
-// bounded as in grid_sync: after a.polls polls flag[0] (this launch's) and
-// flag[1] (sticky) are set and every later wait of the launch passes.
-template <int XM>
+// workgroup reads all the flags until each holds the epoch: lane l reads
+// flags l, l + 64, ... (K = grid / 64 of them), all K loads in flight at once
+// in every poll round, so a round costs one device-scope load latency
+// however many flags a lane covers.  Reading a lane's flags one after the
+// other (each awaited before the next is issued) cost 1.9 us more per
+// iteration of the 1024^2 headline (two barriers; 16.9 vs 15.0 us).  One
+// shared arrival counter was slower still (8 us per barrier).
+// Device-scope fences as the exchange mode needs them (release before the
+// flag, acquire after the wait).  Waits are bounded as in grid_sync: after
+// `limit` poll rounds flag[0] (this launch's) and flag[1] (sticky) are set
+// and every later wait of the launch passes.
+template <int XM, int K>
 __device__ __forceinline__ void resident_sync(unsigned* flags, unsigned epoch, int* flag,
                                               unsigned limit) {
+  static_assert(K >= 1, "at least one flag per polling lane");
   constexpr int LINE = 32;   // unsigned words per 128-byte line
   constexpr bool fence = XM != 2;
   __builtin_amdgcn_s_waitcnt(0x0F70);   // this wave's stores have completed
@@ -1131,16 +1137,25 @@ __device__ __forceinline__ void resident_sync(unsigned* flags, unsigned epoch, i
   }
   if (threadIdx.x < 64) {
     bool gave = __hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
-    for (unsigned j = threadIdx.x; j < gridDim.x; j += 64) {
-      unsigned polls = 0;
-      while (!gave && __hip_atomic_load(flags + j * LINE, __ATOMIC_RELAXED,
-                                        __HIP_MEMORY_SCOPE_AGENT) < epoch) {
-        __builtin_amdgcn_s_sleep(1);
-        if (++polls > limit) {
-          __hip_atomic_store(flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          __hip_atomic_store(flag + 1, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          gave = true;
-        }
+    unsigned polls = 0;
+    while (!gave) {
+      // no branch between the loads (a flag already seen is read again), so
+      // they are issued back to back and awaited once
+      unsigned v[K];
+#pragma unroll
+      for (int q = 0; q < K; ++q) {
+        const unsigned j = min(threadIdx.x + q * 64u, gridDim.x - 1u);
+        v[q] = __hip_atomic_load(flags + j * LINE, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+      bool done = true;
+#pragma unroll
+      for (int q = 0; q < K; ++q) done = done && v[q] >= epoch;
+      if (done) break;
+      __builtin_amdgcn_s_sleep(1);
+      if (++polls > limit) {
+        __hip_atomic_store(flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(flag + 1, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        gave = true;
       }
     }
     if (fence) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
@@ -1255,7 +1270,7 @@ __global__ void __launch_bounds__((N / E) * TT)
         const float bm = block_max(mx, swl);
         if (tid == 0) xstf<XM>(a.sw_maxp, cband, bm);
       }
-      resident_sync<XM>(flags, ++epoch, flag, a.polls);
+      resident_sync<XM, (NB + 63) / 64>(flags, ++epoch, flag, a.polls);
     }
     // row phase (rowpass_item; kNoOut skips Phi).  Every load is issued
     // first: Phi, the band maxima and blur of a fused shrinkwrap's threshold,
@@ -1320,7 +1335,7 @@ __global__ void __launch_bounds__((N / E) * TT)
       sw_store_staged_x<N, TT, XM>(stage, a.sw_tmp + (size_t)rband * N * TT,
                                    [](int v) { return 4 * v; });
     }
-    if (k + 1 < nsteps) resident_sync<XM>(flags, ++epoch, flag, a.polls);
+    if (k + 1 < nsteps) resident_sync<XM, (NB + 63) / 64>(flags, ++epoch, flag, a.polls);
   }
 #pragma unroll
   for (int i = 0; i < E; ++i) gst(a.psi, row_pos<N, TT, E>(rband, b, t, i), pin[i]);

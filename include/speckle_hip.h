@@ -81,8 +81,8 @@ typedef struct spk_tune {
                          1 plain + device-scope fences, 2 device-scope
                          accesses + fences, 3 device-scope accesses without
                          fences (the default); 0 = default                   */
-  int32_t barrier_polls; /* resident and persistent launches: polls of a
-                         grid-barrier wait before it gives up (the launch
+  int32_t barrier_polls; /* resident and persistent launches: poll rounds of
+                         a grid-barrier wait before it gives up (the launch
                          then drains without waiting and spk_plan_check
                          reports it); 0 = default, 262144.  Small values
                          exist to exercise the give-up path.                 */
