@@ -1210,12 +1210,17 @@ __global__ void __launch_bounds__((N / E) * TT)
   c32* phi = a.phi;
   unsigned epoch = 0;
   bool sup_changed = false;
+  // the column phase's FP64 error partial of this thread; the workgroup sum
+  // is formed in the row phase while its Phi loads are in flight (off the
+  // path between the column stores and the barrier)
+  double err = 0.0;
   for (int k = 0; k < nsteps; ++k) {
     const RStep st = steps[k];
     const bool noout = (st.c.flags & kNoOut) != 0;
     const bool swrow = (st.c.flags & kSwRow) != 0;   // shrinkwrap after this step
     const bool swfin = (st.c.flags & kSwFin) != 0;   // ... after the previous one
     const bool col = !noout || st.err || swfin;
+    const bool err_sum = col && st.err && errp0;
     c32 x[E];
     if (col) {
       // column phase (colpass_item MODE 0, MODE 10 for kNoOut, MODE 9's blur)
@@ -1229,7 +1234,7 @@ __global__ void __launch_bounds__((N / E) * TT)
         wcol = sw_weight_fetch(a.swr_tab[st.swf].w);
         rcol = a.swr_tab[st.swf].w.radius;
       }
-      double err = 0.0;
+      err = 0.0;
       fft_regs<N, E, TT, false, true>(x, t, b, lds, tw);
 #pragma unroll
       for (int i = 0; i < E; ++i) x[i] = project_modulus(x[i], m[i], err);
@@ -1244,10 +1249,6 @@ __global__ void __launch_bounds__((N / E) * TT)
         fft_regs<N, E, TT, true, false>(x, t, b, lds, tw);
 #pragma unroll
         for (int i = 0; i < E; ++i) xst<XM>(phi, col_pos<N, TT, E>(cband, b, t, i), x[i]);
-      }
-      if (st.err && errp0) {
-        const double s = block_sum(err, reinterpret_cast<double*>(lds));
-        if (tid == 0) errp0[(size_t)st.kidx * errp_step + cband] = s;
       }
       if (swfin) {
         // the column half of the previous step's shrinkwrap blur (as
@@ -1305,6 +1306,13 @@ __global__ void __launch_bounds__((N / E) * TT)
         if (!((double)bl[i] > thr)) sv[i] = 0.0f;
       sup_changed = true;
       __syncthreads();
+    }
+    if (err_sum) {
+      // the column phase's error partial (the same sum in the same order as
+      // colpass_item's); the reduction uses the shrinkwrap area of the LDS,
+      // which the transforms below do not touch
+      const double s = block_sum(err, reinterpret_cast<double*>(swl));
+      if (tid == 0) errp0[(size_t)st.kidx * errp_step + cband] = s;
     }
     if (!noout) fft_regs<N, E, TT, true, true>(x, t, b, lds, tw);
     float av[E];
