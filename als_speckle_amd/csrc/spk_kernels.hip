@@ -278,12 +278,20 @@ constexpr bool sw_stage_fits() {
   return sw_lds_floats<N, TT>() * 4 <= LdsPlan<N, TT, DW>::TOTAL * 8;
 }
 
+// Slot of staged output i: groups of four floats are permuted within each
+// block of 32 (i ^ (((i >> 5) & 7) << 2)).  The blur writes its outputs 32
+// slots apart across a wavefront's lanes, which without the permutation all
+// fall in two LDS banks; a group of four stays contiguous and aligned for the
+// float4 reads of the store.
+__device__ __forceinline__ int sw_swz(int i) { return i ^ (((i >> 5) & 7) << 2); }
+
 // Store the staged band (N x TT floats) as float4s: group v of four staged
 // floats goes to dst + off(v).
 template <int N, int TT, class OFF>
 __device__ __forceinline__ void sw_store_staged(const float* stage, float* dst, OFF off) {
   for (int v = threadIdx.x; v < N * TT / 4; v += blockDim.x)
-    *reinterpret_cast<float4*>(dst + off(v)) = *reinterpret_cast<const float4*>(stage + 4 * v);
+    *reinterpret_cast<float4*>(dst + off(v)) =
+        *reinterpret_cast<const float4*>(stage + sw_swz(4 * v));
 }
 
 __device__ __forceinline__ void sw_weights(const BlurW& bw, float (&w)[kSwFuseRmax + 1]) {
@@ -518,7 +526,7 @@ __device__ __forceinline__ void colpass_item(const PassArgs& a, int f, int band,
         // staged in tile order: tile q (rows qTT..qTT+TT-1 of the band) at q TT^2
         float* stage = line + sw_stage_off<N, TT>();
         mx = sw_blur_lines<N, TT>(line, w, [&](int l, int row, float v) {
-          stage[(row / TT) * (TT * TT) + (row % TT) * TT + l] = v;
+          stage[sw_swz((row / TT) * (TT * TT) + (row % TT) * TT + l)] = v;
         });
         __syncthreads();
         sw_store_staged<N, TT>(stage, a.sw_blur + fo, [&](int v) {
@@ -881,7 +889,7 @@ __device__ __forceinline__ void rowpass_item(const PassArgs& a, int f, int band,
         // staged in the band's (contiguous) global order
         float* stage = line + sw_stage_off<N, TT>();
         sw_blur_lines<N, TT>(line, w, [&](int l, int col, float v) {
-          stage[(col / TT) * (TT * TT) + l * TT + (col % TT)] = v;
+          stage[sw_swz((col / TT) * (TT * TT) + l * TT + (col % TT))] = v;
         });
         __syncthreads();
         sw_store_staged<N, TT>(stage, tmp, [](int v) { return 4 * v; });
@@ -1098,9 +1106,10 @@ template <int N, int TT, int XM, class OFF>
 __device__ __forceinline__ void sw_store_staged_x(const float* stage, float* dst, OFF off) {
   for (int v = threadIdx.x; v < N * TT / 4; v += blockDim.x) {
     if constexpr (XM != 2) {
-      *reinterpret_cast<float4*>(dst + off(v)) = *reinterpret_cast<const float4*>(stage + 4 * v);
+      *reinterpret_cast<float4*>(dst + off(v)) =
+          *reinterpret_cast<const float4*>(stage + sw_swz(4 * v));
     } else {
-      const uint64_t* s2 = reinterpret_cast<const uint64_t*>(stage + 4 * v);
+      const uint64_t* s2 = reinterpret_cast<const uint64_t*>(stage + sw_swz(4 * v));
       uint64_t* d2 = reinterpret_cast<uint64_t*>(dst + off(v));
       __hip_atomic_store(d2, s2[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       __hip_atomic_store(d2 + 1, s2[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -1261,7 +1270,7 @@ __global__ void __launch_bounds__((N / E) * TT)
         __syncthreads();
         float* stage = swl + sw_stage_off<N, TT>();
         const float mx = sw_blur_lines_win<N, TT>(swl, swwl, [&](int l, int row, float v) {
-          stage[(row / TT) * (TT * TT) + (row % TT) * TT + l] = v;
+          stage[sw_swz((row / TT) * (TT * TT) + (row % TT) * TT + l)] = v;
         });
         __syncthreads();
         sw_store_staged_x<N, TT, XM>(stage, a.sw_blur, [&](int v) {
@@ -1337,7 +1346,7 @@ __global__ void __launch_bounds__((N / E) * TT)
       __syncthreads();
       float* stage = swl + sw_stage_off<N, TT>();
       sw_blur_lines_win<N, TT>(swl, swwl, [&](int l, int c, float v) {
-        stage[(c / TT) * (TT * TT) + l * TT + (c % TT)] = v;
+        stage[sw_swz((c / TT) * (TT * TT) + l * TT + (c % TT))] = v;
       });
       __syncthreads();
       sw_store_staged_x<N, TT, XM>(stage, a.sw_tmp + (size_t)rband * N * TT,
@@ -1568,7 +1577,7 @@ __global__ void __launch_bounds__(256) k_blur_rows(const c32* psi, float* tmp,
 #pragma unroll
     for (int k = 0; k < K; ++k) {
       const int col = c0 + k;
-      stg[(col / TT) * (TT * TT) + b * TT + (col % TT)] = acc[k];
+      stg[sw_swz((col / TT) * (TT * TT) + b * TT + (col % TT))] = acc[k];
     }
   }
   __syncthreads();
@@ -1605,7 +1614,7 @@ __global__ void __launch_bounds__(256) k_blur_cols(const float* tmp, float* blur
 #pragma unroll
     for (int k = 0; k < K; ++k) {
       const int row = r0 + k;
-      stg[(row / TT) * (TT * TT) + (row % TT) * TT + ci] = acc[k];
+      stg[sw_swz((row / TT) * (TT * TT) + (row % TT) * TT + ci)] = acc[k];
       mx = fmaxf(mx, acc[k]);
     }
   }
