@@ -2445,7 +2445,8 @@ static hipError_t resident_nt(int xm, const PassArgs& a, const RStep* steps, int
 
 // the resident form exists for one frame of 256^2 .. 1024^2 in tile 4 with
 // 8 values per thread (one workgroup of at most 512 threads per band, at
-// most 256 bands: one per CU)
+// most 256 bands: one per CU), and for 256^2 and 512^2 also in tile 2 (128
+// and 256 bands: more CUs, half the lines per workgroup)
 template <int N>
 static hipError_t resident_dispatch(int tile, int ept, int xm, const PassArgs& a,
                                     const RStep* steps, int nsteps, double* errp0,
@@ -2454,6 +2455,10 @@ static hipError_t resident_dispatch(int tile, int ept, int xm, const PassArgs& a
   if constexpr (N >= 256 && N <= 1024) {
     if (tile == 4 && ept == 8)
       return resident_nt<N, 4, 8>(xm, a, steps, nsteps, errp0, errp_step, bar, flag, st);
+    if constexpr (N <= 512) {
+      if (tile == 2 && ept == 8)
+        return resident_nt<N, 2, 8>(xm, a, steps, nsteps, errp0, errp_step, bar, flag, st);
+    }
   }
   return hipErrorInvalidValue;
 }
@@ -2672,7 +2677,8 @@ hipError_t launch_rowpass(int n, int tile, int ept, bool use_t, bool emit_b, int
 }
 
 bool resident_supported(int n, int tile, int ept) {
-  return n >= 256 && n <= 1024 && (n & (n - 1)) == 0 && tile == 4 && ept == 8;
+  return n >= 256 && n <= 1024 && (n & (n - 1)) == 0 && ept == 8 &&
+         (tile == 4 || (tile == 2 && n <= 512));
 }
 
 hipError_t launch_resident(int n, int tile, int ept, int xm, const PassArgs& a,

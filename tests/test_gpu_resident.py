@@ -28,9 +28,9 @@ def step(alg, beta=0.8, sem="numpy", sigma=0.0, tau=0.0):
     return (_capi.ALG_CODES[alg], _capi.SEM_CODES[sem], beta, sigma, tau)
 
 
-def run(ctx, frame, guess, calls, resident, exchange=0, trace=True):
+def run(ctx, frame, guess, calls, resident, exchange=0, trace=True, tile=0):
     n = frame.modulus.shape[0]
-    p = Plan(ctx, n, resident=1 if resident else 2, exchange=exchange)
+    p = Plan(ctx, n, resident=1 if resident else 2, exchange=exchange, tile=tile)
     p.set_modulus(torch.from_numpy(frame.modulus).to(ctx.device))
     p.set_support(torch.from_numpy(frame.support).to(ctx.device))
     p.set_psi(torch.from_numpy(guess).to(ctx.device))
@@ -79,6 +79,21 @@ def test_resident_equals_per_pass(gpu_ctx, n, exchange):
     launches, steps = got[3]
     assert launches >= 3 and steps >= 20       # the blocks ran resident
     assert ref[3] == (0, 0)
+
+
+@pytest.mark.parametrize("n", [256, 512])
+def test_resident_in_tile_2_equals_per_pass(gpu_ctx, n):
+    """The resident launch is also built in tile 2 for 256^2 and 512^2 frames
+    (twice the workgroups, half the lines each): the same bitwise agreement
+    with the per-pass launches of a tile-2 plan, fused shrinkwraps included."""
+    print(f"[resident tile 2 n={n}]", flush=True)
+    f, g = data(n, 12)
+    sw = [step("hio", sigma=2.0, tau=0.2) if k % 5 == 4 else step("hio") for k in range(20)]
+    calls = [MIXED, sw + [step("er")] * 3]
+    ref = run(gpu_ctx, f, g, calls, resident=False, tile=2)
+    got = run(gpu_ctx, f, g, calls, resident=True, tile=2)
+    same(got, ref)
+    assert got[3][0] >= 3 and ref[3] == (0, 0)
 
 
 def test_resident_without_error_trace_and_single_step_calls(gpu_ctx):
