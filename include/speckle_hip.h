@@ -68,7 +68,8 @@ typedef struct spk_tune {
   int32_t sw_separate;/* 1: run each shrinkwrap as its own kernels (default:
                          fused into the passes of its step and the next
                          where the blur radius is <= 24 px, i.e. sigma <= 4) */
-  int32_t resident;   /* one frame of 256^2..1024^2 in tile 4, 8 values per
+  int32_t resident;   /* one frame of 256^2..1024^2 in tile 4 (256^2 and
+                         512^2 also in tile 2), 8 values per
                          thread: runs of non-DM steps run as one cooperative
                          resident launch each (the frame's modulus, support
                          and iterate held on chip, Phi exchanged through a
