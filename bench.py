@@ -16,6 +16,9 @@ Reported beside it in the same JSON line:
   roofline      dominant kernel of the headline workload: algorithmic bytes
                 per launch / average launch time (HIP events around each
                 launch on the library stream, one profiled reconstruction)
+  single_sizes  one 512^2 and one 256^2 frame with the headline schedule:
+                iterations/s of the resident launch (tile 4, the default,
+                and tile 2) and of one launch per pass
   batched       64 trials of one 512^2 frame (RAAR beta 0.87 and DM beta 0.7,
                 500 iterations each, aligned and averaged on the GPU), and 16
                 trials of the headline frame and schedule in one plan
@@ -613,6 +616,48 @@ def bench_batched_1024(ctx, frame, trials=16):
             "row_pass_hbm_frac": round(B_ROW * px / (row_us * 1e-6) / HBM_PEAK, 4)}
 
 
+def bench_single_sizes(ctx, reps=3):
+    """One 512^2 and one 256^2 frame (polygon 15 %, Poisson 1e7) with the
+    headline schedule: the resident launch in tile 4 (the default) and tile 2
+    against one launch per pass."""
+    import torch
+    from als_speckle_amd import synth
+    from als_speckle_amd.device import Plan
+    steps, _ = headline_steps(1000)
+    out = {}
+    for n in (512, 256):
+        f = synth.make_frame(n, 3, fill=0.15, photons=1e7)
+        np.random.seed(1)
+        g = torch.from_numpy((np.random.rand(n, n) + 1j * np.random.rand(n, n))
+                             .astype(np.complex64)).to(ctx.device)
+        m = torch.from_numpy(f.modulus).to(ctx.device)
+        sp = torch.from_numpy(f.support).to(ctx.device)
+        forms = {}
+        for name, kw in (("resident", {}), ("resident_tile_2", {"tile": 2}),
+                         ("per_pass", {"resident": 2})):
+            p = Plan(ctx, n, **kw)
+            p.set_modulus(m)
+            err = torch.empty((len(steps), 1), dtype=torch.float64, device=ctx.device)
+
+            def rec():
+                p.set_support(sp)
+                p.set_psi(g, mult_support=True)
+                p.iterate(steps, err_out=err)
+
+            rec()
+            ctx.sync()
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                rec()
+            ctx.sync()
+            dt = (time.perf_counter() - t0) / reps
+            p.check()
+            forms[name] = round(len(steps) / dt, 1)
+            p.close()
+        out[str(n)] = {"iterations_per_s": forms}
+    return out
+
+
 def bench_large_4096(ctx):
     """The large-frame workload: one 4096^2 reconstruction of 2000 iterations."""
     import torch
@@ -830,6 +875,8 @@ def main():
             "steps_per_launch": s["res"]["steps"], "shrinkwraps_per_launch": s["res"]["shrinkwraps"],
             "us_per_step": round(s["res"]["avg"] * 1e6 / s["res"]["steps"], 3)}
     out["hbm_copy_reference"] = copy_bandwidth(ctx)
+    log("single 512^2 and 256^2 frames")
+    out["single_sizes"] = bench_single_sizes(ctx)
     if not a.no_batched:
         log("batched trials 64 x 512^2 (RAAR, DM)")
         out["batched"] = {"trials_512": bench_trials_512(ctx)}
