@@ -405,8 +405,9 @@ class Group:
     """Frames of different sizes in one launch per pass (spk_group_iterate):
     ``iterate`` runs one spk_iterate call per plan with the plans'
     single-projection passes merged into one column-pass and one row-pass
-    launch per step.  Plans come from ``group_plan`` (or any plan in the
-    ``group_shape`` tile) on this group's context."""
+    launch per step.  Plans come from ``Group.plan`` (or are any plans in
+    the ``group_shape`` tile and values per thread) on this group's
+    context."""
 
     def __init__(self, ctx: Context):
         self.ctx, self.lib = ctx, ctx.lib
