@@ -43,10 +43,11 @@ bool is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
 //    register spills (tile 4 needs 16 values per thread there, which spill);
 //    at 2048 it is 13 % faster per iteration for one frame and 17 % for four;
 //  - tile 4 below 2048 also for one frame, although resident launches of a
-//    256^2 or 512^2 frame take 4.7 % / 2.4 % fewer microseconds per step in
-//    tile 2 (a tune choice): the FP64 error partials are summed per column
-//    band, so a frame's error trace is bitwise the same alone and among
-//    other frames only when the tile is;
+//    256^2 frame take 5 % fewer microseconds per step in tile 2 (a tune
+//    choice; at 512^2 the difference changes sign from run to run): the
+//    FP64 error partials are summed per column band, so a frame's error
+//    trace is bitwise the same alone and among other frames only when the
+//    tile is;
 //  - 8 values per thread (more threads, shorter dependency chains) wherever
 //    the workgroup fits, except: 1024^2 with 4 or more frames, where 16
 //    values per thread (radix-16 stages) are 9 % faster per batched
